@@ -1,0 +1,155 @@
+/*
+ * polyspectrum_gpu.h -- C ABI of the MI355X (gfx950) polyspectrum estimator.
+ *
+ * The library replaces the estimator layer of annehutter/polyspectrum
+ * (src/polyspectrum.[ch], src/filter.[ch], src/fft.[ch], src/utils_fftw.[ch])
+ * behind plain C entry points: pointers, sizes and int status codes, no
+ * language runtime types.  A field is uploaded once, transformed on the GPU
+ * and kept resident; estimates are returned as doubles.
+ *
+ * Two forms are exported:
+ *   1. a context API (ps_*) that keeps the spectrum in HBM and evaluates
+ *      single configurations (the reference's per-triangle shape) or whole
+ *      batches of configurations (one captured hipGraph);
+ *   2. drop-in symbols with the reference's exact signatures
+ *      (polyspectrum.h:7-8), so an unmodified run.c can link against them.
+ *
+ * Conventions are the reference's: N^3 grids in C order (grid.c:168), FFTW
+ * sign conventions and no normalisation (fft.c:19-57), the normalisation
+ * V^(n-1) N^(-3n) sum(prod D) / sum(prod I) (polyspectrum.c:155) and
+ * Npolygons = sum(prod I) / N^3 (polyspectrum.c:247).
+ *
+ * Threading: a context is used by one host thread at a time.
+ */
+#ifndef POLYSPECTRUM_GPU_H
+#define POLYSPECTRUM_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------- */
+#define PS_OK 0
+#define PS_ERR_INVALID (-1)        /* bad argument (null pointer, n < 2, ...) */
+#define PS_ERR_UNSUPPORTED_N (-2)  /* grid size without a compiled FFT plan */
+#define PS_ERR_HIP (-3)            /* a HIP runtime call failed */
+#define PS_ERR_NOMEM (-4)          /* device or host allocation failed */
+#define PS_ERR_STATE (-5)          /* no field loaded, or wrong slot */
+#define PS_ERR_NO_DEVICE (-6)      /* no usable GPU */
+
+/* ---- precision of a context ------------------------------------------ */
+#define PS_PREC_FP64 1  /* FP64 FFTs and products: the plain parity path */
+#define PS_PREC_MIXED 0 /* FP32 FFTs, FP64 compensated reductions and normalisation */
+
+/* ---- kbinningCase values (confObj.c:98-105) ---------------------------- */
+#define PS_BINNING_DEFAULT 0
+#define PS_BINNING_DERIVED 1
+#define PS_BINNING_GIVEN 2
+#define PS_BINNING_WATKINSON 3
+
+/* fftw_complex without <complex.h> (FFTW's own fallback typedef) */
+typedef double ps_fftw_complex[2];
+
+typedef struct ps_ctx ps_ctx;
+typedef struct ps_batch ps_batch;
+
+/* One polygon configuration as polyspectrum() receives it: n legs with
+ * k[i], kmin[i], kmax[i] (h/Mpc).  kmin/kmax are read for the last leg in
+ * GIVEN/WATKINSON binning (run.c:59-83) and written for DERIVED binning
+ * (filter.c:112-113, :317-318). */
+typedef struct ps_polygon {
+    int n;      /* 2 (power spectrum) or 3 (bispectrum) */
+    int slot;   /* field slot of the context */
+    double k[3];
+    double kmin[3];
+    double kmax[3];
+} ps_polygon;
+
+/* Human-readable text of a status code. */
+const char *ps_strerror(int status);
+
+/* Number of GPUs visible to HIP (0 when none). */
+int ps_device_count(void);
+
+/* Whether an N^3 grid has a compiled FFT plan (N = 2^a 3^b, 16..512). */
+int ps_grid_supported(int nbins);
+
+/* Create a context for N^3 grids of side boxsize (h^-1 Mpc) on the current
+ * device.  nslots fields can be resident at once (realisations).
+ * Replaces: initGrid_with_values grid.c:47-84, allocate_3D_array_fftw_complex
+ * utils_fftw.c:18-30. */
+int ps_ctx_create(int nbins, double boxsize, int precision, int nslots, ps_ctx **out);
+int ps_ctx_destroy(ps_ctx *ctx);
+
+/* Upload a real N^3 field (float32 if field_is_double == 0, float64
+ * otherwise; host memory unless on_device) into a slot and forward transform
+ * it; the spectrum stays resident.
+ * Replaces: get_FT_field input_field.c:40-81 (read_grid grid.c:121-228 +
+ * fft_real_to_kspace fft.c:19-37). */
+int ps_load_field(ps_ctx *ctx, int slot, const void *field, int field_is_double, int on_device);
+
+/* Use only the phases z/|z| of the spectrum (useOnlyPhase=1).
+ * Replaces: calc_phase_3D_fftw_array utils_fftw.c:78-93 (run.c:48-49); the
+ * normalisation is applied to the cells a shell selects, when they are read. */
+int ps_set_phase_only(ps_ctx *ctx, int phase_only);
+
+/* One configuration, shaped like the reference: per leg a mask, an inverse
+ * FFT of the masked spectrum and of the mask, products into N^3
+ * accumulators, then the sums.  result = polyspectrum value, sums (optional,
+ * 2 doubles) = (Re sum prod D, Re sum prod I).
+ * Replaces: polyspectrum polyspectrum.c:51-161. */
+int ps_polyspectrum(ps_ctx *ctx, int slot, int n, const double *k, double *kmin, double *kmax,
+                    double kbinwidth, int kbinning_case, double *result, double *sums);
+
+/* Replaces: num_polygons polyspectrum.c:163-252. */
+int ps_num_polygons(ps_ctx *ctx, int n, const double *k, double *kmin, double *kmax, double kbinwidth,
+                    int kbinning_case, double *result);
+
+/* Batched evaluation of many configurations: shells shared by several
+ * polygons are filtered and transformed once, only the k-cells a shell can
+ * touch enter the first inverse passes, and the last inverse pass computes
+ * the products and FP64 compensated sums of all polygons of a group without
+ * writing the filtered fields.  The launch sequence is captured once in a
+ * hipGraph and replayed by ps_batch_execute.
+ * values[i]: polyspectrum value; npolygons[i] (optional): Npolygons;
+ * sums (optional, 2*npoly): (sum prod D, sum prod I).
+ * For DERIVED binning the planner writes the derived kmin/kmax back into
+ * polys (as polyspectrum() does).
+ * Replaces: the loop run.c:56-107 over polyspectrum()/num_polygons(). */
+int ps_batch_create(ps_ctx *ctx, ps_polygon *polys, int npoly, double kbinwidth, int kbinning_case,
+                    ps_batch **out);
+int ps_batch_execute(ps_batch *b, double *values, double *npolygons, double *sums);
+/* Time of the last ps_batch_execute on the GPU (ms), from HIP events. */
+double ps_batch_last_ms(const ps_batch *b);
+/* HBM bytes the batch touches per execution by the algorithmic model
+ * (reads + writes of every pass), for roofline reporting. */
+double ps_batch_model_bytes(const ps_batch *b);
+int ps_batch_destroy(ps_batch *b);
+
+/* Re-run the forward transform of a slot from the last host upload (timing
+ * of the FFT passes alone). */
+int ps_forward_again(ps_ctx *ctx, int slot);
+
+/* Copy the resident half spectrum of a slot to host memory: (N/2+1)*N*N
+ * complex values (complex128 for FP64 contexts, complex64 for mixed), laid
+ * out [kz][kx][ky] with kz in [0, N/2]; element (kz, kx, ky) equals the
+ * reference's fft_array[(kx*N + ky)*N + kz] (input_field.c:77). */
+int ps_get_spectrum(ps_ctx *ctx, int slot, void *out);
+
+/* ---- drop-in symbols (reference signatures) ---------------------------- */
+/* polyspectrum.h:7.  fft_array is the full complex N^3 spectrum of a real
+ * field (as get_FT_field returns it); local_n0 must equal nbins and
+ * local_n0_start 0 (serial build).  Runs the FP64 path; returns NaN on error. */
+double polyspectrum(int nbins, int local_n0, int local_n0_start, ps_fftw_complex *fft_array, int n, double *k,
+                    double *kmin, double *kmax, double kbinwidth, int kbinningCase, double boxsize);
+/* polyspectrum.h:8 */
+double num_polygons(int nbins, int local_n0, int local_n0_start, int n, double *k, double *kmin, double *kmax,
+                    double kbinwidth, int kbinningCase, double boxsize);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

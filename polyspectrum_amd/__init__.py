@@ -1,0 +1,281 @@
+"""MI355X (gfx950) polyspectrum estimator -- Python mirror of the C ABI.
+
+The product is the shared library ``libpolyspectrum_gpu.so`` built in-tree
+from ``csrc/`` (C ABI declared in ``include/polyspectrum_gpu.h``).  This module
+binds it with ctypes and mirrors the reference's estimator interface
+(src/polyspectrum.h:7-8, src/run.c): same function names, argument meaning
+and normalisation.  There is no CPU fallback: without the built library, or
+without a GPU, every entry point raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+__all__ = [
+    "PREC_FP64", "PREC_MIXED", "BINNING", "Polygon", "Context", "Batch", "GPUError",
+    "polyspectrum", "num_polygons", "library_path", "load_library", "device_count", "build",
+]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_NAME = "libpolyspectrum_gpu.so"
+
+PREC_MIXED = 0
+PREC_FP64 = 1
+# kbinningCase strings of confObj.c:98-105
+BINNING = {"DEFAULT": 0, "DERIVED_BINNING": 1, "GIVEN_BINNING": 2, "WATKINSON": 3}
+
+_lib = None
+
+
+class GPUError(RuntimeError):
+    pass
+
+
+class Polygon(ctypes.Structure):
+    """ps_polygon: n legs with k, kmin, kmax (h/Mpc) and the field slot."""
+    _fields_ = [("n", ctypes.c_int), ("slot", ctypes.c_int),
+                ("k", ctypes.c_double * 3), ("kmin", ctypes.c_double * 3),
+                ("kmax", ctypes.c_double * 3)]
+
+
+def library_path():
+    return os.path.join(_HERE, _LIB_NAME)
+
+
+def build():
+    """Compile the HIP library for gfx950 (hipcc cross-compiles without a GPU)."""
+    import subprocess
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_vp = ctypes.c_void_p
+
+
+def load_library():
+    """Load the built library; raises if it is missing (no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = library_path()
+    if not os.path.exists(path):
+        raise GPUError(f"{path} is not built; run polyspectrum_amd.build() (make -C polyspectrum_amd)")
+    L = ctypes.CDLL(path)
+    L.ps_strerror.restype = ctypes.c_char_p
+    L.ps_strerror.argtypes = [ctypes.c_int]
+    L.ps_device_count.restype = ctypes.c_int
+    L.ps_grid_supported.argtypes = [ctypes.c_int]
+    L.ps_ctx_create.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]
+    L.ps_ctx_destroy.argtypes = [_vp]
+    L.ps_load_field.argtypes = [_vp, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int]
+    L.ps_set_phase_only.argtypes = [_vp, ctypes.c_int]
+    L.ps_polyspectrum.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, ctypes.c_double,
+                                  ctypes.c_int, _dp, _dp]
+    L.ps_num_polygons.argtypes = [_vp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_double, ctypes.c_int, _dp]
+    L.ps_batch_create.argtypes = [_vp, ctypes.POINTER(Polygon), ctypes.c_int, ctypes.c_double,
+                                  ctypes.c_int, ctypes.POINTER(_vp)]
+    L.ps_batch_execute.argtypes = [_vp, _dp, _dp, _dp]
+    L.ps_batch_last_ms.restype = ctypes.c_double
+    L.ps_batch_last_ms.argtypes = [_vp]
+    L.ps_batch_model_bytes.restype = ctypes.c_double
+    L.ps_batch_model_bytes.argtypes = [_vp]
+    L.ps_batch_destroy.argtypes = [_vp]
+    L.ps_forward_again.argtypes = [_vp, ctypes.c_int]
+    L.ps_get_spectrum.argtypes = [_vp, ctypes.c_int, _vp]
+    L.polyspectrum.restype = ctypes.c_double
+    L.polyspectrum.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int, _dp, _dp, _dp,
+                               ctypes.c_double, ctypes.c_int, ctypes.c_double]
+    L.num_polygons.restype = ctypes.c_double
+    L.num_polygons.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp,
+                               ctypes.c_double, ctypes.c_int, ctypes.c_double]
+    _lib = L
+    return L
+
+
+def _check(status):
+    if status != 0:
+        msg = load_library().ps_strerror(status).decode()
+        raise GPUError(f"polyspectrum_gpu status {status}: {msg}")
+
+
+def device_count():
+    return load_library().ps_device_count()
+
+
+def _darr(a, n=3):
+    a = np.zeros(n) if a is None else np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(_dp)
+
+
+class Context:
+    """A field (or several realisation slots) resident on the GPU.
+
+    precision: PREC_FP64 (plain parity path) or PREC_MIXED (FP32 FFTs, FP64
+    compensated reductions and normalisation)."""
+
+    def __init__(self, nbins, boxsize, precision=PREC_FP64, nslots=1):
+        L = load_library()
+        h = _vp()
+        _check(L.ps_ctx_create(int(nbins), float(boxsize), int(precision), int(nslots), ctypes.byref(h)))
+        self._h = h
+        self.nbins = int(nbins)
+        self.boxsize = float(boxsize)
+        self.precision = precision
+        self.nslots = nslots
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load_library().ps_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def load_field(self, field, slot=0):
+        """get_FT_field (input_field.c:40-81): real N^3 float32/float64 field
+        (numpy array, or a torch CUDA tensor) -> resident spectrum."""
+        try:
+            import torch
+            if isinstance(field, torch.Tensor):
+                if not field.is_cuda:
+                    field = field.numpy()
+                else:
+                    t = field.contiguous()
+                    assert t.numel() == self.nbins ** 3 and t.dtype in (torch.float32, torch.float64)
+                    _check(load_library().ps_load_field(self._h, slot, ctypes.c_void_p(t.data_ptr()),
+                                                        int(t.dtype == torch.float64), 1))
+                    return
+        except ImportError:
+            pass
+        a = np.ascontiguousarray(field)
+        if a.dtype not in (np.float32, np.float64):
+            a = a.astype(np.float64)
+        assert a.size == self.nbins ** 3
+        _check(load_library().ps_load_field(self._h, slot, a.ctypes.data_as(_vp), int(a.dtype == np.float64), 0))
+
+    def spectrum(self, slot=0):
+        """The resident half spectrum as a numpy array [kz][kx][ky] (kz <= N/2)."""
+        N = self.nbins
+        dt = np.complex128 if self.precision == PREC_FP64 else np.complex64
+        out = np.empty((N // 2 + 1, N, N), dtype=dt)
+        _check(load_library().ps_get_spectrum(self._h, slot, out.ctypes.data_as(_vp)))
+        return out
+
+    def forward_again(self, slot=0):
+        _check(load_library().ps_forward_again(self._h, slot))
+
+    def set_phase_only(self, flag):
+        _check(load_library().ps_set_phase_only(self._h, int(flag)))
+
+    def polyspectrum(self, n, k, kmin=None, kmax=None, kbinwidth=1.0, kbinning_case=0, slot=0):
+        """polyspectrum.c:51-161 on the GPU.  Returns (value, kmin, kmax,
+        (sum prod D, sum prod I)); kmin/kmax as updated for DERIVED binning."""
+        k, kp = _darr(k)
+        lo, lop = _darr(kmin)
+        lo = lo.copy(); lop = lo.ctypes.data_as(_dp)
+        hi, hip = _darr(kmax)
+        hi = hi.copy(); hip = hi.ctypes.data_as(_dp)
+        res = ctypes.c_double()
+        sums = np.zeros(2)
+        _check(load_library().ps_polyspectrum(self._h, slot, int(n), kp, lop, hip, float(kbinwidth),
+                                              int(kbinning_case), ctypes.byref(res), sums.ctypes.data_as(_dp)))
+        return res.value, lo, hi, (sums[0], sums[1])
+
+    def num_polygons(self, n, k, kmin=None, kmax=None, kbinwidth=1.0, kbinning_case=0):
+        """polyspectrum.c:163-252 on the GPU."""
+        k, kp = _darr(k)
+        lo = _darr(kmin)[0].copy()
+        hi = _darr(kmax)[0].copy()
+        res = ctypes.c_double()
+        _check(load_library().ps_num_polygons(self._h, int(n), kp, lo.ctypes.data_as(_dp), hi.ctypes.data_as(_dp),
+                                              float(kbinwidth), int(kbinning_case), ctypes.byref(res)))
+        return res.value
+
+    def batch(self, polygons, kbinwidth=1.0, kbinning_case=0):
+        return Batch(self, polygons, kbinwidth, kbinning_case)
+
+
+class Batch:
+    """Many configurations evaluated by one captured launch sequence."""
+
+    def __init__(self, ctx, polygons, kbinwidth=1.0, kbinning_case=0):
+        arr = (Polygon * len(polygons))()
+        for i, p in enumerate(polygons):
+            if isinstance(p, Polygon):
+                arr[i] = p
+                continue
+            n = int(p["n"])
+            arr[i].n = n
+            arr[i].slot = int(p.get("slot", 0))
+            for j in range(3):
+                arr[i].k[j] = float(p["k"][j]) if j < len(p["k"]) else 0.0
+                arr[i].kmin[j] = float(p.get("kmin", [0, 0, 0])[j]) if j < len(p.get("kmin", [])) else 0.0
+                arr[i].kmax[j] = float(p.get("kmax", [0, 0, 0])[j]) if j < len(p.get("kmax", [])) else 0.0
+        h = _vp()
+        _check(load_library().ps_batch_create(ctx._h, arr, len(polygons), float(kbinwidth), int(kbinning_case),
+                                              ctypes.byref(h)))
+        self._h = h
+        self._ctx = ctx  # keeps the context alive
+        self.polygons = arr
+        self.n = len(polygons)
+
+    def execute(self):
+        """Returns (values, npolygons, sums[n,2])."""
+        v = np.zeros(self.n)
+        npol = np.zeros(self.n)
+        s = np.zeros(2 * self.n)
+        _check(load_library().ps_batch_execute(self._h, v.ctypes.data_as(_dp), npol.ctypes.data_as(_dp),
+                                               s.ctypes.data_as(_dp)))
+        return v, npol, s.reshape(self.n, 2)
+
+    @property
+    def last_ms(self):
+        return load_library().ps_batch_last_ms(self._h)
+
+    @property
+    def model_bytes(self):
+        return load_library().ps_batch_model_bytes(self._h)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load_library().ps_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def polyspectrum(nbins, local_n0, local_n0_start, fft_array, n, k, kmin, kmax, kbinwidth, kbinningCase, boxsize):
+    """Drop-in for polyspectrum() (polyspectrum.h:7): fft_array is the full
+    complex128 N^3 spectrum; kmin/kmax (numpy float64) are updated in place."""
+    a = np.ascontiguousarray(fft_array, dtype=np.complex128)
+    k = np.ascontiguousarray(k, dtype=np.float64)
+    assert isinstance(kmin, np.ndarray) and kmin.dtype == np.float64 and kmin.flags.c_contiguous
+    assert isinstance(kmax, np.ndarray) and kmax.dtype == np.float64 and kmax.flags.c_contiguous
+    v = load_library().polyspectrum(nbins, local_n0, local_n0_start, a.ctypes.data_as(_vp), n,
+                                    k.ctypes.data_as(_dp), kmin.ctypes.data_as(_dp), kmax.ctypes.data_as(_dp),
+                                    kbinwidth, kbinningCase, boxsize)
+    return v
+
+
+def num_polygons(nbins, local_n0, local_n0_start, n, k, kmin, kmax, kbinwidth, kbinningCase, boxsize):
+    """Drop-in for num_polygons() (polyspectrum.h:8)."""
+    k = np.ascontiguousarray(k, dtype=np.float64)
+    assert isinstance(kmin, np.ndarray) and kmin.dtype == np.float64
+    assert isinstance(kmax, np.ndarray) and kmax.dtype == np.float64
+    return load_library().num_polygons(nbins, local_n0, local_n0_start, n, k.ctypes.data_as(_dp),
+                                       kmin.ctypes.data_as(_dp), kmax.ctypes.data_as(_dp), kbinwidth,
+                                       kbinningCase, boxsize)

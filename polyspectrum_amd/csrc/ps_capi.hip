@@ -1,0 +1,762 @@
+// ps_capi.hip -- host orchestration behind the C ABI (include/polyspectrum_gpu.h).
+//
+// Two execution shapes:
+//  * ps_polyspectrum / ps_num_polygons: the reference's per-configuration
+//    shape (polyspectrum.c:51-252): for each leg a full-grid mask and inverse
+//    transform of the masked spectrum and of the mask, products into N^3 FP64
+//    accumulators, then FP64 sums.  Used in FP64 as the parity path.
+//  * ps_batch_*: every configuration of a run at once.  Shells are
+//    de-duplicated into legs, the first inverse passes see only the planes
+//    and band a shell can touch, and the last pass multiplies the legs of
+//    every polygon of a group and reduces in FP64 without writing fields.
+//    The launch sequence is captured in a hipGraph.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <tuple>
+#include <vector>
+
+#include "ps_internal.h"
+
+using namespace ps;
+
+// ---------------------------------------------------------------- errors
+#define HIPCK(call)                                                                          \
+    do {                                                                                     \
+        hipError_t e_ = (call);                                                              \
+        if (e_ != hipSuccess) {                                                              \
+            fprintf(stderr, "polyspectrum_gpu: %s failed: %s (%s:%d)\n", #call,              \
+                    hipGetErrorString(e_), __FILE__, __LINE__);                              \
+            return (e_ == hipErrorOutOfMemory) ? PS_ERR_NOMEM : PS_ERR_HIP;                  \
+        }                                                                                    \
+    } while (0)
+#define PSCK(call)                       \
+    do {                                 \
+        int s_ = (call);                 \
+        if (s_ != PS_OK) {               \
+            if (s_ > 0) {                \
+                fprintf(stderr, "polyspectrum_gpu: launch failed: %s (%s:%d)\n", \
+                        hipGetErrorString((hipError_t)s_), __FILE__, __LINE__);  \
+                return PS_ERR_HIP;       \
+            }                            \
+            return s_;                   \
+        }                                \
+    } while (0)
+
+static double ref_pi() { return acos(-1.0); }  // the reference's PI (filter.c:19)
+
+// ---------------------------------------------------------------- context
+struct ps_ctx {
+    int N = 0;
+    double L = 0.0;
+    int fp64 = 1;
+    int nslots = 1;
+    int device = 0;
+    int cus = 256;
+    hipStream_t st = nullptr;
+    void *tw = nullptr;                  // forward twiddles exp(-2 pi i t/N), N entries
+    std::vector<void *> spec;            // half spectra per slot
+    std::vector<int> loaded;
+    void *staging = nullptr;             // field upload buffer (N^3 doubles)
+    int staging_is_double = 0;
+    int phase_only = 0;
+    // reference-shaped path buffers (lazily allocated)
+    void *gd = nullptr, *gi = nullptr;   // half-spectrum sized work buffers
+    double *accD = nullptr, *accI = nullptr;
+    double2 *partials = nullptr;
+    double *dsum = nullptr;              // 2 doubles
+    LegDev *dleg = nullptr;
+    double2 *dcorner = nullptr;          // corner-extreme partials
+    size_t csize() const { return fp64 ? 16 : 8; }
+    int64_t half_elems() const { return (int64_t)(N / 2 + 1) * N * N; }
+    int64_t cells() const { return (int64_t)N * N * N; }
+};
+
+extern "C" const char *ps_strerror(int s)
+{
+    switch (s) {
+    case PS_OK: return "ok";
+    case PS_ERR_INVALID: return "invalid argument";
+    case PS_ERR_UNSUPPORTED_N: return "unsupported grid size (N must be 2^a*3^b in 16..512)";
+    case PS_ERR_HIP: return "HIP runtime error";
+    case PS_ERR_NOMEM: return "out of memory";
+    case PS_ERR_STATE: return "no field loaded in this slot";
+    case PS_ERR_NO_DEVICE: return "no GPU available";
+    }
+    return "unknown status";
+}
+
+extern "C" int ps_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+extern "C" int ps_grid_supported(int nbins) { return supported_grid(nbins); }
+
+template <typename T> static int make_twiddles(int N, void **out)
+{
+    std::vector<T> h(2 * (size_t)N);
+    const long double two_pi = 6.283185307179586476925286766559005768L;
+    for (int t = 0; t < N; ++t) {
+        long double a = two_pi * (long double)t / (long double)N;
+        h[2 * t] = (T)cosl(a);
+        h[2 * t + 1] = (T)(-sinl(a));
+    }
+    HIPCK(hipMalloc(out, sizeof(T) * 2 * (size_t)N));
+    HIPCK(hipMemcpy(*out, h.data(), sizeof(T) * 2 * (size_t)N, hipMemcpyHostToDevice));
+    return PS_OK;
+}
+
+extern "C" int ps_ctx_destroy(ps_ctx *c)
+{
+    if (!c) return PS_ERR_INVALID;
+    (void)hipSetDevice(c->device);
+    if (c->st) (void)hipStreamSynchronize(c->st);
+    for (void *p : c->spec) if (p) (void)hipFree(p);
+    void *bufs[] = {c->tw, c->staging, c->gd, c->gi, c->accD, c->accI, c->partials, c->dsum, c->dleg, c->dcorner};
+    for (void *p : bufs) if (p) (void)hipFree(p);
+    if (c->st) (void)hipStreamDestroy(c->st);
+    delete c;
+    return PS_OK;
+}
+
+extern "C" int ps_ctx_create(int nbins, double boxsize, int precision, int nslots, ps_ctx **out)
+{
+    if (!out || nslots < 1 || !(boxsize > 0.0)) return PS_ERR_INVALID;
+    *out = nullptr;
+    if (!supported_grid(nbins)) return PS_ERR_UNSUPPORTED_N;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return PS_ERR_NO_DEVICE;
+    ps_ctx *c = new ps_ctx();
+    c->N = nbins;
+    c->L = boxsize;
+    c->fp64 = (precision == PS_PREC_FP64) ? 1 : 0;
+    c->nslots = nslots;
+    c->spec.assign(nslots, nullptr);
+    c->loaded.assign(nslots, 0);
+    int st = PS_OK;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&c->device) != hipSuccess || hipGetDeviceProperties(&prop, c->device) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
+        ps_ctx_destroy(c);
+        return PS_ERR_HIP;
+    }
+    c->cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+    st = c->fp64 ? make_twiddles<double>(nbins, &c->tw) : make_twiddles<float>(nbins, &c->tw);
+    if (st != PS_OK) {
+        ps_ctx_destroy(c);
+        return st;
+    }
+    *out = c;
+    return PS_OK;
+}
+
+static int ensure_spec(ps_ctx *c, int slot)
+{
+    if (!c->spec[slot]) HIPCK(hipMalloc(&c->spec[slot], c->csize() * (size_t)c->half_elems()));
+    return PS_OK;
+}
+
+static int forward(ps_ctx *c, int slot, const void *dfield, int is_double)
+{
+    PSCK(ensure_spec(c, slot));
+    PSCK(run_forward(c->fp64, c->N, dfield, is_double, c->spec[slot], c->tw, c->st));
+    return PS_OK;
+}
+
+extern "C" int ps_load_field(ps_ctx *c, int slot, const void *field, int field_is_double, int on_device)
+{
+    if (!c || !field || slot < 0 || slot >= c->nslots) return PS_ERR_INVALID;
+    HIPCK(hipSetDevice(c->device));
+    const void *src = field;
+    if (!on_device) {
+        if (!c->staging) HIPCK(hipMalloc(&c->staging, sizeof(double) * (size_t)c->cells()));
+        const size_t bytes = (field_is_double ? 8 : 4) * (size_t)c->cells();
+        HIPCK(hipMemcpyAsync(c->staging, field, bytes, hipMemcpyHostToDevice, c->st));
+        src = c->staging;
+        c->staging_is_double = field_is_double ? 1 : 0;
+    }
+    PSCK(forward(c, slot, src, field_is_double));
+    HIPCK(hipStreamSynchronize(c->st));
+    c->loaded[slot] = 1;
+    return PS_OK;
+}
+
+extern "C" int ps_forward_again(ps_ctx *c, int slot)
+{
+    if (!c || slot < 0 || slot >= c->nslots) return PS_ERR_INVALID;
+    if (!c->staging || !c->loaded[slot]) return PS_ERR_STATE;
+    return forward(c, slot, c->staging, c->staging_is_double);
+}
+
+extern "C" int ps_get_spectrum(ps_ctx *c, int slot, void *out)
+{
+    if (!c || !out || slot < 0 || slot >= c->nslots) return PS_ERR_INVALID;
+    if (!c->loaded[slot]) return PS_ERR_STATE;
+    HIPCK(hipSetDevice(c->device));
+    HIPCK(hipMemcpyAsync(out, c->spec[slot], c->csize() * (size_t)c->half_elems(), hipMemcpyDeviceToHost, c->st));
+    HIPCK(hipStreamSynchronize(c->st));
+    return PS_OK;
+}
+
+extern "C" int ps_set_phase_only(ps_ctx *c, int phase_only)
+{
+    if (!c) return PS_ERR_INVALID;
+    c->phase_only = (phase_only == 1) ? 1 : 0;  // run.c:48: only the value 1 switches it on
+    return PS_OK;
+}
+
+// ---------------------------------------------------------------- masks (host side)
+static MaskParams mp_shell(int kind, double k, double bw, double L)
+{
+    MaskParams p{};
+    p.kind = kind;
+    p.kkf = k / (2. * ref_pi()) * L;  // filter.c:36
+    p.bw = bw;
+    return p;
+}
+static MaskParams mp_range(int kind, double kmin, double kmax, double bw, double L)
+{
+    MaskParams p{};
+    p.kind = kind;
+    p.mmin = kmin / (2. * ref_pi()) * L;  // filter.c:137-138
+    p.mmax = kmax / (2. * ref_pi()) * L;
+    p.bw = bw;
+    return p;
+}
+
+// largest |m_i| a cell selected by p can have (+1 margin), capped at N/2
+static int shell_band(const MaskParams &p, int N)
+{
+    double r;
+    switch (p.kind) {
+    case MASK_RANGE: r = p.mmax; break;
+    case MASK_WATKINSON_KN: r = std::max(p.mmin, p.mmax); break;
+    default: r = p.kkf;
+    }
+    r += 0.5 * fabs(p.bw);
+    if (!(r < (double)N)) return N / 2;
+    if (r < 0.0) return 0;
+    int K = (int)floor(r) + 1;
+    return std::min(K, N / 2);
+}
+
+// DERIVED binning: kmin/kmax of a shell from its cells' cube corners
+// (construct_kfilter_with_determining_kbinwidth filter.c:78-114)
+static int derived_kminmax(ps_ctx *c, double k, double bw, double *kmin, double *kmax)
+{
+    MaskParams mp = mp_shell(MASK_DETERMINING, k, bw, c->L);
+    const int K = shell_band(mp, c->N);
+    const int nblocks = 256;
+    if (!c->dcorner) HIPCK(hipMalloc(&c->dcorner, sizeof(double2) * nblocks));
+    PSCK(run_corner_extremes(mp, c->N, K, bw, c->dcorner, nblocks, c->st));
+    std::vector<double2> h(nblocks);
+    HIPCK(hipMemcpyAsync(h.data(), c->dcorner, sizeof(double2) * nblocks, hipMemcpyDeviceToHost, c->st));
+    HIPCK(hipStreamSynchronize(c->st));
+    double lo = 1.e30 * 1.e30, hi = 0.0;
+    for (auto &v : h) {
+        lo = std::min(lo, v.x);
+        hi = std::max(hi, v.y);
+    }
+    double mmin = 1.e30, mmax = 0.;  // filter.c:85-86 (unchanged when no cell is selected)
+    if (hi > 0.0) {
+        mmin = sqrt(lo);
+        mmax = sqrt(hi);
+    }
+    *kmin = mmin * 2. * ref_pi() / c->L;  // filter.c:112-113
+    *kmax = mmax * 2. * ref_pi() / c->L;
+    return PS_OK;
+}
+
+// filter.c:291-319
+static void k3_min_and_max(const double *k, double *kmin, double *kmax)
+{
+    double ct = (k[0] * k[0] + k[1] * k[1] - k[2] * k[2]) / (2. * k[0] * k[1]);
+    double t[8];
+    t[0] = kmin[0] * kmin[0] + kmin[1] * kmin[1] - 2. * kmin[0] * kmin[1] * ct;
+    t[1] = kmin[0] * kmin[0] + kmax[1] * kmax[1] - 2. * kmin[0] * kmax[1] * ct;
+    t[2] = kmax[0] * kmax[0] + kmin[1] * kmin[1] - 2. * kmax[0] * kmin[1] * ct;
+    t[3] = kmax[0] * kmax[0] + kmax[1] * kmax[1] - 2. * kmax[0] * kmax[1] * ct;
+    t[4] = k[0] * k[0] + kmax[1] * kmax[1] - 2. * k[0] * kmax[1] * ct;
+    t[5] = kmax[0] * kmax[0] + k[1] * k[1] - 2. * kmax[0] * k[1] * ct;
+    t[6] = k[0] * k[0] + kmin[1] * kmin[1] - 2. * k[0] * kmin[1] * ct;
+    t[7] = kmin[0] * kmin[0] + k[1] * k[1] - 2. * kmin[0] * k[1] * ct;
+    double lo = 1.e30, hi = 0.;
+    for (int i = 0; i < 8; i++) {
+        if (t[i] < lo) lo = t[i];
+        if (t[i] > hi) hi = t[i];
+    }
+    kmin[2] = sqrt(lo);
+    kmax[2] = sqrt(hi);
+}
+
+// the per-leg mask choice of polyspectrum.c:79-118; may update kmin/kmax
+static int leg_mask(ps_ctx *c, int i, int n, const double *k, double *kmin, double *kmax, double bw, int kcase,
+                    MaskParams *out)
+{
+    const double L = c->L;
+    switch (kcase) {
+    case PS_BINNING_DERIVED:
+        if (i < n - 1) {
+            PSCK(derived_kminmax(c, k[i], bw, &kmin[i], &kmax[i]));
+            *out = mp_shell(MASK_DETERMINING, k[i], bw, L);
+        } else {
+            if (n == 3) {
+                k3_min_and_max(k, kmin, kmax);
+            } else {
+                // filter.c:321-325 writes through a mis-indexed pointer; its
+                // intended effect is kmin[n-1] = kmax[n-1] = k[n-1]
+                kmin[n - 1] = k[n - 1];
+                kmax[n - 1] = k[n - 1];
+            }
+            *out = mp_range(MASK_RANGE, kmin[i], kmax[i], bw, L);
+        }
+        break;
+    case PS_BINNING_GIVEN:
+        *out = (i < n - 1) ? mp_shell(MASK_DEFAULT, k[i], bw, L) : mp_range(MASK_RANGE, kmin[i], kmax[i], bw, L);
+        break;
+    case PS_BINNING_WATKINSON:
+        *out = (i < n - 1) ? mp_shell(MASK_WATKINSON, k[i], bw, L)
+                           : mp_range(MASK_WATKINSON_KN, kmin[i], kmax[i], bw, L);
+        break;
+    default:
+        *out = mp_shell(MASK_DEFAULT, k[i], bw, L);
+    }
+    return PS_OK;
+}
+
+// ---------------------------------------------------------------- reference-shaped path
+static int ensure_plain_buffers(ps_ctx *c)
+{
+    const size_t hb = c->csize() * (size_t)c->half_elems();
+    if (!c->gd) HIPCK(hipMalloc(&c->gd, hb));
+    if (!c->gi) HIPCK(hipMalloc(&c->gi, hb));
+    if (!c->accD) HIPCK(hipMalloc(&c->accD, sizeof(double) * (size_t)c->cells()));
+    if (!c->accI) HIPCK(hipMalloc(&c->accI, sizeof(double) * (size_t)c->cells()));
+    const int64_t nb = (int64_t)c->N * c->N / tile_rows(c->N, c->fp64);
+    if (!c->partials) HIPCK(hipMalloc(&c->partials, sizeof(double2) * 2 * (size_t)nb));
+    if (!c->dsum) HIPCK(hipMalloc(&c->dsum, sizeof(double) * 2));
+    if (!c->dleg) HIPCK(hipMalloc(&c->dleg, sizeof(LegDev)));
+    return PS_OK;
+}
+
+// one leg of the reference-shaped path: full (unpruned) inverse transforms
+static int plain_leg(ps_ctx *c, const MaskParams &mp, const void *spec, int mode)
+{
+    const int N = c->N;
+    LegDev L{};
+    L.mask = mp;
+    L.spec = spec;
+    L.phase_only = c->phase_only;
+    L.Kz = N / 2;
+    L.Kx = N / 2;
+    L.Bx = N;
+    L.g1_begin = 0;
+    L.g2_begin = 0;
+    L.g1d = spec ? c->gd : nullptr;
+    L.g1i = c->gi;
+    L.g2d = L.g1d;
+    L.g2i = c->gi;
+    HIPCK(hipMemcpyAsync(c->dleg, &L, sizeof(LegDev), hipMemcpyHostToDevice, c->st));
+    const int tl = g_tile_lines(N, c->fp64);
+    const int64_t g1 = (int64_t)(N / 2 + 1) * ((N + tl - 1) / tl);
+    const int64_t g2 = (int64_t)(N / 2 + 1) * (N / tl);
+    PSCK(run_inverse_g(c->fp64, N, c->dleg, 1, g1, g2, 1 << 30, c->tw, c->st));
+    PSCK(run_zc_plain(c->fp64, N, L.g2d, L.g2i, N / 2, spec ? c->accD : nullptr, c->accI, mode, c->partials, c->tw,
+                      c->st));
+    // the next leg overwrites the host-side descriptor buffer: keep order
+    HIPCK(hipStreamSynchronize(c->st));
+    return PS_OK;
+}
+
+static int plain_run(ps_ctx *c, int slot, int n, const double *k, double *kmin, double *kmax, double bw, int kcase,
+                     double *sumD, double *sumI)
+{
+    PSCK(ensure_plain_buffers(c));
+    const void *spec = (slot >= 0) ? c->spec[slot] : nullptr;
+    // accumulators start at 1 (polyspectrum.c:66-70)
+    PSCK(run_fill(c->accI, c->cells(), 1.0, c->st));
+    if (spec) PSCK(run_fill(c->accD, c->cells(), 1.0, c->st));
+    for (int i = 0; i < n; ++i) {
+        MaskParams mp;
+        PSCK(leg_mask(c, i, n, k, kmin, kmax, bw, kcase, &mp));
+        PSCK(plain_leg(c, mp, spec, (i < n - 1) ? 1 : 2));
+    }
+    const int64_t nb = (int64_t)c->N * c->N / tile_rows(c->N, c->fp64);
+    PSCK(run_reduce_partials(c->partials, 0, nullptr, nb, 1, c->dsum, c->dsum + 1, c->st));
+    double h[2];
+    HIPCK(hipMemcpyAsync(h, c->dsum, sizeof(h), hipMemcpyDeviceToHost, c->st));
+    HIPCK(hipStreamSynchronize(c->st));
+    if (sumD) *sumD = h[0];
+    *sumI = h[1];
+    return PS_OK;
+}
+
+extern "C" int ps_polyspectrum(ps_ctx *c, int slot, int n, const double *k, double *kmin, double *kmax,
+                               double kbinwidth, int kbinning_case, double *result, double *sums)
+{
+    if (!c || !k || !kmin || !kmax || !result || n < 2 || n > 3) return PS_ERR_INVALID;
+    if (slot < 0 || slot >= c->nslots) return PS_ERR_INVALID;
+    if (!c->loaded[slot]) return PS_ERR_STATE;
+    HIPCK(hipSetDevice(c->device));
+    double sD = 0, sI = 0;
+    PSCK(plain_run(c, slot, n, k, kmin, kmax, kbinwidth, kbinning_case, &sD, &sI));
+    const double volume = c->L * c->L * c->L;
+    *result = pow(volume, n - 1) * pow(c->N, -n * 3) * sD / sI;  // polyspectrum.c:155
+    if (sums) {
+        sums[0] = sD;
+        sums[1] = sI;
+    }
+    return PS_OK;
+}
+
+extern "C" int ps_num_polygons(ps_ctx *c, int n, const double *k, double *kmin, double *kmax, double kbinwidth,
+                               int kbinning_case, double *result)
+{
+    if (!c || !k || !kmin || !kmax || !result || n < 2 || n > 3) return PS_ERR_INVALID;
+    HIPCK(hipSetDevice(c->device));
+    double sI = 0;
+    PSCK(plain_run(c, -1, n, k, kmin, kmax, kbinwidth, kbinning_case, nullptr, &sI));
+    *result = sI / (double)((int64_t)c->N * c->N * c->N);  // polyspectrum.c:247
+    return PS_OK;
+}
+
+// ---------------------------------------------------------------- batched path
+struct ps_batch {
+    ps_ctx *c = nullptr;
+    int npoly = 0;
+    std::vector<int> poly_n;
+    std::vector<int> poly_out;           // output slot of each polygon
+    int nout = 0;
+    std::vector<LegDev> legs;
+    std::vector<GroupDev> groups;
+    std::vector<int> suffix;
+    int64_t g1_tiles = 0, g2_tiles = 0, items = 0;
+    int max_chunks = 1;
+    int grid_cap = 2048;
+    std::vector<void *> allocs;
+    LegDev *dlegs = nullptr;
+    GroupDev *dgroups = nullptr;
+    int *dsuffix = nullptr;
+    int *dcounts = nullptr;
+    double2 *dpartials = nullptr;
+    double *dsum = nullptr;              // 2*nout
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_ms = 0.0;
+    double model_bytes = 0.0;
+};
+
+extern "C" int ps_batch_destroy(ps_batch *b)
+{
+    if (!b) return PS_ERR_INVALID;
+    if (b->c && b->c->st) (void)hipStreamSynchronize(b->c->st);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    for (void *p : b->allocs) (void)hipFree(p);
+    void *bufs[] = {b->dlegs, b->dgroups, b->dsuffix, b->dcounts, b->dpartials, b->dsum};
+    for (void *p : bufs) if (p) (void)hipFree(p);
+    delete b;
+    return PS_OK;
+}
+
+static bool same_mask(const MaskParams &a, const MaskParams &b)
+{
+    return a.kind == b.kind && a.kkf == b.kkf && a.mmin == b.mmin && a.mmax == b.mmax && a.bw == b.bw;
+}
+
+static int batch_record(ps_batch *b)
+{
+    ps_ctx *c = b->c;
+    PSCK(run_inverse_g(c->fp64, c->N, b->dlegs, (int)b->legs.size(), b->g1_tiles, b->g2_tiles, b->grid_cap, c->tw,
+                       c->st));
+    PSCK(run_zc_fused(c->fp64, c->N, b->dgroups, (int)b->groups.size(), b->dlegs, b->dsuffix, b->items, b->dpartials,
+                      b->max_chunks, b->grid_cap, c->tw, c->st));
+    PSCK(run_reduce_partials(b->dpartials, b->max_chunks, b->dcounts, 0, b->nout, b->dsum, b->dsum + b->nout, c->st));
+    return PS_OK;
+}
+
+extern "C" int ps_batch_create(ps_ctx *c, ps_polygon *polys, int npoly, double bw, int kcase, ps_batch **out)
+{
+    if (!c || !polys || npoly < 1 || !out) return PS_ERR_INVALID;
+    *out = nullptr;
+    HIPCK(hipSetDevice(c->device));
+    for (int p = 0; p < npoly; ++p) {
+        if (polys[p].n < 2 || polys[p].n > 3) return PS_ERR_INVALID;
+        if (polys[p].slot < 0 || polys[p].slot >= c->nslots) return PS_ERR_INVALID;
+        if (!c->loaded[polys[p].slot]) return PS_ERR_STATE;
+    }
+    const int N = c->N;
+    ps_batch *b = new ps_batch();
+    b->c = c;
+    b->npoly = npoly;
+    // 1. legs of every polygon, de-duplicated by (slot, mask)
+    std::vector<std::vector<int>> pleg(npoly);
+    std::vector<std::pair<int, MaskParams>> keys;
+    for (int p = 0; p < npoly; ++p) {
+        ps_polygon &P = polys[p];
+        for (int i = 0; i < P.n; ++i) {
+            MaskParams mp;
+            int s = leg_mask(c, i, P.n, P.k, P.kmin, P.kmax, bw, kcase, &mp);
+            if (s != PS_OK) {
+                ps_batch_destroy(b);
+                return s;
+            }
+            int id = -1;
+            for (size_t q = 0; q < keys.size(); ++q)
+                if (keys[q].first == P.slot && same_mask(keys[q].second, mp)) { id = (int)q; break; }
+            if (id < 0) {
+                id = (int)keys.size();
+                keys.push_back({P.slot, mp});
+            }
+            pleg[p].push_back(id);
+        }
+    }
+    // 2. leg descriptors, buffers and tile ranges
+    const size_t cs = c->csize();
+    const int tl = g_tile_lines(N, c->fp64);
+    double bytes = 0.0;
+    for (auto &kv : keys) {
+        LegDev L{};
+        L.mask = kv.second;
+        L.spec = c->spec[kv.first];
+        L.phase_only = c->phase_only;
+        const int K = shell_band(kv.second, N);
+        L.Kz = K;
+        L.Kx = K;
+        L.Bx = std::min(2 * K + 1, N);
+        L.g1_begin = b->g1_tiles;
+        L.g2_begin = b->g2_tiles;
+        b->g1_tiles += (int64_t)(K + 1) * ((L.Bx + tl - 1) / tl);
+        b->g2_tiles += (int64_t)(K + 1) * (N / tl);
+        const size_t n1 = (size_t)(K + 1) * L.Bx * N, n2 = (size_t)(K + 1) * N * N;
+        void *p1 = nullptr, *p2 = nullptr;
+        if (hipMalloc(&p1, cs * n1 * 2) != hipSuccess || hipMalloc(&p2, cs * n2 * 2) != hipSuccess) {
+            if (p1) (void)hipFree(p1);
+            ps_batch_destroy(b);
+            return PS_ERR_NOMEM;
+        }
+        b->allocs.push_back(p1);
+        b->allocs.push_back(p2);
+        L.g1d = p1;
+        L.g1i = (char *)p1 + cs * n1;
+        L.g2d = p2;
+        L.g2i = (char *)p2 + cs * n2;
+        b->legs.push_back(L);
+        // model bytes: G1 reads the selected spectrum cells (bounded by the
+        // band) and writes D,I band lines; G2 reads them and writes D,I planes
+        bytes += (double)cs * ((double)(K + 1) * L.Bx * (2 * K + 1) + 2.0 * n1 + 2.0 * n1 + 2.0 * n2);
+    }
+    // 3. groups: polygons sharing their prefix legs
+    std::map<std::tuple<int, int, int>, std::vector<int>> bykey;  // (npre-kind, a, b) -> polygons
+    for (int p = 0; p < npoly; ++p) {
+        const std::vector<int> &l = pleg[p];
+        if (polys[p].n == 3) bykey[{3, l[0], l[1]}].push_back(p);
+        else if (l[0] == l[1]) bykey[{1, -1, -1}].push_back(p);
+        else bykey[{2, l[0], -1}].push_back(p);
+    }
+    const int rows = fused_tile_rows(N, c->fp64);
+    const int64_t tiles_per_group = (int64_t)N * (N / rows);
+    int chunk_tiles = 64;
+    if (const char *e = getenv("PS_CHUNK_TILES")) chunk_tiles = std::max(1, atoi(e));
+    if (const char *e = getenv("PS_GRID_CAP")) b->grid_cap = std::max(1, atoi(e));
+    const int nchunks = (int)((tiles_per_group + chunk_tiles - 1) / chunk_tiles);
+    b->max_chunks = nchunks;
+    b->poly_out.assign(npoly, -1);
+    b->poly_n.assign(npoly, 0);
+    std::vector<int> counts;
+    for (auto &kv : bykey) {
+        const int kind = std::get<0>(kv.first);
+        const std::vector<int> &ps_ = kv.second;
+        for (size_t s0 = 0; s0 < ps_.size(); s0 += MAX_SUFFIX) {
+            const size_t s1 = std::min(ps_.size(), s0 + MAX_SUFFIX);
+            GroupDev G{};
+            if (kind == 3) {
+                G.preA = std::get<1>(kv.first);
+                G.preB = std::get<2>(kv.first);
+                G.pre_sq = (G.preA == G.preB) ? 1 : 0;
+                G.npre = G.pre_sq ? 1 : 2;
+            } else if (kind == 2) {
+                G.npre = 1;
+                G.preA = std::get<1>(kv.first);
+            } else {
+                G.npre = 0;
+                G.suf_sq = 1;
+            }
+            G.nsuf = (int)(s1 - s0);
+            G.suf_begin = (int)b->suffix.size();
+            G.out_begin = b->nout;
+            G.chunk_tiles = chunk_tiles;
+            G.item_begin = b->items;
+            for (size_t q = s0; q < s1; ++q) {
+                const int p = ps_[q];
+                const std::vector<int> &l = pleg[p];
+                b->suffix.push_back(kind == 1 ? l[0] : l[polys[p].n - 1]);
+                b->poly_out[p] = b->nout;
+                b->poly_n[p] = polys[p].n;
+                b->nout++;
+                counts.push_back(nchunks);
+            }
+            b->items += nchunks;
+            b->groups.push_back(G);
+            // model bytes of the fused pass: every leg's D,I planes read once per group
+            int64_t planes = 0;
+            if (G.npre >= 1) planes += b->legs[G.preA].Kz + 1;
+            if (G.npre == 2) planes += b->legs[G.preB].Kz + 1;
+            for (size_t q = s0; q < s1; ++q) planes += b->legs[b->suffix[G.suf_begin + (q - s0)]].Kz + 1;
+            bytes += 2.0 * cs * (double)planes * N * N;
+        }
+    }
+    b->model_bytes = bytes;
+    // 4. device tables
+    int st = PS_OK;
+#define BCK(call)                                       \
+    do {                                                \
+        if ((call) != hipSuccess) { st = PS_ERR_HIP; goto fail; } \
+    } while (0)
+    BCK(hipMalloc(&b->dlegs, sizeof(LegDev) * b->legs.size()));
+    BCK(hipMalloc(&b->dgroups, sizeof(GroupDev) * b->groups.size()));
+    BCK(hipMalloc(&b->dsuffix, sizeof(int) * b->suffix.size()));
+    BCK(hipMalloc(&b->dcounts, sizeof(int) * counts.size()));
+    BCK(hipMalloc(&b->dpartials, sizeof(double2) * 2 * (size_t)b->nout * b->max_chunks));
+    BCK(hipMalloc(&b->dsum, sizeof(double) * 2 * (size_t)b->nout));
+    BCK(hipMemcpy(b->dlegs, b->legs.data(), sizeof(LegDev) * b->legs.size(), hipMemcpyHostToDevice));
+    BCK(hipMemcpy(b->dgroups, b->groups.data(), sizeof(GroupDev) * b->groups.size(), hipMemcpyHostToDevice));
+    BCK(hipMemcpy(b->dsuffix, b->suffix.data(), sizeof(int) * b->suffix.size(), hipMemcpyHostToDevice));
+    BCK(hipMemcpy(b->dcounts, counts.data(), sizeof(int) * counts.size(), hipMemcpyHostToDevice));
+    BCK(hipEventCreate(&b->ev0));
+    BCK(hipEventCreate(&b->ev1));
+    // 5. capture the launch sequence once
+    if (!getenv("PS_NO_GRAPH")) {
+        BCK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+        st = batch_record(b);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(c->st, &g);
+        if (st != PS_OK || e != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            if (st == PS_OK) st = PS_ERR_HIP;
+            goto fail;
+        }
+        b->graph = g;
+        BCK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    }
+#undef BCK
+    *out = b;
+    return PS_OK;
+fail:
+    ps_batch_destroy(b);
+    return st;
+}
+
+extern "C" int ps_batch_execute(ps_batch *b, double *values, double *npolygons, double *sums)
+{
+    if (!b || !values) return PS_ERR_INVALID;
+    ps_ctx *c = b->c;
+    HIPCK(hipSetDevice(c->device));
+    HIPCK(hipEventRecord(b->ev0, c->st));
+    if (b->exec) HIPCK(hipGraphLaunch(b->exec, c->st));
+    else PSCK(batch_record(b));
+    HIPCK(hipEventRecord(b->ev1, c->st));
+    std::vector<double> h(2 * (size_t)b->nout);
+    HIPCK(hipMemcpyAsync(h.data(), b->dsum, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->st));
+    HIPCK(hipStreamSynchronize(c->st));
+    float ms = 0.f;
+    HIPCK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+    b->last_ms = ms;
+    const double volume = c->L * c->L * c->L;
+    const double cells = (double)((int64_t)c->N * c->N * c->N);
+    for (int p = 0; p < b->npoly; ++p) {
+        const int o = b->poly_out[p], n = b->poly_n[p];
+        const double sD = h[o], sI = h[b->nout + o];
+        values[p] = pow(volume, n - 1) * pow(c->N, -n * 3) * sD / sI;  // polyspectrum.c:155
+        if (npolygons) npolygons[p] = sI / cells;                      // polyspectrum.c:247
+        if (sums) {
+            sums[2 * p] = sD;
+            sums[2 * p + 1] = sI;
+        }
+    }
+    return PS_OK;
+}
+
+extern "C" double ps_batch_last_ms(const ps_batch *b) { return b ? b->last_ms : -1.0; }
+extern "C" double ps_batch_model_bytes(const ps_batch *b) { return b ? b->model_bytes : -1.0; }
+
+// ---------------------------------------------------------------- drop-in symbols
+// A process-wide FP64 context per (nbins, boxsize), as the reference's
+// functions carry no handle.
+static ps_ctx *g_legacy = nullptr;
+
+static int legacy_ctx(int nbins, double boxsize)
+{
+    if (g_legacy && (g_legacy->N != nbins || g_legacy->L != boxsize)) {
+        ps_ctx_destroy(g_legacy);
+        g_legacy = nullptr;
+    }
+    if (!g_legacy) return ps_ctx_create(nbins, boxsize, PS_PREC_FP64, 1, &g_legacy);
+    return PS_OK;
+}
+
+// full spectrum rows F[kx][ky][0..N/2] (pitch N/2+1) -> H[kz][kx][ky]
+__global__ void k_full_to_half(const double2 *__restrict__ rows, double2 *__restrict__ H, int N)
+{
+    __shared__ double2 t[32][33];
+    const int NH = N / 2 + 1;
+    const int kx = blockIdx.z;
+    const int ky0 = blockIdx.y * 32, kz0 = blockIdx.x * 32;
+    const int tx = threadIdx.x, ty = threadIdx.y;  // 32 x 8
+    for (int r = ty; r < 32; r += 8) {
+        const int ky = ky0 + r, kz = kz0 + tx;
+        if (ky < N && kz < NH) t[r][tx] = rows[((int64_t)kx * N + ky) * NH + kz];
+    }
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {
+        const int kz = kz0 + r, ky = ky0 + tx;
+        if (ky < N && kz < NH) H[((int64_t)kz * N + kx) * N + ky] = t[tx][r];
+    }
+}
+
+extern "C" double polyspectrum(int nbins, int local_n0, int local_n0_start, ps_fftw_complex *fft_array, int n,
+                               double *k, double *kmin, double *kmax, double kbinwidth, int kbinningCase,
+                               double boxsize)
+{
+    const double nan = std::nan("");
+    if (local_n0 != nbins || local_n0_start != 0 || !fft_array) return nan;
+    if (legacy_ctx(nbins, boxsize) != PS_OK) return nan;
+    ps_ctx *c = g_legacy;
+    const int N = nbins, NH = N / 2 + 1;
+    if (ensure_spec(c, 0) != PS_OK || ensure_plain_buffers(c) != PS_OK) return nan;
+    // the work buffer gd holds exactly N^2 rows of N/2+1 complex doubles
+    double2 *rows = (double2 *)c->gd;
+    if (hipMemcpy2DAsync(rows, NH * 16, fft_array, (size_t)N * 16, NH * 16, (size_t)N * N, hipMemcpyHostToDevice,
+                         c->st) != hipSuccess)
+        return nan;
+    hipLaunchKernelGGL(k_full_to_half, dim3((NH + 31) / 32, (N + 31) / 32, N), dim3(32, 8), 0, c->st, rows,
+                       (double2 *)c->spec[0], N);
+    if (hipStreamSynchronize(c->st) != hipSuccess) return nan;
+    c->loaded[0] = 1;
+    c->phase_only = 0;
+    double r = nan;
+    if (ps_polyspectrum(c, 0, n, k, kmin, kmax, kbinwidth, kbinningCase, &r, nullptr) != PS_OK) return nan;
+    return r;
+}
+
+extern "C" double num_polygons(int nbins, int local_n0, int local_n0_start, int n, double *k, double *kmin,
+                               double *kmax, double kbinwidth, int kbinningCase, double boxsize)
+{
+    const double nan = std::nan("");
+    if (local_n0 != nbins || local_n0_start != 0) return nan;
+    if (legacy_ctx(nbins, boxsize) != PS_OK) return nan;
+    double r = nan;
+    if (ps_num_polygons(g_legacy, n, k, kmin, kmax, kbinwidth, kbinningCase, &r) != PS_OK) return nan;
+    return r;
+}

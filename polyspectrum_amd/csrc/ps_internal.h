@@ -1,0 +1,66 @@
+// ps_internal.h -- structures shared by the kernels and the host orchestration.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/polyspectrum_gpu.h"
+#include "ps_mask.hpp"
+
+namespace ps {
+
+constexpr int MAX_SUFFIX = 64;  // suffix legs per fused group (per-wave accumulators in LDS)
+
+// lines (or rows) per LDS tile: 16 FP32 / 8 FP64 lines of N <= 512 fit 64 KB
+__host__ __device__ constexpr int tile_lines_for(int N, bool fp64)
+{
+    return (N <= 64) ? ((N % 16 == 0) ? 16 : 8) : (fp64 ? 8 : 16);
+}
+
+// One filtered field ("leg"): a shell mask applied to a spectrum (D part) and
+// to unit amplitude (I part), inverse transformed through the G1/G2 passes.
+struct LegDev {
+    MaskParams mask;
+    const void *spec;  // half spectrum H (nullptr: I part only)
+    int phase_only;    // normalise selected cells to z/|z|
+    int Kz;            // planes kz in [0, Kz] can be non-zero
+    int Kx;            // |mx| <= Kx can be non-zero
+    int Bx;            // band width min(2*Kx+1, N)
+    int64_t g1_begin;  // first tile of this leg in the G1 pass
+    int64_t g2_begin;  // first tile of this leg in the G2 pass
+    void *g1d, *g1i;   // band buffers [Kz+1][Bx][N] (D part may be null)
+    void *g2d, *g2i;   // full buffers [Kz+1][N][N]
+};
+
+// A batch of polygons sharing a prefix product: output s of the group is
+// sum_x P(x) * D_{suffix[s]}(x) (and the same for I), P = prod of prefix legs.
+struct GroupDev {
+    int npre;            // 0, 1 or 2 prefix legs
+    int pre_sq;          // prefix leg preA enters squared (k1 == k2)
+    int preA, preB;      // leg indices
+    int nsuf;            // suffix legs, <= MAX_SUFFIX
+    int suf_begin;       // index into the suffix leg list
+    int suf_sq;          // suffix legs enter squared (power spectra, npre == 0)
+    int out_begin;       // first output slot
+    int chunk_tiles;     // Zc tiles per work item
+    int pad_;
+    int64_t item_begin;  // first work item of this group
+};
+
+int supported_grid(int N);
+int tile_rows(int N, int fp64);
+int fused_tile_rows(int N, int fp64);
+int g_tile_lines(int N, int fp64);
+
+int run_forward(int fp64, int N, const void *field, int field_is_double, void *H, const void *tw, hipStream_t st);
+int run_inverse_g(int fp64, int N, const LegDev *legs, int nlegs, int64_t g1_tiles, int64_t g2_tiles, int grid_cap,
+                  const void *tw, hipStream_t st);
+int run_zc_plain(int fp64, int N, const void *gd, const void *gi, int Kz, double *accD, double *accI, int mode,
+                 double2 *partials, const void *tw, hipStream_t st);
+int run_zc_fused(int fp64, int N, const GroupDev *groups, int ngroups, const LegDev *legs, const int *suffix,
+                 int64_t nitems, double2 *partials, int max_chunks, int grid_cap, const void *tw, hipStream_t st);
+int run_reduce_partials(const double2 *partials, int64_t stride, const int *counts, int64_t uniform_count,
+                        int64_t nout, double *outD, double *outI, hipStream_t st);
+int run_fill(double *p, int64_t n, double v, hipStream_t st);
+int run_corner_extremes(const MaskParams &mp, int N, int K, double bw, double2 *partials, int nblocks, hipStream_t st);
+
+}  // namespace ps

@@ -1,0 +1,672 @@
+// ps_kernels.hip -- gfx950 kernels of the polyspectrum estimator.
+//
+// Spectrum layout in HBM ("half spectrum"): H[kz][kx][ky], kz in [0, N/2],
+// kx, ky in [0, N), ky fastest.  The real field is C-ordered [x][y][z]
+// (grid.c:168).  Because the field is real and every shell mask is symmetric
+// under m -> -m, every filtered spectrum is Hermitian and its inverse
+// transform is real; the reference's complex results have imaginary parts at
+// roundoff (polyspectrum.c:142 prints that ratio), so half spectra and real
+// outputs carry all of the estimator.
+//
+// Passes of one 3D transform (each a batch of line FFTs in LDS, ps_fft_core):
+//   forward : Z  (R2C along z, rows read contiguously, transposed tile write)
+//             Y  (C2C along ky, contiguous lines)
+//             X  (C2C along kx, stride N, tiles of adjacent ky columns)
+//   inverse : G1 (C2C along ky of the masked spectrum, gathered on the fly;
+//                 only planes kz <= Kz and the kx band |mx| <= Kx that a shell
+//                 can touch)
+//             G2 (C2C along kx from the band to all x)
+//             Zc (C2R along kz with a fused epilogue: store, multiply into an
+//                 accumulator, dot with an accumulator, or the multi-leg
+//                 product-and-reduce of a batch of polygons)
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "ps_fft_core.hpp"
+#include "ps_internal.h"
+#include "ps_mask.hpp"
+
+namespace ps {
+
+// ---------------------------------------------------------------- helpers
+template <typename T> __device__ __forceinline__ T ld_real(const float *p, int64_t i) { return (T)p[i]; }
+template <typename T> __device__ __forceinline__ T ld_real(const double *p, int64_t i) { return (T)p[i]; }
+
+// rows per tile of the z passes and lines per tile of the other passes
+template <typename T, int N> constexpr int tile_lines() { return tile_lines_for(N, sizeof(T) == 8); }
+
+// ================================================================ forward
+// Z pass: R2C along z.  Two real rows share one complex FFT (z = a + i b);
+// A[k] = (Z[k] + conj Z[N-k]) / 2, B[k] = (Z[k] - conj Z[N-k]) / (2i).
+template <typename T, typename TI, int N>
+__global__ void __launch_bounds__(NT) k_fwd_z(const TI *__restrict__ in, cplx<T> *__restrict__ H,
+                                              const cplx<T> *__restrict__ tw)
+{
+    constexpr int TL = tile_lines<T, N>();  // real rows per tile
+    constexpr int TLC = TL / 2;
+    __shared__ cplx<T> lds[N * TLC];
+    const int tid = threadIdx.x;
+    const int y0 = blockIdx.x * TL, x = blockIdx.y;
+    const int64_t base = ((int64_t)x * N + y0) * N;
+    for (int idx = tid; idx < TL * N; idx += NT) {
+        const int row = idx / N, z = idx % N;
+        T v = ld_real<T>(in, base + idx);
+        T *p = reinterpret_cast<T *>(&lds[lidx<TLC>(z, row >> 1)]);
+        p[row & 1] = v;
+    }
+    __syncthreads();
+    fft_lds<T, N, TLC, -1>(lds, tw, tid);
+    constexpr int NH = N / 2 + 1;
+    for (int idx = tid; idx < NH * TLC; idx += NT) {
+        const int c = idx % TLC, kz = idx / TLC;
+        const cplx<T> z1 = lds[lidx<TLC>(kz, c)];
+        const cplx<T> z2 = lds[lidx<TLC>((N - kz) % N, c)];
+        const cplx<T> A = mk<T>((z1.x + z2.x) * (T)0.5, (z1.y - z2.y) * (T)0.5);
+        const cplx<T> B = mk<T>((z1.y + z2.y) * (T)0.5, (z2.x - z1.x) * (T)0.5);
+        const int64_t o = ((int64_t)kz * N + x) * N + y0 + 2 * c;
+        H[o] = A;
+        H[o + 1] = B;
+    }
+}
+
+// Y pass: C2C along the contiguous axis for nlines consecutive lines.
+template <typename T, int N, int S>
+__global__ void __launch_bounds__(NT) k_c2c_contig(cplx<T> *__restrict__ data, const cplx<T> *__restrict__ tw)
+{
+    constexpr int TLC = tile_lines<T, N>();
+    __shared__ cplx<T> lds[N * TLC];
+    const int tid = threadIdx.x;
+    cplx<T> *base = data + (int64_t)blockIdx.x * TLC * N;
+    for (int idx = tid; idx < TLC * N; idx += NT) lds[lidx<TLC>(idx % N, idx / N)] = base[idx];
+    __syncthreads();
+    fft_lds<T, N, TLC, S>(lds, tw, tid);
+    for (int idx = tid; idx < TLC * N; idx += NT) base[idx] = lds[lidx<TLC>(idx % N, idx / N)];
+}
+
+// X pass: C2C along the stride-N axis; block = (ky tile, plane).
+template <typename T, int N, int S>
+__global__ void __launch_bounds__(NT) k_c2c_strided(cplx<T> *__restrict__ data, const cplx<T> *__restrict__ tw)
+{
+    constexpr int TLC = tile_lines<T, N>();
+    __shared__ cplx<T> lds[N * TLC];
+    const int tid = threadIdx.x;
+    cplx<T> *base = data + (int64_t)blockIdx.y * N * N + blockIdx.x * TLC;
+    for (int idx = tid; idx < TLC * N; idx += NT) {
+        const int l = idx % TLC, e = idx / TLC;
+        lds[lidx<TLC>(e, l)] = base[(int64_t)e * N + l];
+    }
+    __syncthreads();
+    fft_lds<T, N, TLC, S>(lds, tw, tid);
+    for (int idx = tid; idx < TLC * N; idx += NT) {
+        const int l = idx % TLC, e = idx / TLC;
+        base[(int64_t)e * N + l] = lds[lidx<TLC>(e, l)];
+    }
+}
+
+// ================================================================ inverse
+__device__ __forceinline__ int band_to_k(int j, int K, int B, int N) { return (B == N) ? j : (j <= K ? j : j + N - B); }
+__device__ __forceinline__ int k_to_band(int k, int K, int B, int N)
+{
+    if (B == N) return k;
+    if (k <= K) return k;
+    if (k >= N - K) return k - N + B;
+    return -1;
+}
+
+// find the leg (or group) owning a tile: largest i with begin[i] <= t
+template <typename D>
+__device__ __forceinline__ int find_owner(const D *__restrict__ d, int n, int64_t t, int64_t D::*field)
+{
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (d[mid].*field <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// G1: masked gather + C2C along ky.  Tile = (leg, kz, band tile of TLC kx
+// lines); LDS holds TLC D-lines (mask * spectrum) and TLC I-lines (mask).
+template <typename T, int N>
+__global__ void __launch_bounds__(NT) k_inv_g1(const LegDev *__restrict__ legs, int nlegs, int64_t ntiles,
+                                               const cplx<T> *__restrict__ tw)
+{
+    constexpr int TLC = tile_lines<T, N>() / 2;
+    constexpr int TL2 = 2 * TLC;
+    __shared__ cplx<T> lds[N * TL2];
+    const int tid = threadIdx.x;
+    for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const int li = find_owner(legs, nlegs, tile, &LegDev::g1_begin);
+        const LegDev &L = legs[li];
+        const int nbt = (L.Bx + TLC - 1) / TLC;
+        const int64_t local = tile - L.g1_begin;
+        const int kz = (int)(local / nbt);
+        const int j0 = (int)(local % nbt) * TLC;
+        const int mz = lattice(kz, N);
+        const cplx<T> *spec = reinterpret_cast<const cplx<T> *>(L.spec);
+        for (int idx = tid; idx < TLC * N; idx += NT) {
+            const int l = idx / N, ky = idx % N;
+            const int j = j0 + l;
+            cplx<T> d = mk<T>(0, 0), u = mk<T>(0, 0);
+            if (j < L.Bx) {
+                const int kx = band_to_k(j, L.Kx, L.Bx, N);
+                if (mask_eval(L.mask, lattice(kx, N), lattice(ky, N), mz)) {
+                    u.x = (T)1;
+                    if (spec) {
+                        d = spec[((int64_t)kz * N + kx) * N + ky];
+                        if (L.phase_only) {
+                            // z / |z| (utils_fftw.c:78-93), applied to the cells a shell selects
+                            const T a = hypot(d.x, d.y);
+                            d = mk<T>(d.x / a, d.y / a);
+                        }
+                    }
+                }
+            }
+            lds[lidx<TL2>(ky, l)] = d;
+            lds[lidx<TL2>(ky, l + TLC)] = u;
+        }
+        __syncthreads();
+        fft_lds<T, N, TL2, +1>(lds, tw, tid);
+        cplx<T> *gd = reinterpret_cast<cplx<T> *>(L.g1d);
+        cplx<T> *gi = reinterpret_cast<cplx<T> *>(L.g1i);
+        for (int idx = tid; idx < TLC * N; idx += NT) {
+            const int l = idx / N, y = idx % N;
+            const int j = j0 + l;
+            if (j < L.Bx) {
+                const int64_t o = ((int64_t)kz * L.Bx + j) * N + y;
+                if (gd) gd[o] = lds[lidx<TL2>(y, l)];
+                gi[o] = lds[lidx<TL2>(y, l + TLC)];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// G2: C2C along kx from the band to every x.  Tile = (leg, kz, y tile).
+template <typename T, int N>
+__global__ void __launch_bounds__(NT) k_inv_g2(const LegDev *__restrict__ legs, int nlegs, int64_t ntiles,
+                                               const cplx<T> *__restrict__ tw)
+{
+    constexpr int TLC = tile_lines<T, N>() / 2;
+    constexpr int TL2 = 2 * TLC;
+    __shared__ cplx<T> lds[N * TL2];
+    const int tid = threadIdx.x;
+    for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const int li = find_owner(legs, nlegs, tile, &LegDev::g2_begin);
+        const LegDev &L = legs[li];
+        constexpr int nyt = N / TLC;
+        const int64_t local = tile - L.g2_begin;
+        const int kz = (int)(local / nyt);
+        const int y0 = (int)(local % nyt) * TLC;
+        const cplx<T> *gd = reinterpret_cast<const cplx<T> *>(L.g1d);
+        const cplx<T> *gi = reinterpret_cast<const cplx<T> *>(L.g1i);
+        for (int idx = tid; idx < TLC * N; idx += NT) {
+            const int l = idx % TLC, kx = idx / TLC;
+            const int j = k_to_band(kx, L.Kx, L.Bx, N);
+            cplx<T> d = mk<T>(0, 0), u = mk<T>(0, 0);
+            if (j >= 0) {
+                const int64_t o = ((int64_t)kz * L.Bx + j) * N + y0 + l;
+                if (gd) d = gd[o];
+                u = gi[o];
+            }
+            lds[lidx<TL2>(kx, l)] = d;
+            lds[lidx<TL2>(kx, l + TLC)] = u;
+        }
+        __syncthreads();
+        fft_lds<T, N, TL2, +1>(lds, tw, tid);
+        cplx<T> *od = reinterpret_cast<cplx<T> *>(L.g2d);
+        cplx<T> *oi = reinterpret_cast<cplx<T> *>(L.g2i);
+        for (int idx = tid; idx < TLC * N; idx += NT) {
+            const int l = idx % TLC, x = idx / TLC;
+            const int64_t o = ((int64_t)kz * N + x) * N + y0 + l;
+            if (od) od[o] = lds[lidx<TL2>(x, l)];
+            oi[o] = lds[lidx<TL2>(x, l + TLC)];
+        }
+        __syncthreads();
+    }
+}
+
+// Build the complex z-line Z = XD + i XI of one y row from the two half
+// spectra (Hermitian extension; only real parts at kz = 0 and N/2, as a C2R
+// transform uses them).  Line l of the tile = row y0 + l.
+template <typename T, int N, int TLC>
+__device__ __forceinline__ void load_zline_pair(cplx<T> *lds, const cplx<T> *__restrict__ gd,
+                                                const cplx<T> *__restrict__ gi, int Kz, int x, int y0, int tid)
+{
+    constexpr int NH = N / 2 + 1;
+    for (int idx = tid; idx < NH * TLC; idx += NT) {
+        const int l = idx % TLC, kz = idx / TLC;
+        cplx<T> X = mk<T>(0, 0), Y = mk<T>(0, 0);
+        if (kz <= Kz) {
+            const int64_t o = ((int64_t)kz * N + x) * N + y0 + l;
+            if (gd) X = gd[o];
+            if (gi) Y = gi[o];
+        }
+        if (kz == 0 || kz == N / 2) {
+            lds[lidx<TLC>(kz, l)] = mk<T>(X.x, Y.x);
+        } else {
+            lds[lidx<TLC>(kz, l)] = mk<T>(X.x - Y.y, X.y + Y.x);
+            lds[lidx<TLC>(N - kz, l)] = mk<T>(X.x + Y.y, Y.x - X.y);
+        }
+    }
+}
+
+// Zc for the reference-shaped path: per tile (x, y0..y0+TLC), C2R of the
+// D and I half spectra of one leg, then
+//   mode 0: accD = D, accI = I
+//   mode 1: accD *= D, accI *= I                   (utils_fftw.c:32-44)
+//   mode 2: partial sums of accD*D and accI*I       (utils_fftw.c:60-76)
+// Accumulators are FP64 real N^3 grids.
+template <typename T, int N>
+__global__ void __launch_bounds__(NT) k_inv_zc_plain(const cplx<T> *__restrict__ gd, const cplx<T> *__restrict__ gi,
+                                                     int Kz, double *__restrict__ accD, double *__restrict__ accI,
+                                                     int mode, double2 *__restrict__ partials,
+                                                     const cplx<T> *__restrict__ tw)
+{
+    constexpr int TLC = tile_lines<T, N>();
+    __shared__ cplx<T> lds[N * TLC];
+    __shared__ double red[4 * NT / 64];
+    const int tid = threadIdx.x;
+    const int y0 = blockIdx.x * TLC, x = blockIdx.y;
+    load_zline_pair<T, N, TLC>(lds, gd, gi, Kz, x, y0, tid);
+    __syncthreads();
+    fft_lds<T, N, TLC, +1>(lds, tw, tid);
+    const int64_t base = ((int64_t)x * N + y0) * N;
+    KSum sd, si;
+    sd.init();
+    si.init();
+    for (int idx = tid; idx < TLC * N; idx += NT) {
+        const int l = idx / N, z = idx % N;
+        const cplx<T> v = lds[lidx<TLC>(z, l)];
+        const int64_t o = base + idx;
+        if (mode == 0) {
+            if (accD) accD[o] = (double)v.x;
+            accI[o] = (double)v.y;
+        } else if (mode == 1) {
+            if (accD) accD[o] *= (double)v.x;
+            accI[o] *= (double)v.y;
+        } else {
+            if (accD) sd.add(accD[o] * (double)v.x);
+            si.add(accI[o] * (double)v.y);
+        }
+    }
+    if (mode == 2) {
+        double a = sd.s, b = sd.c, c = si.s, d = si.c;
+        wave_sum2(a, b);
+        wave_sum2(c, d);
+        const int w = tid / 64;
+        if ((tid & 63) == 0) {
+            red[4 * w + 0] = a; red[4 * w + 1] = b; red[4 * w + 2] = c; red[4 * w + 3] = d;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            KSum A, B;
+            A.init();
+            B.init();
+            for (int i = 0; i < NT / 64; ++i) {
+                A.add(red[4 * i + 0]); A.add(red[4 * i + 1]);
+                B.add(red[4 * i + 2]); B.add(red[4 * i + 3]);
+            }
+            const int64_t bi = (int64_t)blockIdx.y * gridDim.x + blockIdx.x;
+            partials[2 * bi + 0] = make_double2(A.s, A.c);
+            partials[2 * bi + 1] = make_double2(B.s, B.c);
+        }
+    }
+}
+
+// Zc for batches of polygons: tile = (group, x, y0..y0+TLC).  For each row the
+// C2R of (D + i I) of every leg runs in LDS; the prefix product P of the
+// shared legs stays in registers in FP64 (the product of two FP32 values is
+// exact there) and every suffix leg t adds sum_z P * D_t and sum_z P_I * I_t
+// to compensated per-wavefront accumulators.  A work item is a chunk of
+// consecutive tiles of one group; its partial sums go to
+// partials[out][chunk] as (sum, compensation).
+template <typename T, int N>
+__global__ void __launch_bounds__(NT) k_inv_zc_fused(const GroupDev *__restrict__ groups, int ngroups,
+                                                     const LegDev *__restrict__ legs,
+                                                     const int *__restrict__ suffix, int64_t nitems,
+                                                     double2 *__restrict__ partials, int max_chunks,
+                                                     const cplx<T> *__restrict__ tw)
+{
+    constexpr int TLC = tile_lines<T, N>() / 2;  // rows per tile
+    constexpr int EPT = (TLC * N + NT - 1) / NT;  // elements per thread
+    constexpr int NW = NT / 64;
+    __shared__ cplx<T> lds[N * TLC];
+    __shared__ double acc[NW][MAX_SUFFIX][4];
+    const int tid = threadIdx.x, w = tid / 64, lane = tid & 63;
+    for (int64_t item = blockIdx.x; item < nitems; item += gridDim.x) {
+        const int gi = find_owner(groups, ngroups, item, &GroupDev::item_begin);
+        const GroupDev &G = groups[gi];
+        const int chunk = (int)(item - G.item_begin);
+        constexpr int tiles_per_x = N / TLC;
+        const int64_t t_begin = (int64_t)chunk * G.chunk_tiles;
+        int64_t t_end = t_begin + G.chunk_tiles;
+        const int64_t ntile = (int64_t)N * tiles_per_x;
+        if (t_end > ntile) t_end = ntile;
+        for (int i = tid; i < NW * MAX_SUFFIX * 4; i += NT) (&acc[0][0][0])[i] = 0.0;
+        __syncthreads();
+        for (int64_t t = t_begin; t < t_end; ++t) {
+            const int x = (int)(t / tiles_per_x);
+            const int y0 = (int)(t % tiles_per_x) * TLC;
+            double pd[EPT], pi[EPT];
+#pragma unroll
+            for (int e = 0; e < EPT; ++e) { pd[e] = 1.0; pi[e] = 1.0; }
+            for (int p = 0; p < G.npre; ++p) {
+                const LegDev &L = legs[p == 0 ? G.preA : G.preB];
+                load_zline_pair<T, N, TLC>(lds, reinterpret_cast<const cplx<T> *>(L.g2d),
+                                           reinterpret_cast<const cplx<T> *>(L.g2i), L.Kz, x, y0, tid);
+                __syncthreads();
+                fft_lds<T, N, TLC, +1>(lds, tw, tid);
+#pragma unroll
+                for (int e = 0; e < EPT; ++e) {
+                    const int idx = tid + e * NT;
+                    if (idx < TLC * N) {
+                        const cplx<T> v = lds[lidx<TLC>(idx % N, idx / N)];
+                        const double dv = (double)v.x, iv = (double)v.y;
+                        if (G.pre_sq) { pd[e] *= dv * dv; pi[e] *= iv * iv; }
+                        else { pd[e] *= dv; pi[e] *= iv; }
+                    }
+                }
+                __syncthreads();
+            }
+            for (int s = 0; s < G.nsuf; ++s) {
+                const LegDev &L = legs[suffix[G.suf_begin + s]];
+                load_zline_pair<T, N, TLC>(lds, reinterpret_cast<const cplx<T> *>(L.g2d),
+                                           reinterpret_cast<const cplx<T> *>(L.g2i), L.Kz, x, y0, tid);
+                __syncthreads();
+                fft_lds<T, N, TLC, +1>(lds, tw, tid);
+                double sd = 0.0, si = 0.0;
+#pragma unroll
+                for (int e = 0; e < EPT; ++e) {
+                    const int idx = tid + e * NT;
+                    if (idx < TLC * N) {
+                        const cplx<T> v = lds[lidx<TLC>(idx % N, idx / N)];
+                        const double dv = (double)v.x, iv = (double)v.y;
+                        if (G.suf_sq) { sd += pd[e] * dv * dv; si += pi[e] * iv * iv; }
+                        else { sd += pd[e] * dv; si += pi[e] * iv; }
+                    }
+                }
+                double sdc = 0.0, sic = 0.0;
+                wave_sum2(sd, sdc);
+                wave_sum2(si, sic);
+                if (lane == 0) {
+                    KSum a; a.s = acc[w][s][0]; a.c = acc[w][s][1]; a.add(sd); a.add(sdc);
+                    KSum b; b.s = acc[w][s][2]; b.c = acc[w][s][3]; b.add(si); b.add(sic);
+                    acc[w][s][0] = a.s; acc[w][s][1] = a.c; acc[w][s][2] = b.s; acc[w][s][3] = b.c;
+                }
+                __syncthreads();
+            }
+        }
+        __syncthreads();
+        for (int s = tid; s < G.nsuf; s += NT) {
+            KSum a, b;
+            a.init();
+            b.init();
+            for (int ww = 0; ww < NW; ++ww) {
+                a.add(acc[ww][s][0]); a.add(acc[ww][s][1]);
+                b.add(acc[ww][s][2]); b.add(acc[ww][s][3]);
+            }
+            const int64_t o = (int64_t)(G.out_begin + s) * max_chunks + chunk;
+            partials[2 * o + 0] = make_double2(a.s, a.c);
+            partials[2 * o + 1] = make_double2(b.s, b.c);
+        }
+        __syncthreads();
+    }
+}
+
+// ================================================================ reductions
+// out[i] = compensated sum over j < count[i] of partials[2*(i*stride + j) + which]
+__global__ void __launch_bounds__(NT) k_reduce_partials(const double2 *__restrict__ partials, int64_t stride,
+                                                        const int *__restrict__ counts, int64_t uniform_count,
+                                                        double *__restrict__ outD, double *__restrict__ outI)
+{
+    __shared__ double red[NT / 64][4];
+    const int tid = threadIdx.x;
+    const int64_t i = blockIdx.x;
+    const int64_t cnt = counts ? counts[i] : uniform_count;
+    KSum a, b;
+    a.init();
+    b.init();
+    for (int64_t j = tid; j < cnt; j += NT) {
+        const double2 p = partials[2 * (i * stride + j) + 0];
+        const double2 q = partials[2 * (i * stride + j) + 1];
+        a.add(p.x); a.add(p.y);
+        b.add(q.x); b.add(q.y);
+    }
+    double as = a.s, ac = a.c, bs = b.s, bc = b.c;
+    wave_sum2(as, ac);
+    wave_sum2(bs, bc);
+    if ((tid & 63) == 0) {
+        red[tid / 64][0] = as; red[tid / 64][1] = ac; red[tid / 64][2] = bs; red[tid / 64][3] = bc;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        KSum A, B;
+        A.init();
+        B.init();
+        for (int k = 0; k < NT / 64; ++k) {
+            A.add(red[k][0]); A.add(red[k][1]);
+            B.add(red[k][2]); B.add(red[k][3]);
+        }
+        if (outD) outD[i] = A.value();
+        if (outI) outI[i] = B.value();
+    }
+}
+
+__global__ void k_fill(double *__restrict__ p, int64_t n, double v)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+// DERIVED binning: extremes of the squared corner norms over a shell's cells
+// (filter.c:106-113).  Partial min/max per block over the band |m_i| <= K.
+__global__ void __launch_bounds__(NT) k_shell_corner_extremes(MaskParams mp, int N, int K, double bw,
+                                                              double2 *__restrict__ partials)
+{
+    __shared__ double lo_s[NT], hi_s[NT];
+    const int tid = threadIdx.x;
+    const int B = 2 * K + 1;
+    const int64_t total = (int64_t)B * B * B;
+    double lo = 1.e30 * 1.e30, hi = 0.0;
+    for (int64_t c = (int64_t)blockIdx.x * NT + tid; c < total; c += (int64_t)gridDim.x * NT) {
+        const int mx = (int)(c / ((int64_t)B * B)) - K;
+        const int my = (int)((c / B) % B) - K;
+        const int mz = (int)(c % B) - K;
+        // a lattice coordinate +N/2 does not exist: index N/2 is -N/2
+        if (2 * mx == N || 2 * my == N || 2 * mz == N) continue;
+        if (mask_eval(mp, mx, my, mz)) {
+            double l2, h2;
+            corner_extremes(mx, my, mz, bw, l2, h2);
+            lo = fmin(lo, l2);
+            hi = fmax(hi, h2);
+        }
+    }
+    lo_s[tid] = lo;
+    hi_s[tid] = hi;
+    __syncthreads();
+    for (int s = NT / 2; s > 0; s >>= 1) {
+        if (tid < s) {
+            lo_s[tid] = fmin(lo_s[tid], lo_s[tid + s]);
+            hi_s[tid] = fmax(hi_s[tid], hi_s[tid + s]);
+        }
+        __syncthreads();
+    }
+    if (tid == 0) partials[blockIdx.x] = make_double2(lo_s[0], hi_s[0]);
+}
+
+// ================================================================ launchers
+template <typename T, int N>
+static int launch_forward_n(const void *field, int field_is_double, void *H, const void *tw, hipStream_t st)
+{
+    constexpr int TL = tile_lines<T, N>();
+    constexpr int TLC = tile_lines<T, N>();
+    const cplx<T> *twp = reinterpret_cast<const cplx<T> *>(tw);
+    cplx<T> *h = reinterpret_cast<cplx<T> *>(H);
+    if (field_is_double)
+        hipLaunchKernelGGL((k_fwd_z<T, double, N>), dim3(N / TL, N), dim3(NT), 0, st, (const double *)field, h, twp);
+    else
+        hipLaunchKernelGGL((k_fwd_z<T, float, N>), dim3(N / TL, N), dim3(NT), 0, st, (const float *)field, h, twp);
+    const int64_t nlines = (int64_t)(N / 2 + 1) * N;  // (kz, kx) lines along ky
+    hipLaunchKernelGGL((k_c2c_contig<T, N, -1>), dim3((unsigned)(nlines / TLC)), dim3(NT), 0, st, h, twp);
+    hipLaunchKernelGGL((k_c2c_strided<T, N, -1>), dim3(N / TLC, N / 2 + 1), dim3(NT), 0, st, h, twp);
+    return (int)hipGetLastError();
+}
+
+template <typename T, int N>
+static int launch_inverse_g_n(const LegDev *legs_dev, int nlegs, int64_t g1_tiles, int64_t g2_tiles, int grid_cap,
+                              const void *tw, hipStream_t st)
+{
+    const cplx<T> *twp = reinterpret_cast<const cplx<T> *>(tw);
+    if (g1_tiles > 0) {
+        const unsigned g = (unsigned)((g1_tiles < grid_cap) ? g1_tiles : grid_cap);
+        hipLaunchKernelGGL((k_inv_g1<T, N>), dim3(g), dim3(NT), 0, st, legs_dev, nlegs, g1_tiles, twp);
+    }
+    if (g2_tiles > 0) {
+        const unsigned g = (unsigned)((g2_tiles < grid_cap) ? g2_tiles : grid_cap);
+        hipLaunchKernelGGL((k_inv_g2<T, N>), dim3(g), dim3(NT), 0, st, legs_dev, nlegs, g2_tiles, twp);
+    }
+    return (int)hipGetLastError();
+}
+
+template <typename T, int N>
+static int launch_zc_plain_n(const void *gd, const void *gi, int Kz, double *accD, double *accI, int mode,
+                             double2 *partials, const void *tw, hipStream_t st)
+{
+    constexpr int TLC = tile_lines<T, N>();
+    hipLaunchKernelGGL((k_inv_zc_plain<T, N>), dim3(N / TLC, N), dim3(NT), 0, st,
+                       reinterpret_cast<const cplx<T> *>(gd), reinterpret_cast<const cplx<T> *>(gi), Kz, accD, accI,
+                       mode, partials, reinterpret_cast<const cplx<T> *>(tw));
+    return (int)hipGetLastError();
+}
+
+template <typename T, int N>
+static int launch_zc_fused_n(const GroupDev *groups, int ngroups, const LegDev *legs, const int *suffix,
+                             int64_t nitems, double2 *partials, int max_chunks, int grid_cap, const void *tw,
+                             hipStream_t st)
+{
+    const unsigned g = (unsigned)((nitems < grid_cap) ? nitems : grid_cap);
+    hipLaunchKernelGGL((k_inv_zc_fused<T, N>), dim3(g), dim3(NT), 0, st, groups, ngroups, legs, suffix, nitems,
+                       partials, max_chunks, reinterpret_cast<const cplx<T> *>(tw));
+    return (int)hipGetLastError();
+}
+
+#define PS_FOR_EACH_N(X) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512)
+
+template <typename T>
+static int dispatch_forward(int N, const void *field, int fd, void *H, const void *tw, hipStream_t st)
+{
+    switch (N) {
+#define CASE(n) case n: return launch_forward_n<T, n>(field, fd, H, tw, st);
+        PS_FOR_EACH_N(CASE)
+#undef CASE
+    }
+    return PS_ERR_UNSUPPORTED_N;
+}
+
+template <typename T>
+static int dispatch_inverse_g(int N, const LegDev *legs, int nlegs, int64_t g1, int64_t g2, int cap, const void *tw,
+                              hipStream_t st)
+{
+    switch (N) {
+#define CASE(n) case n: return launch_inverse_g_n<T, n>(legs, nlegs, g1, g2, cap, tw, st);
+        PS_FOR_EACH_N(CASE)
+#undef CASE
+    }
+    return PS_ERR_UNSUPPORTED_N;
+}
+
+template <typename T>
+static int dispatch_zc_plain(int N, const void *gd, const void *gi, int Kz, double *accD, double *accI, int mode,
+                             double2 *partials, const void *tw, hipStream_t st)
+{
+    switch (N) {
+#define CASE(n) case n: return launch_zc_plain_n<T, n>(gd, gi, Kz, accD, accI, mode, partials, tw, st);
+        PS_FOR_EACH_N(CASE)
+#undef CASE
+    }
+    return PS_ERR_UNSUPPORTED_N;
+}
+
+template <typename T>
+static int dispatch_zc_fused(int N, const GroupDev *groups, int ngroups, const LegDev *legs, const int *suffix,
+                             int64_t nitems, double2 *partials, int max_chunks, int cap, const void *tw,
+                             hipStream_t st)
+{
+    switch (N) {
+#define CASE(n) case n: return launch_zc_fused_n<T, n>(groups, ngroups, legs, suffix, nitems, partials, max_chunks, cap, tw, st);
+        PS_FOR_EACH_N(CASE)
+#undef CASE
+    }
+    return PS_ERR_UNSUPPORTED_N;
+}
+
+// ---------------------------------------------------------------- exported (internal API)
+int supported_grid(int N)
+{
+    switch (N) {
+#define CASE(n) case n:
+        PS_FOR_EACH_N(CASE)
+#undef CASE
+        return 1;
+    }
+    return 0;
+}
+
+int tile_rows(int N, int fp64) { return tile_lines_for(N, fp64 != 0); }
+
+int fused_tile_rows(int N, int fp64) { return tile_rows(N, fp64) / 2; }
+int g_tile_lines(int N, int fp64) { return tile_rows(N, fp64) / 2; }
+
+int run_forward(int fp64, int N, const void *field, int field_is_double, void *H, const void *tw, hipStream_t st)
+{
+    return fp64 ? dispatch_forward<double>(N, field, field_is_double, H, tw, st)
+                : dispatch_forward<float>(N, field, field_is_double, H, tw, st);
+}
+
+int run_inverse_g(int fp64, int N, const LegDev *legs, int nlegs, int64_t g1_tiles, int64_t g2_tiles, int grid_cap,
+                  const void *tw, hipStream_t st)
+{
+    return fp64 ? dispatch_inverse_g<double>(N, legs, nlegs, g1_tiles, g2_tiles, grid_cap, tw, st)
+                : dispatch_inverse_g<float>(N, legs, nlegs, g1_tiles, g2_tiles, grid_cap, tw, st);
+}
+
+int run_zc_plain(int fp64, int N, const void *gd, const void *gi, int Kz, double *accD, double *accI, int mode,
+                 double2 *partials, const void *tw, hipStream_t st)
+{
+    return fp64 ? dispatch_zc_plain<double>(N, gd, gi, Kz, accD, accI, mode, partials, tw, st)
+                : dispatch_zc_plain<float>(N, gd, gi, Kz, accD, accI, mode, partials, tw, st);
+}
+
+int run_zc_fused(int fp64, int N, const GroupDev *groups, int ngroups, const LegDev *legs, const int *suffix,
+                 int64_t nitems, double2 *partials, int max_chunks, int grid_cap, const void *tw, hipStream_t st)
+{
+    return fp64 ? dispatch_zc_fused<double>(N, groups, ngroups, legs, suffix, nitems, partials, max_chunks, grid_cap, tw, st)
+                : dispatch_zc_fused<float>(N, groups, ngroups, legs, suffix, nitems, partials, max_chunks, grid_cap, tw, st);
+}
+
+int run_reduce_partials(const double2 *partials, int64_t stride, const int *counts, int64_t uniform_count,
+                        int64_t nout, double *outD, double *outI, hipStream_t st)
+{
+    if (nout <= 0) return 0;
+    hipLaunchKernelGGL(k_reduce_partials, dim3((unsigned)nout), dim3(NT), 0, st, partials, stride, counts,
+                       uniform_count, outD, outI);
+    return (int)hipGetLastError();
+}
+
+int run_fill(double *p, int64_t n, double v, hipStream_t st)
+{
+    int64_t blocks = (n + NT - 1) / NT;
+    if (blocks > 65536) blocks = 65536;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(k_fill, dim3((unsigned)blocks), dim3(NT), 0, st, p, n, v);
+    return (int)hipGetLastError();
+}
+
+int run_corner_extremes(const MaskParams &mp, int N, int K, double bw, double2 *partials, int nblocks, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_shell_corner_extremes, dim3(nblocks), dim3(NT), 0, st, mp, N, K, bw, partials);
+    return (int)hipGetLastError();
+}
+
+}  // namespace ps
