@@ -1,0 +1,216 @@
+#!/usr/bin/env python3
+"""Benchmark: bispectrum triangle configurations per second at 512^3.
+
+Workload: a 512^3 float32 log-normal density field (delta = exp(g - 1/2) - 1,
+g a seeded Gaussian random field with P(k) ~ k^-1.5, sigma = 1), box 300
+Mpc/h; 4 pairs k1 = k2 in {0.1, 0.4, 0.7, 1.0} h/Mpc x 32 theta bins
+(theta_i = pi i / 32, kvectors.c:172-182) = 128 triangles, GIVEN_BINNING with
+cosThetaBinwidth = 0.05 (run.c:68-81), kbinwidth = 1.
+
+A step is one evaluation of all 128 triangles (B and Npolygons) by the batched
+mixed-precision path (FP32 FFTs, FP64 compensated sums), replayed from its
+captured hipGraph, with the field's spectrum already resident in HBM.  Beside
+it the script measures:
+  * the reference-shaped FP64 path (ps_polyspectrum, one triangle per call)
+    on a sample of triangles, and the batched FP64 path on all 128 (the
+    accuracy reference for the mixed path);
+  * per-stage HIP-event times of the batch (the roofline of the dominant
+    kernel) and of the forward transform;
+  * the CPU baseline: the oracle's FP64 restatement of the reference
+    algorithm (C2C FFTs, full-grid masks, products and sums) on one host core.
+
+Prints ONE JSON line.  Only one GPU: --gpus > 1 exits with a message.
+"""
+import argparse
+import ctypes
+import json
+import math
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "bispectrum triangles/sec at 512³ (mixed vs FP64 plain, ≤1e-5 rel err); HBM GB/s"
+
+
+def triangles(pairs=(0.1, 0.4, 0.7, 1.0), ntheta=32, dcos=0.05):
+    """k-lists as run.c builds them for GIVEN binning (kvectors.c:184-247)."""
+    polys = []
+    for k in pairs:
+        for i in range(ntheta):
+            th = math.pi * i / ntheta
+            k3 = math.sqrt(k * k + k * k - 2. * k * k * math.cos(th))
+            lo = math.sqrt(k * k + k * k - 2. * k * k * min(1., math.cos(th) + 0.5 * dcos))
+            hi = math.sqrt(k * k + k * k - 2. * k * k * max(-1., math.cos(th) - 0.5 * dcos))
+            polys.append({"n": 3, "k": [k, k, k3], "kmin": [k, k, lo], "kmax": [k, k, hi]})
+    return polys
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def cpu_baseline(N, L):
+    """Time the oracle (FP64 restatement of polyspectrum.c) on one core: one
+    backward C2C FFT, one mask, one complex product and one complex sum of a
+    512^3 grid; per triangle the reference's polyspectrum() performs 6 FFTs,
+    3 masks, 9 products and 6 sums (polyspectrum.c:51-161, incl. the 4
+    diagnostic sums of :142)."""
+    import oracle as O
+    lib = O.lib()
+    dp = ctypes.POINTER(ctypes.c_double)
+    n3 = N ** 3
+    a = np.empty(2 * n3, dtype=np.float64)
+    a[0::2] = np.random.default_rng(1).standard_normal(n3)
+    a[1::2] = 0.0
+    b = a.copy()
+    mask = np.empty(n3, dtype=np.uint8)
+    t0 = time.perf_counter()
+    lib.or_fft3d(N, a.ctypes.data_as(dp), 1)
+    t_fft = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    lo = ctypes.c_double(0.0)
+    hi = ctypes.c_double(0.0)
+    lib.or_kfilter(N, 0, 0.7, ctypes.byref(lo), ctypes.byref(hi), 1.0, L, mask.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)))
+    t_mask = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    lib.or_product(n3, a.ctypes.data_as(dp), b.ctypes.data_as(dp), b.ctypes.data_as(dp))
+    t_prod = time.perf_counter() - t0
+    s = np.zeros(2)
+    t0 = time.perf_counter()
+    lib.or_sum(n3, b.ctypes.data_as(dp), s.ctypes.data_as(dp))
+    t_sum = time.perf_counter() - t0
+    per_tri = 6 * t_fft + 3 * t_mask + 9 * t_prod + 6 * t_sum
+    return {
+        "value": 1.0 / per_tri,
+        "unit": "triangles/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": (f"oracle/ps_oracle.c on 1 core of '{cpu_model()}' (nproc={os.cpu_count()}): one {N}^3 FP64 C2C "
+                   f"backward FFT {t_fft:.2f} s, mask {t_mask:.2f} s, complex product {t_prod:.2f} s, complex sum "
+                   f"{t_sum:.2f} s, composed as 6 FFT + 3 mask + 9 product + 6 sum = {per_tri:.1f} s per triangle "
+                   f"(polyspectrum.c:51-161)"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--size", type=int, default=512)
+    ap.add_argument("--plain-samples", type=int, default=4)
+    ap.add_argument("--skip-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: the estimator is not sharded across GPUs in this build "
+              "(run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import polyspectrum_amd as P
+    from polyspectrum_amd.synthetic import lognormal_field
+
+    N = args.size
+    L = 300.0 * N / 512
+    polys = triangles()
+    ntri = len(polys)
+    print(f"[bench] generating {N}^3 log-normal field", file=sys.stderr, flush=True)
+    field = lognormal_field(N, 2000, sigma=1.0, alpha=1.5)
+
+    # ---- mixed-precision batched path (the measured value)
+    ctx = P.Context(N, L, P.PREC_MIXED)
+    ctx.load_field(field)
+    batch = ctx.batch(polys, 1.0, P.BINNING["GIVEN_BINNING"])
+    for _ in range(args.warmup):
+        batch.execute()
+    print(f"[bench] timing {args.steps} steps", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        vals32, npol32, _ = batch.execute()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = 1e3 * elapsed / args.steps
+    value = ntri * args.steps / elapsed
+
+    # ---- per-stage HIP-event times (dominant kernel) and the forward FFT
+    stage_ms, stage_bytes = batch.profile(reps=3)
+    names = ["G1 masked gather + ky inverse", "G2 kx inverse (band -> all x)", "Zc fused C2R + product-reduce",
+             "final reduce"]
+    dom = int(np.argmax(stage_ms))
+    fwd_ms, fwd_bytes = ctx.forward_profile(reps=3)
+    stages = {names[i]: {"ms": round(float(stage_ms[i]), 4), "GB": round(float(stage_bytes[i]) / 1e9, 4),
+                         "GB/s": round(float(stage_bytes[i] / stage_ms[i] / 1e6), 1)} for i in range(4)}
+    batch.close()
+    ctx.close()
+
+    # ---- FP64: batched (accuracy reference) and reference-shaped sample
+    print("[bench] FP64 paths", file=sys.stderr, flush=True)
+    c64 = P.Context(N, L, P.PREC_FP64)
+    c64.load_field(field)
+    b64 = c64.batch(polys, 1.0, P.BINNING["GIVEN_BINNING"])
+    vals64, npol64, _ = b64.execute()
+    b64_ms = b64.last_ms
+    b64.close()
+    sample = [polys[i] for i in np.linspace(0, ntri - 1, args.plain_samples).astype(int)]
+    plain = []
+    t0 = time.perf_counter()
+    for p in sample:
+        plain.append(c64.polyspectrum(3, p["k"], p["kmin"], p["kmax"], 1.0, P.BINNING["GIVEN_BINNING"])[0])
+    plain_s = (time.perf_counter() - t0) / len(sample)
+    c64.close()
+    idx = np.linspace(0, ntri - 1, args.plain_samples).astype(int)
+    plain_vs_b64 = float(np.max(np.abs(np.array(plain) - vals64[idx]) / np.abs(vals64[idx])))
+    rel = np.abs(vals32 - vals64) / np.abs(vals64)
+    counts_equal = bool(np.all(np.round(npol32) == np.round(npol64)))
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "triangles/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 FFT/products, f64 compensated sums",
+        "data": "synthetic (seeded log-normal field generated on the host)",
+        "config": {"workload": "equilateral-pair bispectrum sweep: 512^3 log-normal DENS field, L=300 Mpc/h, "
+                               "k1=k2 in {0.1,0.4,0.7,1.0} h/Mpc x 32 theta, GIVEN_BINNING dcos=0.05, kbinwidth=1",
+                   "grid": N, "boxsize": L, "triangles_per_step": ntri},
+        "roofline": {"bound": "hbm", "kernel": names[dom],
+                     "achieved": round(float(stage_bytes[dom] / stage_ms[dom] / 1e6), 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(float(stage_bytes[dom] / stage_ms[dom] / 1e6 / HBM_PEAK_GBS), 4),
+                     "traffic": None},
+        "stages": stages,
+        "forward_fft": {"ms": round(fwd_ms, 4), "GB": round(fwd_bytes / 1e9, 4),
+                        "GB/s": round(fwd_bytes / fwd_ms / 1e6, 1),
+                        "frac": round(fwd_bytes / fwd_ms / 1e6 / HBM_PEAK_GBS, 4)},
+        "accuracy": {"mixed_vs_fp64_rel_err_max": float(rel.max()), "median": float(np.median(rel)),
+                     "fraction_le_1e-5": float(np.mean(rel <= 1e-5)), "npolygons_equal": counts_equal,
+                     "plain_fp64_vs_batched_fp64_max": plain_vs_b64},
+        "fp64_plain": {"triangles_per_s": round(1.0 / plain_s, 3), "sampled_triangles": len(sample),
+                       "speedup_mixed_batched": round(value * plain_s, 1)},
+        "fp64_batched": {"triangles_per_s": round(ntri / (b64_ms / 1e3), 1)},
+    }
+    if not args.skip_cpu:
+        print("[bench] CPU baseline (oracle, 1 core)", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(N, L)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

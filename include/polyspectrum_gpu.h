@@ -129,6 +129,16 @@ double ps_batch_last_ms(const ps_batch *b);
 double ps_batch_model_bytes(const ps_batch *b);
 int ps_batch_destroy(ps_batch *b);
 
+/* Run the batch stage by stage (not from the graph) reps times with HIP
+ * events between stages; stage_ms[4] receives the mean time of the G1, G2,
+ * Zc (fused product-reduce) and final-reduce launches, stage_bytes[4]
+ * (optional) the algorithmic HBM bytes of each. */
+int ps_batch_profile(ps_batch *b, int reps, double *stage_ms, double *stage_bytes);
+
+/* Mean time (ms) of reps forward transforms of a slot's last upload and the
+ * algorithmic bytes of one (3 passes, each reading and writing its array). */
+int ps_forward_profile(ps_ctx *ctx, int slot, int reps, double *ms, double *bytes);
+
 /* Re-run the forward transform of a slot from the last host upload (timing
  * of the FFT passes alone). */
 int ps_forward_again(ps_ctx *ctx, int slot);

@@ -83,6 +83,8 @@ def load_library():
     L.ps_batch_destroy.argtypes = [_vp]
     L.ps_forward_again.argtypes = [_vp, ctypes.c_int]
     L.ps_get_spectrum.argtypes = [_vp, ctypes.c_int, _vp]
+    L.ps_batch_profile.argtypes = [_vp, ctypes.c_int, _dp, _dp]
+    L.ps_forward_profile.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _dp, _dp]
     L.polyspectrum.restype = ctypes.c_double
     L.polyspectrum.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int, _dp, _dp, _dp,
                                ctypes.c_double, ctypes.c_int, ctypes.c_double]
@@ -171,6 +173,13 @@ class Context:
         _check(load_library().ps_get_spectrum(self._h, slot, out.ctypes.data_as(_vp)))
         return out
 
+    def forward_profile(self, reps=5, slot=0):
+        """(mean ms, algorithmic bytes) of one forward R2C transform."""
+        ms = ctypes.c_double()
+        by = ctypes.c_double()
+        _check(load_library().ps_forward_profile(self._h, slot, reps, ctypes.byref(ms), ctypes.byref(by)))
+        return ms.value, by.value
+
     def forward_again(self, slot=0):
         _check(load_library().ps_forward_again(self._h, slot))
 
@@ -237,6 +246,13 @@ class Batch:
         _check(load_library().ps_batch_execute(self._h, v.ctypes.data_as(_dp), npol.ctypes.data_as(_dp),
                                                s.ctypes.data_as(_dp)))
         return v, npol, s.reshape(self.n, 2)
+
+    def profile(self, reps=3):
+        """Per-stage (G1, G2, Zc, reduce) mean ms and algorithmic bytes."""
+        ms = np.zeros(4)
+        by = np.zeros(4)
+        _check(load_library().ps_batch_profile(self._h, reps, ms.ctypes.data_as(_dp), by.ctypes.data_as(_dp)))
+        return ms, by
 
     @property
     def last_ms(self):

@@ -453,6 +453,7 @@ struct ps_batch {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0.0;
     double model_bytes = 0.0;
+    double stage_bytes[4] = {0, 0, 0, 0};  // G1, G2, Zc, reduce
 };
 
 extern "C" int ps_batch_destroy(ps_batch *b)
@@ -553,9 +554,10 @@ extern "C" int ps_batch_create(ps_ctx *c, ps_polygon *polys, int npoly, double b
         L.g2d = p2;
         L.g2i = (char *)p2 + cs * n2;
         b->legs.push_back(L);
-        // model bytes: G1 reads the selected spectrum cells (bounded by the
-        // band) and writes D,I band lines; G2 reads them and writes D,I planes
-        bytes += (double)cs * ((double)(K + 1) * L.Bx * (2 * K + 1) + 2.0 * n1 + 2.0 * n1 + 2.0 * n2);
+        // model bytes: G1 reads the spectrum cells of the band box and writes
+        // the D,I band lines; G2 reads them and writes the D,I planes
+        b->stage_bytes[0] += (double)cs * ((double)(K + 1) * L.Bx * std::min(2 * K + 1, N) + 2.0 * n1);
+        b->stage_bytes[1] += (double)cs * (2.0 * n1 + 2.0 * n2);
     }
     // 3. groups: polygons sharing their prefix legs
     std::map<std::tuple<int, int, int>, std::vector<int>> bykey;  // (npre-kind, a, b) -> polygons
@@ -614,10 +616,13 @@ extern "C" int ps_batch_create(ps_ctx *c, ps_polygon *polys, int npoly, double b
             if (G.npre >= 1) planes += b->legs[G.preA].Kz + 1;
             if (G.npre == 2) planes += b->legs[G.preB].Kz + 1;
             for (size_t q = s0; q < s1; ++q) planes += b->legs[b->suffix[G.suf_begin + (q - s0)]].Kz + 1;
-            bytes += 2.0 * cs * (double)planes * N * N;
+            b->stage_bytes[2] += 2.0 * cs * (double)planes * N * N;
         }
     }
-    b->model_bytes = bytes;
+    b->stage_bytes[3] = 32.0 * (double)b->nout * b->max_chunks;
+    b->stage_bytes[2] += b->stage_bytes[3];
+    b->model_bytes = b->stage_bytes[0] + b->stage_bytes[1] + b->stage_bytes[2] + b->stage_bytes[3];
+    (void)bytes;
     // 4. device tables
     int st = PS_OK;
 #define BCK(call)                                       \
@@ -689,6 +694,74 @@ extern "C" int ps_batch_execute(ps_batch *b, double *values, double *npolygons, 
 }
 
 extern "C" double ps_batch_last_ms(const ps_batch *b) { return b ? b->last_ms : -1.0; }
+
+extern "C" int ps_batch_profile(ps_batch *b, int reps, double *stage_ms, double *stage_bytes)
+{
+    if (!b || reps < 1 || !stage_ms) return PS_ERR_INVALID;
+    ps_ctx *c = b->c;
+    HIPCK(hipSetDevice(c->device));
+    hipEvent_t ev[5];
+    for (auto &e : ev) HIPCK(hipEventCreate(&e));
+    double acc[4] = {0, 0, 0, 0};
+    int st = PS_OK;
+    for (int r = 0; r < reps && st == PS_OK; ++r) {
+        (void)hipEventRecord(ev[0], c->st);
+        st = run_inverse_g(c->fp64, c->N, b->dlegs, (int)b->legs.size(), b->g1_tiles, 0, b->grid_cap, c->tw, c->st);
+        (void)hipEventRecord(ev[1], c->st);
+        if (st == PS_OK)
+            st = run_inverse_g(c->fp64, c->N, b->dlegs, (int)b->legs.size(), 0, b->g2_tiles, b->grid_cap, c->tw, c->st);
+        (void)hipEventRecord(ev[2], c->st);
+        if (st == PS_OK)
+            st = run_zc_fused(c->fp64, c->N, b->dgroups, (int)b->groups.size(), b->dlegs, b->dsuffix, b->items,
+                              b->dpartials, b->max_chunks, b->grid_cap, c->tw, c->st);
+        (void)hipEventRecord(ev[3], c->st);
+        if (st == PS_OK)
+            st = run_reduce_partials(b->dpartials, b->max_chunks, b->dcounts, 0, b->nout, b->dsum,
+                                     b->dsum + b->nout, c->st);
+        (void)hipEventRecord(ev[4], c->st);
+        if (hipEventSynchronize(ev[4]) != hipSuccess) st = PS_ERR_HIP;
+        for (int i = 0; i < 4 && st == PS_OK; ++i) {
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, ev[i], ev[i + 1]);
+            acc[i] += ms;
+        }
+    }
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    if (st != PS_OK) return st > 0 ? PS_ERR_HIP : st;
+    for (int i = 0; i < 4; ++i) {
+        stage_ms[i] = acc[i] / reps;
+        if (stage_bytes) stage_bytes[i] = b->stage_bytes[i];
+    }
+    return PS_OK;
+}
+
+extern "C" int ps_forward_profile(ps_ctx *c, int slot, int reps, double *ms, double *bytes)
+{
+    if (!c || reps < 1 || !ms || slot < 0 || slot >= c->nslots) return PS_ERR_INVALID;
+    if (!c->staging || !c->loaded[slot]) return PS_ERR_STATE;
+    HIPCK(hipSetDevice(c->device));
+    hipEvent_t e0, e1;
+    HIPCK(hipEventCreate(&e0));
+    HIPCK(hipEventCreate(&e1));
+    (void)hipEventRecord(e0, c->st);
+    int st = PS_OK;
+    for (int r = 0; r < reps && st == PS_OK; ++r) st = forward(c, slot, c->staging, c->staging_is_double);
+    (void)hipEventRecord(e1, c->st);
+    (void)hipEventSynchronize(e1);
+    float t = 0.f;
+    (void)hipEventElapsedTime(&t, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (st != PS_OK) return st > 0 ? PS_ERR_HIP : st;
+    *ms = t / reps;
+    if (bytes) {
+        // R2C model (three passes, each reading and writing its array once):
+        // s*N^3 read + cs*NH*N^2 written, then 2 x (read + write) of the half spectrum
+        const double n3 = (double)c->cells(), half = (double)c->half_elems();
+        *bytes = (c->staging_is_double ? 8.0 : 4.0) * n3 + c->csize() * half * 5.0;
+    }
+    return PS_OK;
+}
 extern "C" double ps_batch_model_bytes(const ps_batch *b) { return b ? b->model_bytes : -1.0; }
 
 // ---------------------------------------------------------------- drop-in symbols

@@ -8,7 +8,7 @@
 
 namespace ps {
 
-constexpr int MAX_SUFFIX = 64;  // suffix legs per fused group (per-wave accumulators in LDS)
+constexpr int MAX_SUFFIX = 32;  // suffix legs per fused group (per-wave accumulators in LDS)
 
 // lines (or rows) per LDS tile: 16 FP32 / 8 FP64 lines of N <= 512 fit 64 KB
 __host__ __device__ constexpr int tile_lines_for(int N, bool fp64)

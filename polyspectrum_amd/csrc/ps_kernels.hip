@@ -25,6 +25,7 @@
 #include "ps_fft_core.hpp"
 #include "ps_internal.h"
 #include "ps_mask.hpp"
+#include "ps_wave_fft.hpp"
 
 namespace ps {
 
@@ -315,11 +316,14 @@ __global__ void __launch_bounds__(NT) k_inv_zc_plain(const cplx<T> *__restrict__
     }
 }
 
-// Zc for batches of polygons: tile = (group, x, y0..y0+TLC).  For each row the
-// C2R of (D + i I) of every leg runs in LDS; the prefix product P of the
-// shared legs stays in registers in FP64 (the product of two FP32 values is
-// exact there) and every suffix leg t adds sum_z P * D_t and sum_z P_I * I_t
-// to compensated per-wavefront accumulators.  A work item is a chunk of
+// Zc for batches of polygons: tile = (group, x, ROWS adjacent y rows).  For
+// every leg the workgroup stages the leg's planes kz <= Kz of the tile
+// (coalesced, ROWS adjacent values per plane); then each wavefront runs the
+// C2R of (D + i I) of its LW rows with wavefront-private exchanges.  The
+// prefix product P of the shared legs stays in registers in FP64 (the product
+// of two FP32 values is exact there); every suffix leg t adds
+// sum_z P*D_t and sum_z P_I*I_t, tree-summed over the wavefront and
+// accumulated with compensation per wavefront.  A work item is a chunk of
 // consecutive tiles of one group; its partial sums go to
 // partials[out][chunk] as (sum, compensation).
 template <typename T, int N>
@@ -327,83 +331,99 @@ __global__ void __launch_bounds__(NT) k_inv_zc_fused(const GroupDev *__restrict_
                                                      const LegDev *__restrict__ legs,
                                                      const int *__restrict__ suffix, int64_t nitems,
                                                      double2 *__restrict__ partials, int max_chunks,
-                                                     const cplx<T> *__restrict__ tw)
+                                                     const cplx<T> *__restrict__ twg)
 {
-    constexpr int TLC = tile_lines<T, N>() / 2;  // rows per tile
-    constexpr int EPT = (TLC * N + NT - 1) / NT;  // elements per thread
-    constexpr int NW = NT / 64;
-    __shared__ cplx<T> lds[N * TLC];
-    __shared__ double acc[NW][MAX_SUFFIX][4];
+    constexpr int LW = wave_lines<T, N>();
+    constexpr int NWV = NT / 64;
+    constexpr int ROWS = NWV * LW;
+    constexpr int SP = ROWS + 1;  // staging pitch
+    constexpr int NH = N / 2 + 1;
+    constexpr int LP = wave_pitch<N>();
+    using WO = WaveOut<T, N, LW>;
+    __shared__ cplx<T> stD[NH * SP], stI[NH * SP];
+    __shared__ cplx<T> wbuf[NWV * LW * LP];
+    __shared__ double acc[NWV][MAX_SUFFIX][4];
     const int tid = threadIdx.x, w = tid / 64, lane = tid & 63;
+    const cplx<T> *__restrict__ tw = twg;
+    cplx<T> *wb = wbuf + w * LW * LP;
+    constexpr int tiles_per_x = N / ROWS;
+    const int64_t ntile = (int64_t)N * tiles_per_x;
     for (int64_t item = blockIdx.x; item < nitems; item += gridDim.x) {
         const int gi = find_owner(groups, ngroups, item, &GroupDev::item_begin);
         const GroupDev &G = groups[gi];
         const int chunk = (int)(item - G.item_begin);
-        constexpr int tiles_per_x = N / TLC;
         const int64_t t_begin = (int64_t)chunk * G.chunk_tiles;
-        int64_t t_end = t_begin + G.chunk_tiles;
-        const int64_t ntile = (int64_t)N * tiles_per_x;
-        if (t_end > ntile) t_end = ntile;
-        for (int i = tid; i < NW * MAX_SUFFIX * 4; i += NT) (&acc[0][0][0])[i] = 0.0;
+        const int64_t t_end = (t_begin + G.chunk_tiles < ntile) ? t_begin + G.chunk_tiles : ntile;
+        for (int i = tid; i < NWV * MAX_SUFFIX * 4; i += NT) (&acc[0][0][0])[i] = 0.0;
         __syncthreads();
         for (int64_t t = t_begin; t < t_end; ++t) {
             const int x = (int)(t / tiles_per_x);
-            const int y0 = (int)(t % tiles_per_x) * TLC;
-            double pd[EPT], pi[EPT];
+            const int y0 = (int)(t % tiles_per_x) * ROWS;
+            double pd[WO::IT][WO::R], pi[WO::IT][WO::R];
 #pragma unroll
-            for (int e = 0; e < EPT; ++e) { pd[e] = 1.0; pi[e] = 1.0; }
-            for (int p = 0; p < G.npre; ++p) {
-                const LegDev &L = legs[p == 0 ? G.preA : G.preB];
-                load_zline_pair<T, N, TLC>(lds, reinterpret_cast<const cplx<T> *>(L.g2d),
-                                           reinterpret_cast<const cplx<T> *>(L.g2i), L.Kz, x, y0, tid);
-                __syncthreads();
-                fft_lds<T, N, TLC, +1>(lds, tw, tid);
+            for (int i = 0; i < WO::IT; ++i)
 #pragma unroll
-                for (int e = 0; e < EPT; ++e) {
-                    const int idx = tid + e * NT;
-                    if (idx < TLC * N) {
-                        const cplx<T> v = lds[lidx<TLC>(idx % N, idx / N)];
-                        const double dv = (double)v.x, iv = (double)v.y;
-                        if (G.pre_sq) { pd[e] *= dv * dv; pi[e] *= iv * iv; }
-                        else { pd[e] *= dv; pi[e] *= iv; }
-                    }
+                for (int r = 0; r < WO::R; ++r) { pd[i][r] = 1.0; pi[i][r] = 1.0; }
+            const int nleg = G.npre + G.nsuf;
+            for (int q = 0; q < nleg; ++q) {
+                const bool pre = q < G.npre;
+                const LegDev &L = legs[pre ? (q == 0 ? G.preA : G.preB) : suffix[G.suf_begin + q - G.npre]];
+                const int Kz = L.Kz;
+                const cplx<T> *gd = reinterpret_cast<const cplx<T> *>(L.g2d);
+                const cplx<T> *gi = reinterpret_cast<const cplx<T> *>(L.g2i);
+                for (int idx = tid; idx < (Kz + 1) * ROWS; idx += NT) {
+                    const int row = idx % ROWS, s = idx / ROWS;
+                    const int64_t o = ((int64_t)s * N + x) * N + y0 + row;
+                    stD[s * SP + row] = gd ? gd[o] : mk<T>(0, 0);
+                    stI[s * SP + row] = gi ? gi[o] : mk<T>(0, 0);
                 }
                 __syncthreads();
-            }
-            for (int s = 0; s < G.nsuf; ++s) {
-                const LegDev &L = legs[suffix[G.suf_begin + s]];
-                load_zline_pair<T, N, TLC>(lds, reinterpret_cast<const cplx<T> *>(L.g2d),
-                                           reinterpret_cast<const cplx<T> *>(L.g2i), L.Kz, x, y0, tid);
-                __syncthreads();
-                fft_lds<T, N, TLC, +1>(lds, tw, tid);
-                double sd = 0.0, si = 0.0;
-#pragma unroll
-                for (int e = 0; e < EPT; ++e) {
-                    const int idx = tid + e * NT;
-                    if (idx < TLC * N) {
-                        const cplx<T> v = lds[lidx<TLC>(idx % N, idx / N)];
-                        const double dv = (double)v.x, iv = (double)v.y;
-                        if (G.suf_sq) { sd += pd[e] * dv * dv; si += pi[e] * iv * iv; }
-                        else { sd += pd[e] * dv; si += pi[e] * iv; }
+                auto src = [&](int l, int e) -> cplx<T> {
+                    const int row = w * LW + l;
+                    const int s = (e <= N / 2) ? e : N - e;
+                    cplx<T> X = mk<T>(0, 0), Y = mk<T>(0, 0);
+                    if (s <= Kz) {
+                        X = stD[s * SP + row];
+                        Y = stI[s * SP + row];
                     }
-                }
-                double sdc = 0.0, sic = 0.0;
-                wave_sum2(sd, sdc);
-                wave_sum2(si, sic);
-                if (lane == 0) {
-                    KSum a; a.s = acc[w][s][0]; a.c = acc[w][s][1]; a.add(sd); a.add(sdc);
-                    KSum b; b.s = acc[w][s][2]; b.c = acc[w][s][3]; b.add(si); b.add(sic);
-                    acc[w][s][0] = a.s; acc[w][s][1] = a.c; acc[w][s][2] = b.s; acc[w][s][3] = b.c;
+                    if (e == 0 || e == N / 2) return mk<T>(X.x, Y.x);
+                    if (e < N / 2) return mk<T>(X.x - Y.y, X.y + Y.x);
+                    return mk<T>(X.x + Y.y, Y.x - X.y);
+                };
+                if (pre) {
+                    const bool sq = G.pre_sq != 0;
+                    auto sink = [&](int i, int r, int, int, cplx<T> v) {
+                        const double dv = (double)v.x, iv = (double)v.y;
+                        pd[i][r] *= sq ? dv * dv : dv;
+                        pi[i][r] *= sq ? iv * iv : iv;
+                    };
+                    wave_fft<T, N, LW, +1, 0>(wb, tw, lane, src, sink);
+                } else {
+                    double sd = 0.0, si = 0.0;
+                    const bool sq = G.suf_sq != 0;
+                    auto sink = [&](int i, int r, int, int, cplx<T> v) {
+                        const double dv = (double)v.x, iv = (double)v.y;
+                        sd += pd[i][r] * (sq ? dv * dv : dv);
+                        si += pi[i][r] * (sq ? iv * iv : iv);
+                    };
+                    wave_fft<T, N, LW, +1, 0>(wb, tw, lane, src, sink);
+                    sd = wave_sum(sd);
+                    si = wave_sum(si);
+                    if (lane == 0) {
+                        const int s = q - G.npre;
+                        KSum a; a.s = acc[w][s][0]; a.c = acc[w][s][1]; a.add(sd);
+                        KSum b; b.s = acc[w][s][2]; b.c = acc[w][s][3]; b.add(si);
+                        acc[w][s][0] = a.s; acc[w][s][1] = a.c; acc[w][s][2] = b.s; acc[w][s][3] = b.c;
+                    }
                 }
                 __syncthreads();
             }
         }
-        __syncthreads();
         for (int s = tid; s < G.nsuf; s += NT) {
             KSum a, b;
             a.init();
             b.init();
-            for (int ww = 0; ww < NW; ++ww) {
+            for (int ww = 0; ww < NWV; ++ww) {
                 a.add(acc[ww][s][0]); a.add(acc[ww][s][1]);
                 b.add(acc[ww][s][2]); b.add(acc[ww][s][3]);
             }
@@ -615,7 +635,15 @@ int supported_grid(int N)
 
 int tile_rows(int N, int fp64) { return tile_lines_for(N, fp64 != 0); }
 
-int fused_tile_rows(int N, int fp64) { return tile_rows(N, fp64) / 2; }
+int fused_tile_rows(int N, int fp64)
+{
+    switch (N) {
+#define CASE(n) case n: return 4 * (fp64 ? wave_lines<double, n>() : wave_lines<float, n>());
+        PS_FOR_EACH_N(CASE)
+#undef CASE
+    }
+    return 0;
+}
 int g_tile_lines(int N, int fp64) { return tile_rows(N, fp64) / 2; }
 
 int run_forward(int fp64, int N, const void *field, int field_is_double, void *H, const void *tw, hipStream_t st)

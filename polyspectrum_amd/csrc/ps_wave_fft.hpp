@@ -1,0 +1,123 @@
+// ps_wave_fft.hpp -- wavefront-private FFTs: one wavefront transforms LW
+// whole lines of length N.  Butterfly b = lane + 64*i of a stage works on line
+// b / (N/R); lines are exchanged between stages through the wavefront's own
+// LDS region, so no workgroup barrier is needed inside a transform.  The first
+// stage takes its inputs from a producer functor (e.g. a Hermitian view of a
+// staged half spectrum) and the last stage hands its outputs to a consumer
+// functor in registers; every thread sees the same output positions for every
+// transform, which lets per-cell products stay in registers across lines.
+#pragma once
+#include "ps_fft_core.hpp"
+
+namespace ps {
+
+// the LDS write -> other-lane read hand-off of one wavefront
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+constexpr int largest_divisor_leq(int n, int cap)
+{
+    for (int d = cap; d >= 1; --d)
+        if (n % d == 0) return d;
+    return 1;
+}
+
+// lines per wavefront: about 1024 complex values in flight, dividing N/4 so
+// that 4 wavefronts cover a whole number of rows of a tile
+template <typename T, int N> constexpr int wave_lines()
+{
+    constexpr int cap = (1024 / N < 1) ? 1 : 1024 / N;
+    constexpr int d = largest_divisor_leq(N / 4, cap);
+    return (sizeof(T) == 8 && d > 1) ? d / 2 : d;
+}
+
+template <int N> constexpr int wave_pitch() { return N + N / 8 + 1; }
+__device__ __forceinline__ int wpad(int e) { return e + (e >> 3); }
+
+template <typename T, int N, int LW, int S> struct WaveStage {
+    static constexpr int R = stage_radix(N, S);
+    static constexpr int NS = stage_span(N, S);
+    static constexpr int NB = N / R;
+    static constexpr int TOT = LW * NB;
+    static constexpr int IT = (TOT + 63) / 64;
+    static constexpr bool LAST = (S == num_stages(N) - 1);
+};
+
+// outputs each thread holds after the last stage
+template <typename T, int N, int LW> struct WaveOut {
+    using L = WaveStage<T, N, LW, num_stages(N) - 1>;
+    static constexpr int IT = L::IT;
+    static constexpr int R = L::R;
+    static constexpr int NB = L::NB;
+    static constexpr int TOT = L::TOT;
+};
+
+// Inverse (SIGN = +1) or forward (SIGN = -1) transform of LW lines.
+//   src(l, e)          -> input element e of line l (first stage)
+//   sink(i, r, l, z, v) <- output z of line l (held as v[i][r] of this thread)
+template <typename T, int N, int LW, int SIGN, int S, class Src, class Sink>
+__device__ __forceinline__ void wave_fft(cplx<T> *__restrict__ wbuf, const cplx<T> *__restrict__ tw, int lane,
+                                         Src &src, Sink &sink)
+{
+    using St = WaveStage<T, N, LW, S>;
+    constexpr int R = St::R, NS = St::NS, NB = St::NB, TOT = St::TOT, IT = St::IT;
+    constexpr int LP = wave_pitch<N>();
+    cplx<T> v[IT][R];
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+        const int b = lane + 64 * i;
+        if (TOT % 64 == 0 || b < TOT) {
+            const int j = b % NB, l = b / NB;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                if constexpr (S == 0) v[i][r] = src(l, j + r * NB);
+                else v[i][r] = wbuf[l * LP + wpad(j + r * NB)];
+            }
+            if constexpr (NS > 1) {
+                const int k = j % NS;
+#pragma unroll
+                for (int r = 1; r < R; ++r) v[i][r] = cmul<T>(v[i][r], twiddle<T, SIGN>(tw, r * k * (N / (NS * R))));
+            }
+            dft<T, R, SIGN>(v[i]);
+        }
+    }
+    if constexpr (St::LAST) {
+#pragma unroll
+        for (int i = 0; i < IT; ++i) {
+            const int b = lane + 64 * i;
+            if (TOT % 64 == 0 || b < TOT) {
+                const int j = b % NB, l = b / NB;
+#pragma unroll
+                for (int r = 0; r < R; ++r) sink(i, r, l, j + r * NB, v[i][r]);
+            }
+        }
+    } else {
+        wave_sync();
+#pragma unroll
+        for (int i = 0; i < IT; ++i) {
+            const int b = lane + 64 * i;
+            if (TOT % 64 == 0 || b < TOT) {
+                const int j = b % NB, l = b / NB;
+                const int d = (j / NS) * NS * R + j % NS;
+#pragma unroll
+                for (int r = 0; r < R; ++r) wbuf[l * LP + wpad(d + r * NS)] = v[i][r];
+            }
+        }
+        wave_sync();
+        wave_fft<T, N, LW, SIGN, S + 1>(wbuf, tw, lane, src, sink);
+    }
+}
+
+// plain FP64 tree sum over a wavefront (valid in every lane)
+__device__ __forceinline__ double wave_sum(double s)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off, 64);
+    return s;
+}
+
+}  // namespace ps
