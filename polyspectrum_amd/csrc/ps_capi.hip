@@ -367,7 +367,7 @@ static int plain_leg(ps_ctx *c, const MaskParams &mp, const void *spec, int mode
     HIPCK(hipMemcpyAsync(c->dleg, &L, sizeof(LegDev), hipMemcpyHostToDevice, c->st));
     const int tl = g_tile_lines(N, c->fp64);
     const int64_t g1 = (int64_t)(N / 2 + 1) * ((N + tl - 1) / tl);
-    const int64_t g2 = (int64_t)(N / 2 + 1) * (N / tl);
+    const int64_t g2 = 2 * (int64_t)(N / 2 + 1) * (N / tile_rows(N, c->fp64));
     PSCK(run_inverse_g(c->fp64, N, c->dleg, 1, g1, g2, 1 << 30, c->tw, c->st));
     PSCK(run_zc_plain(c->fp64, N, L.g2d, L.g2i, N / 2, spec ? c->accD : nullptr, c->accI, mode, c->partials, c->tw,
                       c->st));
@@ -441,11 +441,13 @@ struct ps_batch {
     int64_t g1_tiles = 0, g2_tiles = 0, items = 0;
     int max_chunks = 1;
     int grid_cap = 2048;
+    int kz_cap = 1;                      // 1 + largest Kz of the legs
     std::vector<void *> allocs;
     LegDev *dlegs = nullptr;
     GroupDev *dgroups = nullptr;
     int *dsuffix = nullptr;
     int *dcounts = nullptr;
+    double *dscale = nullptr;            // per-leg amplitude scales (mixed precision)
     double2 *dpartials = nullptr;
     double *dsum = nullptr;              // 2*nout
     hipGraph_t graph = nullptr;
@@ -465,7 +467,7 @@ extern "C" int ps_batch_destroy(ps_batch *b)
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
     for (void *p : b->allocs) (void)hipFree(p);
-    void *bufs[] = {b->dlegs, b->dgroups, b->dsuffix, b->dcounts, b->dpartials, b->dsum};
+    void *bufs[] = {b->dlegs, b->dgroups, b->dsuffix, b->dcounts, b->dpartials, b->dsum, b->dscale};
     for (void *p : bufs) if (p) (void)hipFree(p);
     delete b;
     return PS_OK;
@@ -479,10 +481,11 @@ static bool same_mask(const MaskParams &a, const MaskParams &b)
 static int batch_record(ps_batch *b)
 {
     ps_ctx *c = b->c;
+    if (b->dscale) PSCK(run_leg_scale(c->fp64, c->N, b->dlegs, (int)b->legs.size(), c->st));
     PSCK(run_inverse_g(c->fp64, c->N, b->dlegs, (int)b->legs.size(), b->g1_tiles, b->g2_tiles, b->grid_cap, c->tw,
                        c->st));
     PSCK(run_zc_fused(c->fp64, c->N, b->dgroups, (int)b->groups.size(), b->dlegs, b->dsuffix, b->items, b->dpartials,
-                      b->max_chunks, b->grid_cap, c->tw, c->st));
+                      b->max_chunks, b->grid_cap, b->kz_cap, c->tw, c->st));
     PSCK(run_reduce_partials(b->dpartials, b->max_chunks, b->dcounts, 0, b->nout, b->dsum, b->dsum + b->nout, c->st));
     return PS_OK;
 }
@@ -535,11 +538,12 @@ extern "C" int ps_batch_create(ps_ctx *c, ps_polygon *polys, int npoly, double b
         const int K = shell_band(kv.second, N);
         L.Kz = K;
         L.Kx = K;
+        b->kz_cap = std::max(b->kz_cap, K + 1);
         L.Bx = std::min(2 * K + 1, N);
         L.g1_begin = b->g1_tiles;
         L.g2_begin = b->g2_tiles;
         b->g1_tiles += (int64_t)(K + 1) * ((L.Bx + tl - 1) / tl);
-        b->g2_tiles += (int64_t)(K + 1) * (N / tl);
+        b->g2_tiles += 2 * (int64_t)(K + 1) * (N / tile_rows(N, c->fp64));
         const size_t n1 = (size_t)(K + 1) * L.Bx * N, n2 = (size_t)(K + 1) * N * N;
         void *p1 = nullptr, *p2 = nullptr;
         if (hipMalloc(&p1, cs * n1 * 2) != hipSuccess || hipMalloc(&p2, cs * n2 * 2) != hipSuccess) {
@@ -629,6 +633,10 @@ extern "C" int ps_batch_create(ps_ctx *c, ps_polygon *polys, int npoly, double b
     do {                                                \
         if ((call) != hipSuccess) { st = PS_ERR_HIP; goto fail; } \
     } while (0)
+    if (!c->fp64) {
+        BCK(hipMalloc(&b->dscale, sizeof(double) * b->legs.size()));
+        for (size_t i = 0; i < b->legs.size(); ++i) b->legs[i].scale = b->dscale + i;
+    }
     BCK(hipMalloc(&b->dlegs, sizeof(LegDev) * b->legs.size()));
     BCK(hipMalloc(&b->dgroups, sizeof(GroupDev) * b->groups.size()));
     BCK(hipMalloc(&b->dsuffix, sizeof(int) * b->suffix.size()));
@@ -706,14 +714,17 @@ extern "C" int ps_batch_profile(ps_batch *b, int reps, double *stage_ms, double 
     int st = PS_OK;
     for (int r = 0; r < reps && st == PS_OK; ++r) {
         (void)hipEventRecord(ev[0], c->st);
-        st = run_inverse_g(c->fp64, c->N, b->dlegs, (int)b->legs.size(), b->g1_tiles, 0, b->grid_cap, c->tw, c->st);
+        if (b->dscale) st = run_leg_scale(c->fp64, c->N, b->dlegs, (int)b->legs.size(), c->st);
+        if (st == PS_OK)
+            st = run_inverse_g(c->fp64, c->N, b->dlegs, (int)b->legs.size(), b->g1_tiles, 0, b->grid_cap, c->tw,
+                               c->st);
         (void)hipEventRecord(ev[1], c->st);
         if (st == PS_OK)
             st = run_inverse_g(c->fp64, c->N, b->dlegs, (int)b->legs.size(), 0, b->g2_tiles, b->grid_cap, c->tw, c->st);
         (void)hipEventRecord(ev[2], c->st);
         if (st == PS_OK)
             st = run_zc_fused(c->fp64, c->N, b->dgroups, (int)b->groups.size(), b->dlegs, b->dsuffix, b->items,
-                              b->dpartials, b->max_chunks, b->grid_cap, c->tw, c->st);
+                              b->dpartials, b->max_chunks, b->grid_cap, b->kz_cap, c->tw, c->st);
         (void)hipEventRecord(ev[3], c->st);
         if (st == PS_OK)
             st = run_reduce_partials(b->dpartials, b->max_chunks, b->dcounts, 0, b->nout, b->dsum,

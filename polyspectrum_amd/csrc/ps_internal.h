@@ -29,6 +29,7 @@ struct LegDev {
     int64_t g2_begin;  // first tile of this leg in the G2 pass
     void *g1d, *g1i;   // band buffers [Kz+1][Bx][N] (D part may be null)
     void *g2d, *g2i;   // full buffers [Kz+1][N][N]
+    double *scale;     // device: rms |spectrum| over the shell (D part enters G1 divided by it)
 };
 
 // A batch of polygons sharing a prefix product: output s of the group is
@@ -57,10 +58,12 @@ int run_inverse_g(int fp64, int N, const LegDev *legs, int nlegs, int64_t g1_til
 int run_zc_plain(int fp64, int N, const void *gd, const void *gi, int Kz, double *accD, double *accI, int mode,
                  double2 *partials, const void *tw, hipStream_t st);
 int run_zc_fused(int fp64, int N, const GroupDev *groups, int ngroups, const LegDev *legs, const int *suffix,
-                 int64_t nitems, double2 *partials, int max_chunks, int grid_cap, const void *tw, hipStream_t st);
+                 int64_t nitems, double2 *partials, int max_chunks, int grid_cap, int kz_cap, const void *tw,
+                 hipStream_t st);
 int run_reduce_partials(const double2 *partials, int64_t stride, const int *counts, int64_t uniform_count,
                         int64_t nout, double *outD, double *outI, hipStream_t st);
 int run_fill(double *p, int64_t n, double v, hipStream_t st);
+int run_leg_scale(int fp64, int N, const LegDev *legs, int nlegs, hipStream_t st);
 int run_corner_extremes(const MaskParams &mp, int N, int K, double bw, double2 *partials, int nblocks, hipStream_t st);
 
 }  // namespace ps

@@ -146,6 +146,7 @@ __global__ void __launch_bounds__(NT) k_inv_g1(const LegDev *__restrict__ legs, 
         const int j0 = (int)(local % nbt) * TLC;
         const int mz = lattice(kz, N);
         const cplx<T> *spec = reinterpret_cast<const cplx<T> *>(L.spec);
+        const T inv_scale = L.scale ? (T)(1.0 / *L.scale) : (T)1;
         for (int idx = tid; idx < TLC * N; idx += NT) {
             const int l = idx / N, ky = idx % N;
             const int j = j0 + l;
@@ -161,6 +162,7 @@ __global__ void __launch_bounds__(NT) k_inv_g1(const LegDev *__restrict__ legs, 
                             const T a = hypot(d.x, d.y);
                             d = mk<T>(d.x / a, d.y / a);
                         }
+                        d = mk<T>(d.x * inv_scale, d.y * inv_scale);
                     }
                 }
             }
@@ -184,45 +186,41 @@ __global__ void __launch_bounds__(NT) k_inv_g1(const LegDev *__restrict__ legs, 
     }
 }
 
-// G2: C2C along kx from the band to every x.  Tile = (leg, kz, y tile).
+// G2: C2C along kx from the band to every x.  Tile = (leg, part, kz, y tile
+// of TLC adjacent columns), part 0 = D, 1 = I; stores are TLC-value segments.
 template <typename T, int N>
 __global__ void __launch_bounds__(NT) k_inv_g2(const LegDev *__restrict__ legs, int nlegs, int64_t ntiles,
                                                const cplx<T> *__restrict__ tw)
 {
-    constexpr int TLC = tile_lines<T, N>() / 2;
-    constexpr int TL2 = 2 * TLC;
-    __shared__ cplx<T> lds[N * TL2];
+    constexpr int TLC = tile_lines<T, N>();
+    __shared__ cplx<T> lds[N * TLC];
     const int tid = threadIdx.x;
+    constexpr int nyt = N / TLC;
     for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         const int li = find_owner(legs, nlegs, tile, &LegDev::g2_begin);
         const LegDev &L = legs[li];
-        constexpr int nyt = N / TLC;
         const int64_t local = tile - L.g2_begin;
-        const int kz = (int)(local / nyt);
-        const int y0 = (int)(local % nyt) * TLC;
-        const cplx<T> *gd = reinterpret_cast<const cplx<T> *>(L.g1d);
-        const cplx<T> *gi = reinterpret_cast<const cplx<T> *>(L.g1i);
+        const int64_t per_part = (int64_t)(L.Kz + 1) * nyt;
+        const int part = (int)(local / per_part);
+        const int64_t rest = local - part * per_part;
+        const int kz = (int)(rest / nyt);
+        const int y0 = (int)(rest % nyt) * TLC;
+        const cplx<T> *g = reinterpret_cast<const cplx<T> *>(part ? L.g1i : L.g1d);
+        cplx<T> *o = reinterpret_cast<cplx<T> *>(part ? L.g2i : L.g2d);
         for (int idx = tid; idx < TLC * N; idx += NT) {
             const int l = idx % TLC, kx = idx / TLC;
             const int j = k_to_band(kx, L.Kx, L.Bx, N);
-            cplx<T> d = mk<T>(0, 0), u = mk<T>(0, 0);
-            if (j >= 0) {
-                const int64_t o = ((int64_t)kz * L.Bx + j) * N + y0 + l;
-                if (gd) d = gd[o];
-                u = gi[o];
-            }
-            lds[lidx<TL2>(kx, l)] = d;
-            lds[lidx<TL2>(kx, l + TLC)] = u;
+            cplx<T> d = mk<T>(0, 0);
+            if (j >= 0 && g) d = g[((int64_t)kz * L.Bx + j) * N + y0 + l];
+            lds[lidx<TLC>(kx, l)] = d;
         }
         __syncthreads();
-        fft_lds<T, N, TL2, +1>(lds, tw, tid);
-        cplx<T> *od = reinterpret_cast<cplx<T> *>(L.g2d);
-        cplx<T> *oi = reinterpret_cast<cplx<T> *>(L.g2i);
-        for (int idx = tid; idx < TLC * N; idx += NT) {
-            const int l = idx % TLC, x = idx / TLC;
-            const int64_t o = ((int64_t)kz * N + x) * N + y0 + l;
-            if (od) od[o] = lds[lidx<TL2>(x, l)];
-            oi[o] = lds[lidx<TL2>(x, l + TLC)];
+        fft_lds<T, N, TLC, +1>(lds, tw, tid);
+        if (o) {
+            for (int idx = tid; idx < TLC * N; idx += NT) {
+                const int l = idx % TLC, x = idx / TLC;
+                o[((int64_t)kz * N + x) * N + y0 + l] = lds[lidx<TLC>(x, l)];
+            }
         }
         __syncthreads();
     }
@@ -318,33 +316,46 @@ __global__ void __launch_bounds__(NT) k_inv_zc_plain(const cplx<T> *__restrict__
 
 // Zc for batches of polygons: tile = (group, x, ROWS adjacent y rows).  For
 // every leg the workgroup stages the leg's planes kz <= Kz of the tile
-// (coalesced, ROWS adjacent values per plane); then each wavefront runs the
-// C2R of (D + i I) of its LW rows with wavefront-private exchanges.  The
-// prefix product P of the shared legs stays in registers in FP64 (the product
-// of two FP32 values is exact there); every suffix leg t adds
-// sum_z P*D_t and sum_z P_I*I_t, tree-summed over the wavefront and
-// accumulated with compensation per wavefront.  A work item is a chunk of
-// consecutive tiles of one group; its partial sums go to
-// partials[out][chunk] as (sum, compensation).
+// (coalesced, ROWS adjacent values per plane) in one of two LDS buffers
+// while the wavefronts transform the previous leg from the other buffer:
+// each wavefront runs the C2R of (D + i I) of its LW rows with
+// wavefront-private exchanges.  The prefix product P of the shared legs stays
+// in registers in FP64 (the product of two FP32 values is exact there); every
+// suffix leg t adds sum_z P*D_t and sum_z P_I*I_t, tree-summed over the
+// wavefront and accumulated with compensation per wavefront.  A work item is
+// a chunk of consecutive tiles of one group; its partial sums go to
+// partials[out][chunk] as (sum, compensation).  Dynamic LDS: two staging
+// buffers of kz_cap planes (kz_cap = 1 + the largest Kz of the batch).
+template <typename T, int N> struct ZcCfg {
+    static constexpr int LW = wave_lines<T, N>();
+    static constexpr int NWV = NT / 64;
+    static constexpr int ROWS = NWV * LW;
+    static constexpr int SP = ROWS + 1;  // staging pitch
+    static constexpr int NH = N / 2 + 1;
+    // staged values per thread when prefetching (legs with more planes are
+    // staged without overlap)
+    static constexpr int MPT = ((NH * ROWS + NT - 1) / NT < 4) ? (NH * ROWS + NT - 1) / NT : 4;
+};
+
 template <typename T, int N>
 __global__ void __launch_bounds__(NT) k_inv_zc_fused(const GroupDev *__restrict__ groups, int ngroups,
                                                      const LegDev *__restrict__ legs,
                                                      const int *__restrict__ suffix, int64_t nitems,
                                                      double2 *__restrict__ partials, int max_chunks,
-                                                     const cplx<T> *__restrict__ twg)
+                                                     const cplx<T> *__restrict__ twg, int kz_cap)
 {
-    constexpr int LW = wave_lines<T, N>();
-    constexpr int NWV = NT / 64;
-    constexpr int ROWS = NWV * LW;
-    constexpr int SP = ROWS + 1;  // staging pitch
-    constexpr int NH = N / 2 + 1;
+    using C = ZcCfg<T, N>;
+    constexpr int LW = C::LW, NWV = C::NWV, ROWS = C::ROWS, SP = C::SP, MPT = C::MPT;
     constexpr int LP = wave_pitch<N>();
     using WO = WaveOut<T, N, LW>;
-    __shared__ cplx<T> stD[NH * SP], stI[NH * SP];
+    extern __shared__ unsigned char dyn_lds[];
     __shared__ cplx<T> wbuf[NWV * LW * LP];
     __shared__ double acc[NWV][MAX_SUFFIX][4];
+    cplx<T> *stage_buf = reinterpret_cast<cplx<T> *>(dyn_lds);  // [2 buffers][2 parts][kz_cap * SP]
+    const int part = kz_cap * SP;
     const int tid = threadIdx.x, w = tid / 64, lane = tid & 63;
-    const cplx<T> *__restrict__ tw = twg;
+    WaveTw<T, N, LW> tw;
+    tw.template load<+1>(twg, lane);
     cplx<T> *wb = wbuf + w * LW * LP;
     constexpr int tiles_per_x = N / ROWS;
     const int64_t ntile = (int64_t)N * tiles_per_x;
@@ -354,84 +365,190 @@ __global__ void __launch_bounds__(NT) k_inv_zc_fused(const GroupDev *__restrict_
         const int chunk = (int)(item - G.item_begin);
         const int64_t t_begin = (int64_t)chunk * G.chunk_tiles;
         const int64_t t_end = (t_begin + G.chunk_tiles < ntile) ? t_begin + G.chunk_tiles : ntile;
+        const int nleg = G.npre + G.nsuf;
+        const int64_t nsteps = (t_end - t_begin) * nleg;  // (tile, leg) steps of this item
         for (int i = tid; i < NWV * MAX_SUFFIX * 4; i += NT) (&acc[0][0][0])[i] = 0.0;
-        __syncthreads();
-        for (int64_t t = t_begin; t < t_end; ++t) {
-            const int x = (int)(t / tiles_per_x);
-            const int y0 = (int)(t % tiles_per_x) * ROWS;
-            double pd[WO::IT][WO::R], pi[WO::IT][WO::R];
+        // staged copy of step s = (tile, leg) into buffer b
+        auto leg_of = [&](int q) -> const LegDev & {
+            return legs[q < G.npre ? (q == 0 ? G.preA : G.preB) : suffix[G.suf_begin + q - G.npre]];
+        };
+        cplx<T> pfD[MPT], pfI[MPT];
+        // step (t, q) = (tile, leg): registers for up to MPT*NT staged values,
+        // the rest (legs with many planes) copied directly at store time
+        auto fetch = [&](int64_t t, int q) {
+            const LegDev &L = leg_of(q);
+            const int x = (int)(t / tiles_per_x), y0 = (int)(t % tiles_per_x) * ROWS;
+            const int cnt = (L.Kz + 1) * ROWS;
+            const cplx<T> *gd = reinterpret_cast<const cplx<T> *>(L.g2d);
+            const cplx<T> *gi = reinterpret_cast<const cplx<T> *>(L.g2i);
 #pragma unroll
-            for (int i = 0; i < WO::IT; ++i)
-#pragma unroll
-                for (int r = 0; r < WO::R; ++r) { pd[i][r] = 1.0; pi[i][r] = 1.0; }
-            const int nleg = G.npre + G.nsuf;
-            for (int q = 0; q < nleg; ++q) {
-                const bool pre = q < G.npre;
-                const LegDev &L = legs[pre ? (q == 0 ? G.preA : G.preB) : suffix[G.suf_begin + q - G.npre]];
-                const int Kz = L.Kz;
-                const cplx<T> *gd = reinterpret_cast<const cplx<T> *>(L.g2d);
-                const cplx<T> *gi = reinterpret_cast<const cplx<T> *>(L.g2i);
-                for (int idx = tid; idx < (Kz + 1) * ROWS; idx += NT) {
-                    const int row = idx % ROWS, s = idx / ROWS;
-                    const int64_t o = ((int64_t)s * N + x) * N + y0 + row;
-                    stD[s * SP + row] = gd ? gd[o] : mk<T>(0, 0);
-                    stI[s * SP + row] = gi ? gi[o] : mk<T>(0, 0);
+            for (int m = 0; m < MPT; ++m) {
+                const int idx = tid + m * NT;
+                pfD[m] = mk<T>(0, 0);
+                pfI[m] = mk<T>(0, 0);
+                if (idx < cnt) {
+                    const int row = idx % ROWS, kz = idx / ROWS;
+                    const int64_t o = ((int64_t)kz * N + x) * N + y0 + row;
+                    if (gd) pfD[m] = gd[o];
+                    if (gi) pfI[m] = gi[o];
                 }
-                __syncthreads();
-                auto src = [&](int l, int e) -> cplx<T> {
-                    const int row = w * LW + l;
-                    const int s = (e <= N / 2) ? e : N - e;
-                    cplx<T> X = mk<T>(0, 0), Y = mk<T>(0, 0);
-                    if (s <= Kz) {
-                        X = stD[s * SP + row];
-                        Y = stI[s * SP + row];
-                    }
-                    if (e == 0 || e == N / 2) return mk<T>(X.x, Y.x);
-                    if (e < N / 2) return mk<T>(X.x - Y.y, X.y + Y.x);
-                    return mk<T>(X.x + Y.y, Y.x - X.y);
-                };
-                if (pre) {
-                    const bool sq = G.pre_sq != 0;
-                    auto sink = [&](int i, int r, int, int, cplx<T> v) {
-                        const double dv = (double)v.x, iv = (double)v.y;
-                        pd[i][r] *= sq ? dv * dv : dv;
-                        pi[i][r] *= sq ? iv * iv : iv;
-                    };
-                    wave_fft<T, N, LW, +1, 0>(wb, tw, lane, src, sink);
+            }
+        };
+        auto store = [&](int64_t t, int q, int b) {
+            const LegDev &L = leg_of(q);
+            const int x = (int)(t / tiles_per_x), y0 = (int)(t % tiles_per_x) * ROWS;
+            const int cnt = (L.Kz + 1) * ROWS;
+            cplx<T> *sd = stage_buf + (2 * b) * part, *si = stage_buf + (2 * b + 1) * part;
+            // the z line of row y is Z[k] = X[k] + i Y[k] for k <= N/2 and
+            // Z[N-k] = conj X[k] + i conj Y[k]; only real parts at k = 0, N/2
+            auto put = [&](int kz, int row, cplx<T> X, cplx<T> Y) {
+                if (kz == 0 || kz == N / 2) {
+                    sd[kz * SP + row] = mk<T>(X.x, Y.x);
                 } else {
-                    double sd = 0.0, si = 0.0;
-                    const bool sq = G.suf_sq != 0;
-                    auto sink = [&](int i, int r, int, int, cplx<T> v) {
-                        const double dv = (double)v.x, iv = (double)v.y;
-                        sd += pd[i][r] * (sq ? dv * dv : dv);
-                        si += pi[i][r] * (sq ? iv * iv : iv);
-                    };
-                    wave_fft<T, N, LW, +1, 0>(wb, tw, lane, src, sink);
-                    sd = wave_sum(sd);
-                    si = wave_sum(si);
-                    if (lane == 0) {
-                        const int s = q - G.npre;
-                        KSum a; a.s = acc[w][s][0]; a.c = acc[w][s][1]; a.add(sd);
-                        KSum b; b.s = acc[w][s][2]; b.c = acc[w][s][3]; b.add(si);
-                        acc[w][s][0] = a.s; acc[w][s][1] = a.c; acc[w][s][2] = b.s; acc[w][s][3] = b.c;
-                    }
+                    sd[kz * SP + row] = mk<T>(X.x - Y.y, X.y + Y.x);
+                    si[kz * SP + row] = mk<T>(X.x + Y.y, Y.x - X.y);
                 }
-                __syncthreads();
+            };
+#pragma unroll
+            for (int m = 0; m < MPT; ++m) {
+                const int idx = tid + m * NT;
+                if (idx < cnt) put(idx / ROWS, idx % ROWS, pfD[m], pfI[m]);
             }
+            const cplx<T> *gd = reinterpret_cast<const cplx<T> *>(L.g2d);
+            const cplx<T> *gi = reinterpret_cast<const cplx<T> *>(L.g2i);
+            for (int idx = tid + MPT * NT; idx < cnt; idx += NT) {
+                const int row = idx % ROWS, kz = idx / ROWS;
+                const int64_t o = ((int64_t)kz * N + x) * N + y0 + row;
+                put(kz, row, gd ? gd[o] : mk<T>(0, 0), gi ? gi[o] : mk<T>(0, 0));
+            }
+        };
+        fetch(t_begin, 0);
+        store(t_begin, 0, 0);
+        __syncthreads();
+        int q = 0;
+        int64_t tcur = t_begin;
+        // prefix product of the shared legs per output cell, in the precision
+        // of the transforms; each suffix product is then formed in FP64
+        T pd[WO::IT][WO::R], pi[WO::IT][WO::R];
+        for (int64_t s = 0; s < nsteps; ++s) {
+            const int b = (int)(s & 1);
+            const int qn = (q + 1 == nleg) ? 0 : q + 1;
+            const int64_t tn = (qn == 0) ? tcur + 1 : tcur;
+            if (s + 1 < nsteps) fetch(tn, qn);
+            if (q == 0) {
+#pragma unroll
+                for (int i = 0; i < WO::IT; ++i)
+#pragma unroll
+                    for (int r = 0; r < WO::R; ++r) { pd[i][r] = (T)1; pi[i][r] = (T)1; }
+            }
+            const int Kz = leg_of(q).Kz;
+            const cplx<T> *sD = stage_buf + (2 * b) * part, *sI = stage_buf + (2 * b + 1) * part;
+            auto src = [&](int l, int e) -> cplx<T> {
+                const int row = w * LW + l;
+                const bool lo = e <= N / 2;
+                const int k = lo ? e : N - e;
+                if (k > Kz) return mk<T>(0, 0);
+                return (lo ? sD : sI)[k * SP + row];
+            };
+            if (q < G.npre) {
+                const bool sq = G.pre_sq != 0;
+                auto sink = [&](int i, int r, int, int, cplx<T> v) {
+                    pd[i][r] *= sq ? v.x * v.x : v.x;
+                    pi[i][r] *= sq ? v.y * v.y : v.y;
+                };
+                if (Kz < N / stage_radix(N, 0)) wave_fft<T, N, LW, +1, 0, true>(wb, tw, lane, src, sink);
+                else wave_fft<T, N, LW, +1, 0, false>(wb, tw, lane, src, sink);
+            } else {
+                double sd = 0.0, si = 0.0;
+                const bool sq = G.suf_sq != 0;
+                auto sink = [&](int i, int r, int, int, cplx<T> v) {
+                    const double dv = (double)v.x, iv = (double)v.y;
+                    sd += (double)pd[i][r] * (sq ? dv * dv : dv);
+                    si += (double)pi[i][r] * (sq ? iv * iv : iv);
+                };
+                if (Kz < N / stage_radix(N, 0)) wave_fft<T, N, LW, +1, 0, true>(wb, tw, lane, src, sink);
+                else wave_fft<T, N, LW, +1, 0, false>(wb, tw, lane, src, sink);
+                sd = wave_sum(sd);
+                si = wave_sum(si);
+                if (lane == 0) {
+                    const int o = q - G.npre;
+                    KSum a; a.s = acc[w][o][0]; a.c = acc[w][o][1]; a.add(sd);
+                    KSum c; c.s = acc[w][o][2]; c.c = acc[w][o][3]; c.add(si);
+                    acc[w][o][0] = a.s; acc[w][o][1] = a.c; acc[w][o][2] = c.s; acc[w][o][3] = c.c;
+                }
+            }
+            if (s + 1 < nsteps) store(tn, qn, b ^ 1);
+            __syncthreads();
+            q = qn;
+            tcur = tn;
         }
-        for (int s = tid; s < G.nsuf; s += NT) {
-            KSum a, b;
+        for (int o = tid; o < G.nsuf; o += NT) {
+            KSum a, c;
             a.init();
-            b.init();
+            c.init();
             for (int ww = 0; ww < NWV; ++ww) {
-                a.add(acc[ww][s][0]); a.add(acc[ww][s][1]);
-                b.add(acc[ww][s][2]); b.add(acc[ww][s][3]);
+                a.add(acc[ww][o][0]); a.add(acc[ww][o][1]);
+                c.add(acc[ww][o][2]); c.add(acc[ww][o][3]);
             }
-            const int64_t o = (int64_t)(G.out_begin + s) * max_chunks + chunk;
-            partials[2 * o + 0] = make_double2(a.s, a.c);
-            partials[2 * o + 1] = make_double2(b.s, b.c);
+            // undo the legs' amplitude scales (D parts entered G1 divided by them)
+            double f = 1.0;
+            if (G.npre >= 1) {
+                const double sa = legs[G.preA].scale ? *legs[G.preA].scale : 1.0;
+                f *= G.pre_sq ? sa * sa : sa;
+            }
+            if (G.npre == 2 && legs[G.preB].scale) f *= *legs[G.preB].scale;
+            const LegDev &Ls = legs[suffix[G.suf_begin + o]];
+            const double st = Ls.scale ? *Ls.scale : 1.0;
+            f *= G.suf_sq ? st * st : st;
+            const int64_t po = (int64_t)(G.out_begin + o) * max_chunks + chunk;
+            partials[2 * po + 0] = make_double2(a.s * f, a.c * f);
+            partials[2 * po + 1] = make_double2(c.s, c.c);
         }
         __syncthreads();
+    }
+}
+
+// Amplitude scale of each leg: the rms of |spectrum| over (a sample of at
+// most ~16k) cells of its shell.  Packing D + i I into one complex transform
+// in the last pass gives both parts the rounding noise of the larger one;
+// dividing the D part by this scale keeps both near unit amplitude.
+template <typename T, int N>
+__global__ void __launch_bounds__(NT) k_leg_scale(const LegDev *__restrict__ legs)
+{
+    __shared__ double red[2][NT];
+    const int tid = threadIdx.x;
+    const LegDev &L = legs[blockIdx.x];
+    if (!L.scale) return;
+    const cplx<T> *spec = reinterpret_cast<const cplx<T> *>(L.spec);
+    double pw = 0.0, cnt = 0.0;
+    if (spec && !L.phase_only) {
+        const int B = L.Bx;
+        const int64_t total = (int64_t)(L.Kz + 1) * B * B;
+        const int64_t step = (total > 16384) ? total / 16384 : 1;
+        for (int64_t c = (int64_t)tid * step; c < total; c += (int64_t)NT * step) {
+            const int kz = (int)(c / ((int64_t)B * B));
+            const int jx = (int)((c / B) % B), jy = (int)(c % B);
+            const int kx = band_to_k(jx, L.Kx, B, N), ky = band_to_k(jy, L.Kx, B, N);
+            if (mask_eval(L.mask, lattice(kx, N), lattice(ky, N), lattice(kz, N))) {
+                const cplx<T> d = spec[((int64_t)kz * N + kx) * N + ky];
+                pw += (double)d.x * (double)d.x + (double)d.y * (double)d.y;
+                cnt += 1.0;
+            }
+        }
+    }
+    red[0][tid] = pw;
+    red[1][tid] = cnt;
+    __syncthreads();
+    for (int h = NT / 2; h > 0; h >>= 1) {
+        if (tid < h) {
+            red[0][tid] += red[0][tid + h];
+            red[1][tid] += red[1][tid + h];
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        const double sc = (red[1][0] > 0.0 && red[0][0] > 0.0) ? sqrt(red[0][0] / red[1][0]) : 1.0;
+        *L.scale = (sc > 0.0 && sc < 1e300) ? sc : 1.0;
     }
 }
 
@@ -562,12 +679,18 @@ static int launch_zc_plain_n(const void *gd, const void *gi, int Kz, double *acc
 
 template <typename T, int N>
 static int launch_zc_fused_n(const GroupDev *groups, int ngroups, const LegDev *legs, const int *suffix,
-                             int64_t nitems, double2 *partials, int max_chunks, int grid_cap, const void *tw,
-                             hipStream_t st)
+                             int64_t nitems, double2 *partials, int max_chunks, int grid_cap, int kz_cap,
+                             const void *tw, hipStream_t st)
 {
+    using C = ZcCfg<T, N>;
+    if (kz_cap < 1 || kz_cap > C::NH) kz_cap = C::NH;
+    const size_t dyn = sizeof(cplx<T>) * 4 * (size_t)kz_cap * C::SP;
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_inv_zc_fused<T, N>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
+    if (e != hipSuccess) return (int)e;
     const unsigned g = (unsigned)((nitems < grid_cap) ? nitems : grid_cap);
-    hipLaunchKernelGGL((k_inv_zc_fused<T, N>), dim3(g), dim3(NT), 0, st, groups, ngroups, legs, suffix, nitems,
-                       partials, max_chunks, reinterpret_cast<const cplx<T> *>(tw));
+    hipLaunchKernelGGL((k_inv_zc_fused<T, N>), dim3(g), dim3(NT), dyn, st, groups, ngroups, legs, suffix, nitems,
+                       partials, max_chunks, reinterpret_cast<const cplx<T> *>(tw), kz_cap);
     return (int)hipGetLastError();
 }
 
@@ -610,11 +733,11 @@ static int dispatch_zc_plain(int N, const void *gd, const void *gi, int Kz, doub
 
 template <typename T>
 static int dispatch_zc_fused(int N, const GroupDev *groups, int ngroups, const LegDev *legs, const int *suffix,
-                             int64_t nitems, double2 *partials, int max_chunks, int cap, const void *tw,
+                             int64_t nitems, double2 *partials, int max_chunks, int cap, int kz_cap, const void *tw,
                              hipStream_t st)
 {
     switch (N) {
-#define CASE(n) case n: return launch_zc_fused_n<T, n>(groups, ngroups, legs, suffix, nitems, partials, max_chunks, cap, tw, st);
+#define CASE(n) case n: return launch_zc_fused_n<T, n>(groups, ngroups, legs, suffix, nitems, partials, max_chunks, cap, kz_cap, tw, st);
         PS_FOR_EACH_N(CASE)
 #undef CASE
     }
@@ -667,10 +790,11 @@ int run_zc_plain(int fp64, int N, const void *gd, const void *gi, int Kz, double
 }
 
 int run_zc_fused(int fp64, int N, const GroupDev *groups, int ngroups, const LegDev *legs, const int *suffix,
-                 int64_t nitems, double2 *partials, int max_chunks, int grid_cap, const void *tw, hipStream_t st)
+                 int64_t nitems, double2 *partials, int max_chunks, int grid_cap, int kz_cap, const void *tw,
+                 hipStream_t st)
 {
-    return fp64 ? dispatch_zc_fused<double>(N, groups, ngroups, legs, suffix, nitems, partials, max_chunks, grid_cap, tw, st)
-                : dispatch_zc_fused<float>(N, groups, ngroups, legs, suffix, nitems, partials, max_chunks, grid_cap, tw, st);
+    return fp64 ? dispatch_zc_fused<double>(N, groups, ngroups, legs, suffix, nitems, partials, max_chunks, grid_cap, kz_cap, tw, st)
+                : dispatch_zc_fused<float>(N, groups, ngroups, legs, suffix, nitems, partials, max_chunks, grid_cap, kz_cap, tw, st);
 }
 
 int run_reduce_partials(const double2 *partials, int64_t stride, const int *counts, int64_t uniform_count,
@@ -689,6 +813,23 @@ int run_fill(double *p, int64_t n, double v, hipStream_t st)
     if (blocks < 1) blocks = 1;
     hipLaunchKernelGGL(k_fill, dim3((unsigned)blocks), dim3(NT), 0, st, p, n, v);
     return (int)hipGetLastError();
+}
+
+template <typename T>
+static int dispatch_leg_scale(int N, const LegDev *legs, int nlegs, hipStream_t st)
+{
+    switch (N) {
+#define CASE(n) case n: hipLaunchKernelGGL((k_leg_scale<T, n>), dim3(nlegs), dim3(NT), 0, st, legs); return (int)hipGetLastError();
+        PS_FOR_EACH_N(CASE)
+#undef CASE
+    }
+    return PS_ERR_UNSUPPORTED_N;
+}
+
+int run_leg_scale(int fp64, int N, const LegDev *legs, int nlegs, hipStream_t st)
+{
+    if (nlegs <= 0) return 0;
+    return fp64 ? dispatch_leg_scale<double>(N, legs, nlegs, st) : dispatch_leg_scale<float>(N, legs, nlegs, st);
 }
 
 int run_corner_extremes(const MaskParams &mp, int N, int K, double bw, double2 *partials, int nblocks, hipStream_t st)

@@ -56,11 +56,56 @@ template <typename T, int N, int LW> struct WaveOut {
     static constexpr int TOT = L::TOT;
 };
 
+// Per-thread twiddles of every stage after the first, loaded once: stage S
+// butterfly i needs W^(r*k*(N/(NS*R))), k = (j mod NS), j = (lane + 64 i) mod NB.
+// Butterflies i that share j share their twiddles (sets(S) distinct sets).
+template <typename T, int N, int LW> struct WaveTw {
+    static constexpr int sets(int S)
+    {
+        const int NB = N / stage_radix(N, S);
+        const int IT = (LW * NB + 63) / 64;
+        const int d = (64 % NB == 0) ? 1 : ((NB % 64 == 0) ? NB / 64 : IT);
+        return d < IT ? d : IT;
+    }
+    static constexpr int off(int S)
+    {
+        int o = 0;
+        for (int s = 1; s < S; ++s) o += sets(s) * (stage_radix(N, s) - 1);
+        return o;
+    }
+    static constexpr int total() { return off(num_stages(N)) > 0 ? off(num_stages(N)) : 1; }
+    template <int S> __device__ static constexpr int at(int i, int r)
+    {
+        return off(S) + (i % sets(S)) * (stage_radix(N, S) - 1) + r - 1;
+    }
+    cplx<T> w[total()];
+
+    template <int SIGN, int S = 1> __device__ __forceinline__ void load(const cplx<T> *__restrict__ tw, int lane)
+    {
+        if constexpr (S < num_stages(N)) {
+            using St = WaveStage<T, N, LW, S>;
+#pragma unroll
+            for (int i = 0; i < sets(S); ++i) {
+                const int b = lane + 64 * i;
+                const int j = (b < St::TOT ? b : St::TOT - 1) % St::NB;
+                const int k = j % St::NS;
+#pragma unroll
+                for (int r = 1; r < St::R; ++r)
+                    w[at<S>(i, r)] = twiddle<T, SIGN>(tw, r * k * (N / (St::NS * St::R)));
+            }
+            load<SIGN, S + 1>(tw, lane);
+        }
+    }
+};
+
 // Inverse (SIGN = +1) or forward (SIGN = -1) transform of LW lines.
 //   src(l, e)          -> input element e of line l (first stage)
 //   sink(i, r, l, z, v) <- output z of line l (held as v[i][r] of this thread)
-template <typename T, int N, int LW, int SIGN, int S, class Src, class Sink>
-__device__ __forceinline__ void wave_fft(cplx<T> *__restrict__ wbuf, const cplx<T> *__restrict__ tw, int lane,
+//   EDGE (first stage only): inputs e with NB <= e < N - NB are known to be
+//   zero (a spectrum band-limited to |k| < N/R_0), so only r = 0 and r = R-1
+//   are read and the butterfly is constant-folded.
+template <typename T, int N, int LW, int SIGN, int S, bool EDGE = false, class Src, class Sink>
+__device__ __forceinline__ void wave_fft(cplx<T> *__restrict__ wbuf, const WaveTw<T, N, LW> &tw, int lane,
                                          Src &src, Sink &sink)
 {
     using St = WaveStage<T, N, LW, S>;
@@ -74,13 +119,17 @@ __device__ __forceinline__ void wave_fft(cplx<T> *__restrict__ wbuf, const cplx<
             const int j = b % NB, l = b / NB;
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                if constexpr (S == 0) v[i][r] = src(l, j + r * NB);
-                else v[i][r] = wbuf[l * LP + wpad(j + r * NB)];
+                if constexpr (S == 0) {
+                    if (EDGE && r != 0 && r != R - 1) v[i][r] = mk<T>(0, 0);
+                    else v[i][r] = src(l, j + r * NB);
+                } else {
+                    v[i][r] = wbuf[l * LP + wpad(j + r * NB)];
+                }
             }
             if constexpr (NS > 1) {
-                const int k = j % NS;
 #pragma unroll
-                for (int r = 1; r < R; ++r) v[i][r] = cmul<T>(v[i][r], twiddle<T, SIGN>(tw, r * k * (N / (NS * R))));
+                for (int r = 1; r < R; ++r)
+                    v[i][r] = cmul<T>(v[i][r], tw.w[WaveTw<T, N, LW>::template at<S>(i, r)]);
             }
             dft<T, R, SIGN>(v[i]);
         }
@@ -108,7 +157,7 @@ __device__ __forceinline__ void wave_fft(cplx<T> *__restrict__ wbuf, const cplx<
             }
         }
         wave_sync();
-        wave_fft<T, N, LW, SIGN, S + 1>(wbuf, tw, lane, src, sink);
+        wave_fft<T, N, LW, SIGN, S + 1, false>(wbuf, tw, lane, src, sink);
     }
 }
 
