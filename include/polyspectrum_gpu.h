@@ -139,6 +139,17 @@ int ps_batch_profile(ps_batch *b, int reps, double *stage_ms, double *stage_byte
  * algorithmic bytes of one (3 passes, each reading and writing its array). */
 int ps_forward_profile(ps_ctx *ctx, int slot, int reps, double *ms, double *bytes);
 
+/* Adjoint: the gradient of sum_p weights[p] * values[p] (values of the last
+ * ps_batch_execute) with respect to the real field of a slot, dL/d delta(x)
+ * for weights[p] = dL/dvalues[p].  For a group of polygons sharing legs A, B
+ * and suffix legs t (coefficients a_t = weights * V^(n-1) N^(-3n) / sum prod I)
+ * it evaluates L_A[D_B E] + L_B[D_A E] + L_W[D_A D_B] with L_X = IFFT * mask_X * FFT,
+ * D_X = L_X delta, W = sum_t a_t mask_t and E = L_W delta: every filtered field
+ * is recomputed, none is stored per polygon.  grad: N^3 values (float64 if
+ * grad_is_double, else float32), host memory unless on_device.  Not available
+ * with useOnlyPhase (PS_ERR_INVALID).  No counterpart in the reference. */
+int ps_batch_adjoint(ps_batch *b, int slot, const double *weights, void *grad, int grad_is_double, int on_device);
+
 /* Re-run the forward transform of a slot from the last host upload (timing
  * of the FFT passes alone). */
 int ps_forward_again(ps_ctx *ctx, int slot);

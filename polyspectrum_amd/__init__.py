@@ -61,6 +61,14 @@ def load_library():
     path = library_path()
     if not os.path.exists(path):
         raise GPUError(f"{path} is not built; run polyspectrum_amd.build() (make -C polyspectrum_amd)")
+    # One HIP runtime per process: torch ships its own libamdhip64.so.7.  When
+    # torch is importable it is loaded first, and the library's dependency on
+    # that soname then resolves to torch's copy instead of a second runtime
+    # (which would leave torch without a device).
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(path)
     L.ps_strerror.restype = ctypes.c_char_p
     L.ps_strerror.argtypes = [ctypes.c_int]
@@ -84,6 +92,7 @@ def load_library():
     L.ps_forward_again.argtypes = [_vp, ctypes.c_int]
     L.ps_get_spectrum.argtypes = [_vp, ctypes.c_int, _vp]
     L.ps_batch_profile.argtypes = [_vp, ctypes.c_int, _dp, _dp]
+    L.ps_batch_adjoint.argtypes = [_vp, ctypes.c_int, _dp, _vp, ctypes.c_int, ctypes.c_int]
     L.ps_forward_profile.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _dp, _dp]
     L.polyspectrum.restype = ctypes.c_double
     L.polyspectrum.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int, _dp, _dp, _dp,
@@ -213,6 +222,11 @@ class Context:
     def batch(self, polygons, kbinwidth=1.0, kbinning_case=0):
         return Batch(self, polygons, kbinwidth, kbinning_case)
 
+    def batch_for_field(self, polygons, field, kbinwidth=1.0, kbinning_case=0, slot=0):
+        """Load a field into the slot, then plan a batch on it."""
+        self.load_field(field, slot)
+        return Batch(self, polygons, kbinwidth, kbinning_case)
+
 
 class Batch:
     """Many configurations evaluated by one captured launch sequence."""
@@ -246,6 +260,23 @@ class Batch:
         _check(load_library().ps_batch_execute(self._h, v.ctypes.data_as(_dp), npol.ctypes.data_as(_dp),
                                                s.ctypes.data_as(_dp)))
         return v, npol, s.reshape(self.n, 2)
+
+    def adjoint(self, weights, slot=0, out=None):
+        """dL/d field for L with dL/dvalues = weights (after execute()).  out:
+        None (returns a float64 numpy N^3 array) or a torch CUDA tensor
+        (float32/float64, N^3 elements) filled in place."""
+        w = np.ascontiguousarray(weights, dtype=np.float64)
+        assert w.size == self.n
+        N = self._ctx.nbins
+        if out is None:
+            g = np.empty((N, N, N), dtype=np.float64)
+            _check(load_library().ps_batch_adjoint(self._h, slot, w.ctypes.data_as(_dp), g.ctypes.data_as(_vp), 1, 0))
+            return g
+        import torch
+        assert out.is_cuda and out.is_contiguous() and out.numel() == N ** 3
+        _check(load_library().ps_batch_adjoint(self._h, slot, w.ctypes.data_as(_dp), ctypes.c_void_p(out.data_ptr()),
+                                               int(out.dtype == torch.float64), 1))
+        return out
 
     def profile(self, reps=3):
         """Per-stage (G1, G2, Zc, reduce) mean ms and algorithmic bytes."""

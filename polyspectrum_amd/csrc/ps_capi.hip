@@ -455,6 +455,9 @@ struct ps_batch {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0.0;
     double model_bytes = 0.0;
+    std::vector<int> leg_slot;           // field slot of every leg
+    std::vector<double> last_sumI;       // sum prod I of every output (last execute)
+    int have_sums = 0;
     double stage_bytes[4] = {0, 0, 0, 0};  // G1, G2, Zc, reduce
 };
 
@@ -526,6 +529,7 @@ extern "C" int ps_batch_create(ps_ctx *c, ps_polygon *polys, int npoly, double b
             pleg[p].push_back(id);
         }
     }
+    for (auto &kv : keys) b->leg_slot.push_back(kv.first);
     // 2. leg descriptors, buffers and tile ranges
     const size_t cs = c->csize();
     const int tl = g_tile_lines(N, c->fp64);
@@ -686,6 +690,8 @@ extern "C" int ps_batch_execute(ps_batch *b, double *values, double *npolygons, 
     float ms = 0.f;
     HIPCK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
     b->last_ms = ms;
+    b->last_sumI.assign(h.begin() + b->nout, h.end());
+    b->have_sums = 1;
     const double volume = c->L * c->L * c->L;
     const double cells = (double)((int64_t)c->N * c->N * c->N);
     for (int p = 0; p < b->npoly; ++p) {
@@ -699,6 +705,163 @@ extern "C" int ps_batch_execute(ps_batch *b, double *values, double *npolygons, 
         }
     }
     return PS_OK;
+}
+
+// ---------------------------------------------------------------- adjoint
+__global__ void k_to_float(float *__restrict__ out, const double *__restrict__ in, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = (float)in[i];
+}
+
+// gradient of sum_p weights[p] * values[p] with respect to the field of a slot
+extern "C" int ps_batch_adjoint(ps_batch *b, int slot, const double *weights, void *grad, int grad_is_double,
+                                int on_device)
+{
+    if (!b || !weights || !grad) return PS_ERR_INVALID;
+    ps_ctx *c = b->c;
+    if (slot < 0 || slot >= c->nslots || !c->loaded[slot]) return PS_ERR_STATE;
+    if (c->phase_only) return PS_ERR_INVALID;  // the phase-only adjoint is not provided
+    if (!b->have_sums) return PS_ERR_STATE;
+    HIPCK(hipSetDevice(c->device));
+    const int N = c->N;
+    const int64_t cells = c->cells(), half = c->half_elems();
+    const size_t hb = c->csize() * (size_t)half, fb = sizeof(double) * (size_t)cells;
+    // coefficient of every output: dL/dS_o = sum_p w_p * c_n / C_o
+    std::vector<double> coef(b->nout, 0.0);
+    const double volume = c->L * c->L * c->L;
+    for (int p = 0; p < b->npoly; ++p) {
+        const int o = b->poly_out[p], n = b->poly_n[p];
+        const double cn = pow(volume, n - 1) * pow(c->N, -n * 3);
+        coef[o] += weights[p] * cn / b->last_sumI[o];
+    }
+    void *H1 = nullptr, *H2 = nullptr;
+    double *DA = nullptr, *DB = nullptr, *E = nullptr, *Q = nullptr, *G = nullptr, *dw = nullptr;
+    MaskParams *dm = nullptr;
+    int st = PS_OK;
+    const int maxm = MAX_SUFFIX + 2;
+#define ACK(call)                                                   \
+    do {                                                            \
+        hipError_t e_ = (call);                                     \
+        if (e_ != hipSuccess) { st = (e_ == hipErrorOutOfMemory) ? PS_ERR_NOMEM : PS_ERR_HIP; goto done; } \
+    } while (0)
+#define RCK(call)                                                   \
+    do {                                                            \
+        int r_ = (call);                                            \
+        if (r_ != PS_OK) { st = (r_ > 0) ? PS_ERR_HIP : r_; goto done; } \
+    } while (0)
+    ACK(hipMalloc(&H1, hb));
+    ACK(hipMalloc(&H2, hb));
+    ACK(hipMalloc(&DA, fb));
+    ACK(hipMalloc(&DB, fb));
+    ACK(hipMalloc(&E, fb));
+    ACK(hipMalloc(&Q, fb));
+    ACK(hipMalloc(&G, fb));
+    ACK(hipMalloc(&dm, sizeof(MaskParams) * maxm * 4));
+    ACK(hipMalloc(&dw, sizeof(double) * maxm * 4));
+    RCK(run_fill(G, cells, 0.0, c->st));
+    for (const GroupDev &g : b->groups) {
+        const int first_leg = b->suffix[g.suf_begin];
+        if (b->leg_slot[first_leg] != slot) continue;
+        // host mask lists: [0] leg A, [1] leg B, [2..] the weighted suffix masks W
+        std::vector<MaskParams> hm;
+        std::vector<double> hw;
+        // power spectra: dS/d delta = 2 L_A L_t delta = 2 N^3 IFFT(F_A F_t FFT delta)
+        // (the unnormalised transforms give IFFT FFT = N^3)
+        const double two = (g.npre == 0 || (g.npre == 1 && !g.pre_sq)) ? 2.0 * (double)cells : 1.0;
+        hm.push_back(g.npre >= 1 ? b->legs[g.preA].mask : MaskParams{});
+        hw.push_back(1.0);
+        hm.push_back(g.npre == 2 ? b->legs[g.preB].mask : MaskParams{});
+        hw.push_back(1.0);
+        for (int q = 0; q < g.nsuf; ++q) {
+            hm.push_back(b->legs[b->suffix[g.suf_begin + q]].mask);
+            hw.push_back(two * coef[g.out_begin + q]);
+        }
+        // slot 0 of the device lists: A (1 mask), slot 1: B, slot 2: W (nsuf masks)
+        MaskParams *mA = dm, *mB = dm + maxm, *mW = dm + 2 * maxm;
+        double *wA = dw, *wB = dw + maxm, *wW = dw + 2 * maxm;
+        ACK(hipMemcpyAsync(mA, &hm[0], sizeof(MaskParams), hipMemcpyHostToDevice, c->st));
+        ACK(hipMemcpyAsync(wA, &hw[0], sizeof(double), hipMemcpyHostToDevice, c->st));
+        ACK(hipMemcpyAsync(mB, &hm[1], sizeof(MaskParams), hipMemcpyHostToDevice, c->st));
+        ACK(hipMemcpyAsync(wB, &hw[1], sizeof(double), hipMemcpyHostToDevice, c->st));
+        ACK(hipMemcpyAsync(mW, &hm[2], sizeof(MaskParams) * g.nsuf, hipMemcpyHostToDevice, c->st));
+        ACK(hipMemcpyAsync(wW, &hw[2], sizeof(double) * g.nsuf, hipMemcpyHostToDevice, c->st));
+        const void *spec = c->spec[slot];
+        auto filtered = [&](const MaskParams *m, const double *w, int nm, double *out) -> int {
+            // out = IFFT(mask * spectrum)
+            hipError_t e = hipMemcpyAsync(H1, spec, hb, hipMemcpyDeviceToDevice, c->st);
+            if (e != hipSuccess) return PS_ERR_HIP;
+            int r = run_apply_masks(c->fp64, N, H1, m, w, nm, c->st);
+            if (r != PS_OK) return r;
+            return run_inverse_real(c->fp64, N, H1, out, 0, c->tw, c->st);
+        };
+        auto project = [&](const double *f, void *H, const MaskParams *m, const double *w, int nm) -> int {
+            // H = mask * FFT(f)
+            int r = run_forward(c->fp64, N, f, 1, H, c->tw, c->st);
+            if (r != PS_OK) return r;
+            return run_apply_masks(c->fp64, N, H, m, w, nm, c->st);
+        };
+        if (g.npre == 0) {
+            // S_t = sum D_t^2: dS_t = 2 L_t delta (masks are idempotent), the
+            // factor 2 is in the weights
+            ACK(hipMemcpyAsync(H1, spec, hb, hipMemcpyDeviceToDevice, c->st));
+            RCK(run_apply_masks(c->fp64, N, H1, mW, wW, g.nsuf, c->st));
+            RCK(run_inverse_real(c->fp64, N, H1, G, 1, c->tw, c->st));
+        } else if (g.npre == 1 && !g.pre_sq) {
+            // S_t = sum D_A D_t: dS = L_A[E] + L_W[D_A] = 2 IFFT(F_A W delta)
+            ACK(hipMemcpyAsync(H1, spec, hb, hipMemcpyDeviceToDevice, c->st));
+            RCK(run_apply_masks(c->fp64, N, H1, mA, wA, 1, c->st));
+            RCK(run_apply_masks(c->fp64, N, H1, mW, wW, g.nsuf, c->st));
+            RCK(run_inverse_real(c->fp64, N, H1, G, 1, c->tw, c->st));
+        } else if (g.pre_sq) {
+            // S_t = sum D_A^2 D_t: grad = L_A[2 D_A E] + L_W[D_A^2]
+            RCK(filtered(mA, wA, 1, DA));
+            RCK(filtered(mW, wW, g.nsuf, E));
+            RCK(run_mul_fields(Q, DA, E, 2.0, cells, c->st));
+            RCK(project(Q, H2, mA, wA, 1));
+            RCK(run_mul_fields(Q, DA, nullptr, 1.0, cells, c->st));
+            RCK(project(Q, H1, mW, wW, g.nsuf));
+            RCK(run_add_spec(c->fp64, H1, H2, half, c->st));
+            RCK(run_inverse_real(c->fp64, N, H1, G, 1, c->tw, c->st));
+        } else {
+            // S_t = sum D_A D_B D_t: grad = L_A[D_B E] + L_B[D_A E] + L_W[D_A D_B]
+            RCK(filtered(mA, wA, 1, DA));
+            RCK(filtered(mB, wB, 1, DB));
+            RCK(filtered(mW, wW, g.nsuf, E));
+            RCK(run_mul_fields(Q, DB, E, 1.0, cells, c->st));
+            RCK(project(Q, H2, mA, wA, 1));
+            RCK(run_mul_fields(Q, DA, E, 1.0, cells, c->st));
+            RCK(project(Q, H1, mB, wB, 1));
+            RCK(run_add_spec(c->fp64, H2, H1, half, c->st));
+            RCK(run_mul_fields(Q, DA, DB, 1.0, cells, c->st));
+            RCK(project(Q, H1, mW, wW, g.nsuf));
+            RCK(run_add_spec(c->fp64, H1, H2, half, c->st));
+            RCK(run_inverse_real(c->fp64, N, H1, G, 1, c->tw, c->st));
+        }
+        ACK(hipStreamSynchronize(c->st));  // the host mask lists are reused
+    }
+    if (on_device) {
+        if (grad_is_double) {
+            ACK(hipMemcpyAsync(grad, G, fb, hipMemcpyDeviceToDevice, c->st));
+        } else {
+            hipLaunchKernelGGL(k_to_float, dim3(4096), dim3(256), 0, c->st, (float *)grad, G, cells);
+            ACK(hipGetLastError());
+        }
+    } else if (grad_is_double) {
+        ACK(hipMemcpyAsync(grad, G, fb, hipMemcpyDeviceToHost, c->st));
+    } else {
+        hipLaunchKernelGGL(k_to_float, dim3(4096), dim3(256), 0, c->st, (float *)Q, G, cells);
+        ACK(hipGetLastError());
+        ACK(hipMemcpyAsync(grad, Q, sizeof(float) * (size_t)cells, hipMemcpyDeviceToHost, c->st));
+    }
+    ACK(hipStreamSynchronize(c->st));
+done:
+#undef ACK
+#undef RCK
+    (void)hipStreamSynchronize(c->st);
+    void *bufs[] = {H1, H2, DA, DB, E, Q, G, dm, dw};
+    for (void *p : bufs) if (p) (void)hipFree(p);
+    return st;
 }
 
 extern "C" double ps_batch_last_ms(const ps_batch *b) { return b ? b->last_ms : -1.0; }

@@ -64,6 +64,12 @@ int run_reduce_partials(const double2 *partials, int64_t stride, const int *coun
                         int64_t nout, double *outD, double *outI, hipStream_t st);
 int run_fill(double *p, int64_t n, double v, hipStream_t st);
 int run_leg_scale(int fp64, int N, const LegDev *legs, int nlegs, hipStream_t st);
+// adjoint helpers: inverse C2R of a half spectrum (destroyed) into a real
+// double field; H *= sum_j w_j mask_j; a += b; out = alpha a b
+int run_inverse_real(int fp64, int N, void *H, double *out, int accumulate, const void *tw, hipStream_t st);
+int run_apply_masks(int fp64, int N, void *H, const MaskParams *masks, const double *w, int nm, hipStream_t st);
+int run_add_spec(int fp64, void *a, const void *b, int64_t n, hipStream_t st);
+int run_mul_fields(double *out, const double *a, const double *b, double alpha, int64_t n, hipStream_t st);
 int run_corner_extremes(const MaskParams &mp, int N, int K, double bw, double2 *partials, int nblocks, hipStream_t st);
 
 }  // namespace ps

@@ -552,6 +552,77 @@ __global__ void __launch_bounds__(NT) k_leg_scale(const LegDev *__restrict__ leg
     }
 }
 
+// ================================================================ adjoint helpers
+// H *= sum_j w_j * mask_j(m) over the whole half spectrum, in place.
+template <typename T, int N>
+__global__ void __launch_bounds__(NT) k_apply_masks(cplx<T> *__restrict__ H, const MaskParams *__restrict__ masks,
+                                                    const double *__restrict__ w, int nm)
+{
+    const int64_t total = (int64_t)(N / 2 + 1) * N * N;
+    for (int64_t c = (int64_t)blockIdx.x * NT + threadIdx.x; c < total; c += (int64_t)gridDim.x * NT) {
+        const int kz = (int)(c / ((int64_t)N * N));
+        const int kx = (int)((c / N) % N), ky = (int)(c % N);
+        const int mx = lattice(kx, N), my = lattice(ky, N), mz = lattice(kz, N);
+        double f = 0.0;
+        for (int j = 0; j < nm; ++j)
+            if (mask_eval(masks[j], mx, my, mz)) f += w[j];
+        const cplx<T> v = H[c];
+        H[c] = mk<T>((T)((double)v.x * f), (T)((double)v.y * f));
+    }
+}
+
+// a += b over a half spectrum
+template <typename T>
+__global__ void __launch_bounds__(NT) k_add_spec(cplx<T> *__restrict__ a, const cplx<T> *__restrict__ b, int64_t n)
+{
+    for (int64_t c = (int64_t)blockIdx.x * NT + threadIdx.x; c < n; c += (int64_t)gridDim.x * NT)
+        a[c] = cadd<T>(a[c], b[c]);
+}
+
+// out = alpha * a * b (b == nullptr: alpha * a * a)
+__global__ void __launch_bounds__(NT) k_mul_fields(double *__restrict__ out, const double *__restrict__ a,
+                                                   const double *__restrict__ b, double alpha, int64_t n)
+{
+    for (int64_t c = (int64_t)blockIdx.x * NT + threadIdx.x; c < n; c += (int64_t)gridDim.x * NT) {
+        const double x = a[c];
+        out[c] = alpha * x * (b ? b[c] : x);
+    }
+}
+
+// C2R along kz of a half spectrum whose kx, ky axes are already inverse
+// transformed: two adjacent y rows share one complex line.  accumulate: out +=.
+template <typename T, int N>
+__global__ void __launch_bounds__(NT) k_inv_z_real(const cplx<T> *__restrict__ H, double *__restrict__ out,
+                                                   int accumulate, const cplx<T> *__restrict__ tw)
+{
+    constexpr int TL = tile_lines<T, N>();
+    constexpr int TLC = TL / 2;
+    constexpr int NH = N / 2 + 1;
+    __shared__ cplx<T> lds[N * TLC];
+    const int tid = threadIdx.x;
+    const int y0 = blockIdx.x * TL, x = blockIdx.y;
+    for (int idx = tid; idx < NH * TLC; idx += NT) {
+        const int c = idx % TLC, kz = idx / TLC;
+        const int64_t o = ((int64_t)kz * N + x) * N + y0 + 2 * c;
+        const cplx<T> X = H[o], Y = H[o + 1];
+        if (kz == 0 || kz == N / 2) {
+            lds[lidx<TLC>(kz, c)] = mk<T>(X.x, Y.x);
+        } else {
+            lds[lidx<TLC>(kz, c)] = mk<T>(X.x - Y.y, X.y + Y.x);
+            lds[lidx<TLC>(N - kz, c)] = mk<T>(X.x + Y.y, Y.x - X.y);
+        }
+    }
+    __syncthreads();
+    fft_lds<T, N, TLC, +1>(lds, tw, tid);
+    const int64_t base = ((int64_t)x * N + y0) * N;
+    for (int idx = tid; idx < TL * N; idx += NT) {
+        const int row = idx / N, z = idx % N;
+        const cplx<T> v = lds[lidx<TLC>(z, row >> 1)];
+        const double r = (double)((row & 1) ? v.y : v.x);
+        out[base + idx] = accumulate ? out[base + idx] + r : r;
+    }
+}
+
 // ================================================================ reductions
 // out[i] = compensated sum over j < count[i] of partials[2*(i*stride + j) + which]
 __global__ void __launch_bounds__(NT) k_reduce_partials(const double2 *__restrict__ partials, int64_t stride,
@@ -830,6 +901,78 @@ int run_leg_scale(int fp64, int N, const LegDev *legs, int nlegs, hipStream_t st
 {
     if (nlegs <= 0) return 0;
     return fp64 ? dispatch_leg_scale<double>(N, legs, nlegs, st) : dispatch_leg_scale<float>(N, legs, nlegs, st);
+}
+
+static unsigned grid_for(int64_t n)
+{
+    int64_t b = (n + NT - 1) / NT;
+    if (b > 16384) b = 16384;
+    return (unsigned)(b < 1 ? 1 : b);
+}
+
+template <typename T, int N>
+static int launch_inverse_real_n(void *H, double *out, int accumulate, const void *tw, hipStream_t st)
+{
+    constexpr int TLC = tile_lines<T, N>();
+    cplx<T> *h = reinterpret_cast<cplx<T> *>(H);
+    const cplx<T> *twp = reinterpret_cast<const cplx<T> *>(tw);
+    const int64_t nlines = (int64_t)(N / 2 + 1) * N;
+    hipLaunchKernelGGL((k_c2c_contig<T, N, +1>), dim3((unsigned)(nlines / TLC)), dim3(NT), 0, st, h, twp);
+    hipLaunchKernelGGL((k_c2c_strided<T, N, +1>), dim3(N / TLC, N / 2 + 1), dim3(NT), 0, st, h, twp);
+    hipLaunchKernelGGL((k_inv_z_real<T, N>), dim3(N / TLC, N), dim3(NT), 0, st, h, out, accumulate, twp);
+    return (int)hipGetLastError();
+}
+
+template <typename T, int N>
+static int launch_apply_masks_n(void *H, const MaskParams *masks, const double *w, int nm, hipStream_t st)
+{
+    const int64_t total = (int64_t)(N / 2 + 1) * N * N;
+    hipLaunchKernelGGL((k_apply_masks<T, N>), dim3(grid_for(total)), dim3(NT), 0, st,
+                       reinterpret_cast<cplx<T> *>(H), masks, w, nm);
+    return (int)hipGetLastError();
+}
+
+int run_inverse_real(int fp64, int N, void *H, double *out, int accumulate, const void *tw, hipStream_t st)
+{
+    switch (N) {
+#define CASE(n)                                                                                     \
+    case n:                                                                                         \
+        return fp64 ? launch_inverse_real_n<double, n>(H, out, accumulate, tw, st)                  \
+                    : launch_inverse_real_n<float, n>(H, out, accumulate, tw, st);
+        PS_FOR_EACH_N(CASE)
+#undef CASE
+    }
+    return PS_ERR_UNSUPPORTED_N;
+}
+
+int run_apply_masks(int fp64, int N, void *H, const MaskParams *masks, const double *w, int nm, hipStream_t st)
+{
+    switch (N) {
+#define CASE(n)                                                                                     \
+    case n:                                                                                         \
+        return fp64 ? launch_apply_masks_n<double, n>(H, masks, w, nm, st)                          \
+                    : launch_apply_masks_n<float, n>(H, masks, w, nm, st);
+        PS_FOR_EACH_N(CASE)
+#undef CASE
+    }
+    return PS_ERR_UNSUPPORTED_N;
+}
+
+int run_add_spec(int fp64, void *a, const void *b, int64_t n, hipStream_t st)
+{
+    if (fp64)
+        hipLaunchKernelGGL(k_add_spec<double>, dim3(grid_for(n)), dim3(NT), 0, st, (cplx<double> *)a,
+                           (const cplx<double> *)b, n);
+    else
+        hipLaunchKernelGGL(k_add_spec<float>, dim3(grid_for(n)), dim3(NT), 0, st, (cplx<float> *)a,
+                           (const cplx<float> *)b, n);
+    return (int)hipGetLastError();
+}
+
+int run_mul_fields(double *out, const double *a, const double *b, double alpha, int64_t n, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_mul_fields, dim3(grid_for(n)), dim3(NT), 0, st, out, a, b, alpha, n);
+    return (int)hipGetLastError();
 }
 
 int run_corner_extremes(const MaskParams &mp, int N, int K, double bw, double2 *partials, int nblocks, hipStream_t st)
