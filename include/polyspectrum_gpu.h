@@ -122,6 +122,21 @@ int ps_num_polygons(ps_ctx *ctx, int n, const double *k, double *kmin, double *k
 int ps_batch_create(ps_ctx *ctx, ps_polygon *polys, int npoly, double kbinwidth, int kbinning_case,
                     ps_batch **out);
 int ps_batch_execute(ps_batch *b, double *values, double *npolygons, double *sums);
+/* Values of the batch's last execution (by ps_batch_execute or as part of a
+ * ps_multi), without launching anything. */
+int ps_batch_results(ps_batch *b, double *values, double *npolygons, double *sums);
+
+/* Several batches -- contexts of different grid sizes, each with any number
+ * of field slots (realisations) -- captured into one hipGraph on the first
+ * batch's stream.  ps_multi_execute replays it and waits; read each batch's
+ * values with ps_batch_results.  All contexts must be on one device.
+ * Replaces: one reference run (main.c) per field and grid size. */
+typedef struct ps_multi ps_multi;
+int ps_multi_create(ps_batch **batches, int nbatch, ps_multi **out);
+int ps_multi_execute(ps_multi *m);
+double ps_multi_last_ms(const ps_multi *m);
+int ps_multi_destroy(ps_multi *m);
+
 /* Time of the last ps_batch_execute on the GPU (ms), from HIP events. */
 double ps_batch_last_ms(const ps_batch *b);
 /* HBM bytes the batch touches per execution by the algorithmic model

@@ -14,7 +14,7 @@ import numpy as np
 
 __all__ = [
     "PREC_FP64", "PREC_MIXED", "BINNING", "Polygon", "Context", "Batch", "GPUError",
-    "polyspectrum", "num_polygons", "library_path", "load_library", "device_count", "build",
+    "polyspectrum", "num_polygons", "library_path", "load_library", "device_count", "build", "MultiBatch",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -94,6 +94,12 @@ def load_library():
     L.ps_batch_profile.argtypes = [_vp, ctypes.c_int, _dp, _dp]
     L.ps_batch_adjoint.argtypes = [_vp, ctypes.c_int, _dp, _vp, ctypes.c_int, ctypes.c_int]
     L.ps_forward_profile.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _dp, _dp]
+    L.ps_batch_results.argtypes = [_vp, _dp, _dp, _dp]
+    L.ps_multi_create.argtypes = [ctypes.POINTER(_vp), ctypes.c_int, ctypes.POINTER(_vp)]
+    L.ps_multi_execute.argtypes = [_vp]
+    L.ps_multi_last_ms.restype = ctypes.c_double
+    L.ps_multi_last_ms.argtypes = [_vp]
+    L.ps_multi_destroy.argtypes = [_vp]
     L.polyspectrum.restype = ctypes.c_double
     L.polyspectrum.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int, _dp, _dp, _dp,
                                ctypes.c_double, ctypes.c_int, ctypes.c_double]
@@ -278,6 +284,15 @@ class Batch:
                                                int(out.dtype == torch.float64), 1))
         return out
 
+    def results(self):
+        """(values, npolygons, sums) of the last execution, no launch."""
+        v = np.zeros(self.n)
+        npol = np.zeros(self.n)
+        s = np.zeros(2 * self.n)
+        _check(load_library().ps_batch_results(self._h, v.ctypes.data_as(_dp), npol.ctypes.data_as(_dp),
+                                               s.ctypes.data_as(_dp)))
+        return v, npol, s.reshape(self.n, 2)
+
     def profile(self, reps=3):
         """Per-stage (G1, G2, Zc, reduce) mean ms and algorithmic bytes."""
         ms = np.zeros(4)
@@ -326,3 +341,34 @@ def num_polygons(nbins, local_n0, local_n0_start, n, k, kmin, kmax, kbinwidth, k
     return load_library().num_polygons(nbins, local_n0, local_n0_start, n, k.ctypes.data_as(_dp),
                                        kmin.ctypes.data_as(_dp), kmax.ctypes.data_as(_dp), kbinwidth,
                                        kbinningCase, boxsize)
+
+
+class MultiBatch:
+    """Several batches (any grid sizes and slots) replayed as one hipGraph."""
+
+    def __init__(self, batches):
+        arr = (_vp * len(batches))(*[b._h for b in batches])
+        h = _vp()
+        _check(load_library().ps_multi_create(arr, len(batches), ctypes.byref(h)))
+        self._h = h
+        self.batches = list(batches)
+
+    def execute(self):
+        """Replays the graph; returns [(values, npolygons, sums)] per batch."""
+        _check(load_library().ps_multi_execute(self._h))
+        return [b.results() for b in self.batches]
+
+    @property
+    def last_ms(self):
+        return load_library().ps_multi_last_ms(self._h)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load_library().ps_multi_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

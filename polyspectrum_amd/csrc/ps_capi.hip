@@ -481,17 +481,19 @@ static bool same_mask(const MaskParams &a, const MaskParams &b)
     return a.kind == b.kind && a.kkf == b.kkf && a.mmin == b.mmin && a.mmax == b.mmax && a.bw == b.bw;
 }
 
-static int batch_record(ps_batch *b)
+// the launch sequence of one batch on a stream (captured into graphs)
+static int batch_record_on(ps_batch *b, hipStream_t s)
 {
     ps_ctx *c = b->c;
-    if (b->dscale) PSCK(run_leg_scale(c->fp64, c->N, b->dlegs, (int)b->legs.size(), c->st));
+    if (b->dscale) PSCK(run_leg_scale(c->fp64, c->N, b->dlegs, (int)b->legs.size(), s));
     PSCK(run_inverse_g(c->fp64, c->N, b->dlegs, (int)b->legs.size(), b->g1_tiles, b->g2_tiles, b->grid_cap, c->tw,
-                       c->st));
+                       s));
     PSCK(run_zc_fused(c->fp64, c->N, b->dgroups, (int)b->groups.size(), b->dlegs, b->dsuffix, b->items, b->dpartials,
-                      b->max_chunks, b->grid_cap, b->kz_cap, c->tw, c->st));
-    PSCK(run_reduce_partials(b->dpartials, b->max_chunks, b->dcounts, 0, b->nout, b->dsum, b->dsum + b->nout, c->st));
+                      b->max_chunks, b->grid_cap, b->kz_cap, c->tw, s));
+    PSCK(run_reduce_partials(b->dpartials, b->max_chunks, b->dcounts, 0, b->nout, b->dsum, b->dsum + b->nout, s));
     return PS_OK;
 }
+static int batch_record(ps_batch *b) { return batch_record_on(b, b->c->st); }
 
 extern "C" int ps_batch_create(ps_ctx *c, ps_polygon *polys, int npoly, double bw, int kcase, ps_batch **out)
 {
@@ -675,21 +677,13 @@ fail:
     return st;
 }
 
-extern "C" int ps_batch_execute(ps_batch *b, double *values, double *npolygons, double *sums)
+// device sums of the last execution -> normalised values (polyspectrum.c:155, :247)
+static int batch_results(ps_batch *b, double *values, double *npolygons, double *sums)
 {
-    if (!b || !values) return PS_ERR_INVALID;
     ps_ctx *c = b->c;
-    HIPCK(hipSetDevice(c->device));
-    HIPCK(hipEventRecord(b->ev0, c->st));
-    if (b->exec) HIPCK(hipGraphLaunch(b->exec, c->st));
-    else PSCK(batch_record(b));
-    HIPCK(hipEventRecord(b->ev1, c->st));
     std::vector<double> h(2 * (size_t)b->nout);
     HIPCK(hipMemcpyAsync(h.data(), b->dsum, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->st));
     HIPCK(hipStreamSynchronize(c->st));
-    float ms = 0.f;
-    HIPCK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
-    b->last_ms = ms;
     b->last_sumI.assign(h.begin() + b->nout, h.end());
     b->have_sums = 1;
     const double volume = c->L * c->L * c->L;
@@ -707,6 +701,103 @@ extern "C" int ps_batch_execute(ps_batch *b, double *values, double *npolygons, 
     return PS_OK;
 }
 
+extern "C" int ps_batch_execute(ps_batch *b, double *values, double *npolygons, double *sums)
+{
+    if (!b || !values) return PS_ERR_INVALID;
+    ps_ctx *c = b->c;
+    HIPCK(hipSetDevice(c->device));
+    HIPCK(hipEventRecord(b->ev0, c->st));
+    if (b->exec) HIPCK(hipGraphLaunch(b->exec, c->st));
+    else PSCK(batch_record(b));
+    HIPCK(hipEventRecord(b->ev1, c->st));
+    HIPCK(hipEventSynchronize(b->ev1));
+    float ms = 0.f;
+    HIPCK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+    b->last_ms = ms;
+    return batch_results(b, values, npolygons, sums);
+}
+
+extern "C" int ps_batch_results(ps_batch *b, double *values, double *npolygons, double *sums)
+{
+    if (!b || !values) return PS_ERR_INVALID;
+    HIPCK(hipSetDevice(b->c->device));
+    return batch_results(b, values, npolygons, sums);
+}
+
+// ---------------------------------------------------------------- multi-batch
+// Several batches (contexts of different grid sizes, many field slots) in
+// one captured launch sequence on the stream of the first batch's context.
+struct ps_multi {
+    std::vector<ps_batch *> b;
+    hipStream_t st = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_ms = 0.0;
+};
+
+extern "C" int ps_multi_destroy(ps_multi *m)
+{
+    if (!m) return PS_ERR_INVALID;
+    if (m->st) (void)hipStreamSynchronize(m->st);
+    if (m->exec) (void)hipGraphExecDestroy(m->exec);
+    if (m->graph) (void)hipGraphDestroy(m->graph);
+    if (m->ev0) (void)hipEventDestroy(m->ev0);
+    if (m->ev1) (void)hipEventDestroy(m->ev1);
+    delete m;
+    return PS_OK;
+}
+
+extern "C" int ps_multi_create(ps_batch **batches, int nbatch, ps_multi **out)
+{
+    if (!batches || nbatch < 1 || !out) return PS_ERR_INVALID;
+    *out = nullptr;
+    for (int i = 0; i < nbatch; ++i)
+        if (!batches[i] || batches[i]->c->device != batches[0]->c->device) return PS_ERR_INVALID;
+    HIPCK(hipSetDevice(batches[0]->c->device));
+    ps_multi *m = new ps_multi();
+    m->b.assign(batches, batches + nbatch);
+    m->st = batches[0]->c->st;
+    int st = PS_OK;
+    // the other contexts' pending work (field uploads) must be complete
+    for (int i = 0; i < nbatch; ++i)
+        if (hipStreamSynchronize(batches[i]->c->st) != hipSuccess) st = PS_ERR_HIP;
+    if (st == PS_OK && (hipEventCreate(&m->ev0) != hipSuccess || hipEventCreate(&m->ev1) != hipSuccess))
+        st = PS_ERR_HIP;
+    if (st == PS_OK && hipStreamBeginCapture(m->st, hipStreamCaptureModeThreadLocal) != hipSuccess) st = PS_ERR_HIP;
+    if (st == PS_OK) {
+        for (int i = 0; i < nbatch && st == PS_OK; ++i) st = batch_record_on(batches[i], m->st);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(m->st, &g);
+        if (e != hipSuccess && st == PS_OK) st = PS_ERR_HIP;
+        m->graph = g;
+        if (st == PS_OK && hipGraphInstantiate(&m->exec, m->graph, nullptr, nullptr, 0) != hipSuccess)
+            st = PS_ERR_HIP;
+    }
+    if (st != PS_OK) {
+        ps_multi_destroy(m);
+        return st;
+    }
+    *out = m;
+    return PS_OK;
+}
+
+extern "C" int ps_multi_execute(ps_multi *m)
+{
+    if (!m) return PS_ERR_INVALID;
+    HIPCK(hipSetDevice(m->b[0]->c->device));
+    HIPCK(hipEventRecord(m->ev0, m->st));
+    HIPCK(hipGraphLaunch(m->exec, m->st));
+    HIPCK(hipEventRecord(m->ev1, m->st));
+    HIPCK(hipEventSynchronize(m->ev1));
+    float ms = 0.f;
+    HIPCK(hipEventElapsedTime(&ms, m->ev0, m->ev1));
+    m->last_ms = ms;
+    return PS_OK;
+}
+
+extern "C" double ps_multi_last_ms(const ps_multi *m) { return m ? m->last_ms : -1.0; }
+
 // ---------------------------------------------------------------- adjoint
 __global__ void k_to_float(float *__restrict__ out, const double *__restrict__ in, int64_t n)
 {
@@ -721,7 +812,8 @@ extern "C" int ps_batch_adjoint(ps_batch *b, int slot, const double *weights, vo
     if (!b || !weights || !grad) return PS_ERR_INVALID;
     ps_ctx *c = b->c;
     if (slot < 0 || slot >= c->nslots || !c->loaded[slot]) return PS_ERR_STATE;
-    if (c->phase_only) return PS_ERR_INVALID;  // the phase-only adjoint is not provided
+    for (const LegDev &L : b->legs)
+        if (L.phase_only) return PS_ERR_INVALID;  // the phase-only adjoint is not provided
     if (!b->have_sums) return PS_ERR_STATE;
     HIPCK(hipSetDevice(c->device));
     const int N = c->N;
