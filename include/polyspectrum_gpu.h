@@ -121,6 +121,27 @@ int ps_num_polygons(ps_ctx *ctx, int n, const double *k, double *kmin, double *k
  * Replaces: the loop run.c:56-107 over polyspectrum()/num_polygons(). */
 int ps_batch_create(ps_ctx *ctx, ps_polygon *polys, int npoly, double kbinwidth, int kbinning_case,
                     ps_batch **out);
+
+/* Launch shape of a batch: persistent grid size of the pass kernels, Zc tiles
+ * per work item (sets how many compensated partial sums each polygon has)
+ * and the divisor of the G2 tile width (1: 16 FP32 / 8 FP64 columns, 2: half). */
+typedef struct ps_launch_params {
+    int grid_cap;
+    int chunk_tiles;
+    int g2_div;
+} ps_launch_params;
+
+/* ps_batch_create with explicit launch parameters (NULL: the tuned ones for
+ * this batch shape if ps_batch_autotune ran, else defaults). */
+int ps_batch_create_ex(ps_ctx *ctx, ps_polygon *polys, int npoly, double kbinwidth, int kbinning_case,
+                       const ps_launch_params *params, ps_batch **out);
+
+/* Autotuner: times candidate launch shapes on this batch (reps replays each),
+ * keeps the fastest for later ps_batch_create calls of the same shape (and
+ * appends it to the file PS_TUNE_FILE names, if set). */
+int ps_batch_autotune(ps_ctx *ctx, ps_polygon *polys, int npoly, double kbinwidth, int kbinning_case, int reps,
+                      ps_launch_params *best, double *best_ms);
+int ps_batch_params(const ps_batch *b, ps_launch_params *out);
 int ps_batch_execute(ps_batch *b, double *values, double *npolygons, double *sums);
 /* Values of the batch's last execution (by ps_batch_execute or as part of a
  * ps_multi), without launching anything. */

@@ -39,6 +39,26 @@ class Polygon(ctypes.Structure):
                 ("kmax", ctypes.c_double * 3)]
 
 
+class LaunchParams(ctypes.Structure):
+    """ps_launch_params: persistent grid size, Zc tiles per work item, G2 tile divisor."""
+    _fields_ = [("grid_cap", ctypes.c_int), ("chunk_tiles", ctypes.c_int), ("g2_div", ctypes.c_int)]
+
+
+def _polygon_array(polygons):
+    arr = (Polygon * len(polygons))()
+    for i, p in enumerate(polygons):
+        if isinstance(p, Polygon):
+            arr[i] = p
+            continue
+        arr[i].n = int(p["n"])
+        arr[i].slot = int(p.get("slot", 0))
+        for j in range(3):
+            arr[i].k[j] = float(p["k"][j]) if j < len(p["k"]) else 0.0
+            arr[i].kmin[j] = float(p.get("kmin", [0, 0, 0])[j]) if j < len(p.get("kmin", [])) else 0.0
+            arr[i].kmax[j] = float(p.get("kmax", [0, 0, 0])[j]) if j < len(p.get("kmax", [])) else 0.0
+    return arr
+
+
 def library_path():
     return os.path.join(_HERE, _LIB_NAME)
 
@@ -95,6 +115,11 @@ def load_library():
     L.ps_batch_adjoint.argtypes = [_vp, ctypes.c_int, _dp, _vp, ctypes.c_int, ctypes.c_int]
     L.ps_forward_profile.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _dp, _dp]
     L.ps_batch_results.argtypes = [_vp, _dp, _dp, _dp]
+    L.ps_batch_create_ex.argtypes = [_vp, ctypes.POINTER(Polygon), ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                     ctypes.POINTER(LaunchParams), ctypes.POINTER(_vp)]
+    L.ps_batch_autotune.argtypes = [_vp, ctypes.POINTER(Polygon), ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                    ctypes.c_int, ctypes.POINTER(LaunchParams), _dp]
+    L.ps_batch_params.argtypes = [_vp, ctypes.POINTER(LaunchParams)]
     L.ps_multi_create.argtypes = [ctypes.POINTER(_vp), ctypes.c_int, ctypes.POINTER(_vp)]
     L.ps_multi_execute.argtypes = [_vp]
     L.ps_multi_last_ms.restype = ctypes.c_double
@@ -225,8 +250,18 @@ class Context:
                                               float(kbinwidth), int(kbinning_case), ctypes.byref(res)))
         return res.value
 
-    def batch(self, polygons, kbinwidth=1.0, kbinning_case=0):
-        return Batch(self, polygons, kbinwidth, kbinning_case)
+    def batch(self, polygons, kbinwidth=1.0, kbinning_case=0, params=None):
+        return Batch(self, polygons, kbinwidth, kbinning_case, params)
+
+    def autotune(self, polygons, kbinwidth=1.0, kbinning_case=0, reps=2):
+        """Times candidate launch shapes; later batches of this shape use the best.
+        Returns ({grid_cap, chunk_tiles, g2_div}, ms)."""
+        arr = _polygon_array(polygons)
+        lp = LaunchParams()
+        ms = ctypes.c_double()
+        _check(load_library().ps_batch_autotune(self._h, arr, len(polygons), float(kbinwidth), int(kbinning_case),
+                                                int(reps), ctypes.byref(lp), ctypes.byref(ms)))
+        return {"grid_cap": lp.grid_cap, "chunk_tiles": lp.chunk_tiles, "g2_div": lp.g2_div}, ms.value
 
     def batch_for_field(self, polygons, field, kbinwidth=1.0, kbinning_case=0, slot=0):
         """Load a field into the slot, then plan a batch on it."""
@@ -237,22 +272,14 @@ class Context:
 class Batch:
     """Many configurations evaluated by one captured launch sequence."""
 
-    def __init__(self, ctx, polygons, kbinwidth=1.0, kbinning_case=0):
-        arr = (Polygon * len(polygons))()
-        for i, p in enumerate(polygons):
-            if isinstance(p, Polygon):
-                arr[i] = p
-                continue
-            n = int(p["n"])
-            arr[i].n = n
-            arr[i].slot = int(p.get("slot", 0))
-            for j in range(3):
-                arr[i].k[j] = float(p["k"][j]) if j < len(p["k"]) else 0.0
-                arr[i].kmin[j] = float(p.get("kmin", [0, 0, 0])[j]) if j < len(p.get("kmin", [])) else 0.0
-                arr[i].kmax[j] = float(p.get("kmax", [0, 0, 0])[j]) if j < len(p.get("kmax", [])) else 0.0
+    def __init__(self, ctx, polygons, kbinwidth=1.0, kbinning_case=0, params=None):
+        arr = _polygon_array(polygons)
         h = _vp()
-        _check(load_library().ps_batch_create(ctx._h, arr, len(polygons), float(kbinwidth), int(kbinning_case),
-                                              ctypes.byref(h)))
+        lp = None
+        if params is not None:
+            lp = LaunchParams(int(params["grid_cap"]), int(params["chunk_tiles"]), int(params.get("g2_div", 1)))
+        _check(load_library().ps_batch_create_ex(ctx._h, arr, len(polygons), float(kbinwidth), int(kbinning_case),
+                                                 ctypes.byref(lp) if lp is not None else None, ctypes.byref(h)))
         self._h = h
         self._ctx = ctx  # keeps the context alive
         self.polygons = arr
@@ -283,6 +310,12 @@ class Batch:
         _check(load_library().ps_batch_adjoint(self._h, slot, w.ctypes.data_as(_dp), ctypes.c_void_p(out.data_ptr()),
                                                int(out.dtype == torch.float64), 1))
         return out
+
+    @property
+    def params(self):
+        lp = LaunchParams()
+        _check(load_library().ps_batch_params(self._h, ctypes.byref(lp)))
+        return {"grid_cap": lp.grid_cap, "chunk_tiles": lp.chunk_tiles, "g2_div": lp.g2_div}
 
     def results(self):
         """(values, npolygons, sums) of the last execution, no launch."""

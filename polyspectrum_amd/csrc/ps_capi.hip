@@ -18,6 +18,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <string>
 #include <tuple>
 #include <vector>
 
@@ -368,7 +369,7 @@ static int plain_leg(ps_ctx *c, const MaskParams &mp, const void *spec, int mode
     const int tl = g_tile_lines(N, c->fp64);
     const int64_t g1 = (int64_t)(N / 2 + 1) * ((N + tl - 1) / tl);
     const int64_t g2 = 2 * (int64_t)(N / 2 + 1) * (N / tile_rows(N, c->fp64));
-    PSCK(run_inverse_g(c->fp64, N, c->dleg, 1, g1, g2, 1 << 30, c->tw, c->st));
+    PSCK(run_inverse_g(c->fp64, N, c->dleg, 1, g1, g2, 1 << 30, 1, c->tw, c->st));
     PSCK(run_zc_plain(c->fp64, N, L.g2d, L.g2i, N / 2, spec ? c->accD : nullptr, c->accI, mode, c->partials, c->tw,
                       c->st));
     // the next leg overwrites the host-side descriptor buffer: keep order
@@ -442,6 +443,8 @@ struct ps_batch {
     int max_chunks = 1;
     int grid_cap = 2048;
     int kz_cap = 1;                      // 1 + largest Kz of the legs
+    int g2_div = 1;                      // G2 tile width divisor (1 or 2)
+    int chunk_tiles = 64;                // Zc tiles per work item
     std::vector<void *> allocs;
     LegDev *dlegs = nullptr;
     GroupDev *dgroups = nullptr;
@@ -486,8 +489,8 @@ static int batch_record_on(ps_batch *b, hipStream_t s)
 {
     ps_ctx *c = b->c;
     if (b->dscale) PSCK(run_leg_scale(c->fp64, c->N, b->dlegs, (int)b->legs.size(), s));
-    PSCK(run_inverse_g(c->fp64, c->N, b->dlegs, (int)b->legs.size(), b->g1_tiles, b->g2_tiles, b->grid_cap, c->tw,
-                       s));
+    PSCK(run_inverse_g(c->fp64, c->N, b->dlegs, (int)b->legs.size(), b->g1_tiles, b->g2_tiles, b->grid_cap, b->g2_div,
+                       c->tw, s));
     PSCK(run_zc_fused(c->fp64, c->N, b->dgroups, (int)b->groups.size(), b->dlegs, b->dsuffix, b->items, b->dpartials,
                       b->max_chunks, b->grid_cap, b->kz_cap, c->tw, s));
     PSCK(run_reduce_partials(b->dpartials, b->max_chunks, b->dcounts, 0, b->nout, b->dsum, b->dsum + b->nout, s));
@@ -495,11 +498,71 @@ static int batch_record_on(ps_batch *b, hipStream_t s)
 }
 static int batch_record(ps_batch *b) { return batch_record_on(b, b->c->st); }
 
+// ---------------------------------------------------------------- launch-shape tuning
+// Best launch parameters found by ps_batch_autotune, per batch shape; kept for
+// the process and, when PS_TUNE_FILE names a file, appended to / read from it.
+static std::map<std::string, ps_launch_params> g_tuned;
+static bool g_tune_loaded = false;
+
+static std::string tune_key(const ps_ctx *c, const ps_polygon *polys, int npoly, double bw, int kcase)
+{
+    double ks = 0.0;
+    int slots = 0;
+    for (int p = 0; p < npoly; ++p) {
+        for (int i = 0; i < polys[p].n; ++i) ks += polys[p].k[i] + polys[p].kmin[i] + polys[p].kmax[i];
+        slots = std::max(slots, polys[p].slot + 1);
+    }
+    char buf[256];
+    snprintf(buf, sizeof(buf), "N%d_fp%d_cu%d_np%d_kc%d_sl%d_bw%.6g_ks%.9g", c->N, c->fp64, c->cus, npoly, kcase, slots,
+             bw, ks);
+    return buf;
+}
+
+static void tune_load()
+{
+    if (g_tune_loaded) return;
+    g_tune_loaded = true;
+    const char *f = getenv("PS_TUNE_FILE");
+    if (!f) return;
+    FILE *fp = fopen(f, "r");
+    if (!fp) return;
+    char key[256];
+    ps_launch_params lp;
+    while (fscanf(fp, "%255s %d %d %d", key, &lp.grid_cap, &lp.chunk_tiles, &lp.g2_div) == 4) g_tuned[key] = lp;
+    fclose(fp);
+}
+
+static ps_launch_params default_params(const ps_ctx *c)
+{
+    ps_launch_params lp;
+    lp.grid_cap = 8 * c->cus;
+    lp.chunk_tiles = 64;
+    lp.g2_div = 1;
+    if (const char *e = getenv("PS_CHUNK_TILES")) lp.chunk_tiles = std::max(1, atoi(e));
+    if (const char *e = getenv("PS_GRID_CAP")) lp.grid_cap = std::max(1, atoi(e));
+    return lp;
+}
+
 extern "C" int ps_batch_create(ps_ctx *c, ps_polygon *polys, int npoly, double bw, int kcase, ps_batch **out)
+{
+    return ps_batch_create_ex(c, polys, npoly, bw, kcase, nullptr, out);
+}
+
+extern "C" int ps_batch_create_ex(ps_ctx *c, ps_polygon *polys, int npoly, double bw, int kcase,
+                                  const ps_launch_params *params, ps_batch **out)
 {
     if (!c || !polys || npoly < 1 || !out) return PS_ERR_INVALID;
     *out = nullptr;
     HIPCK(hipSetDevice(c->device));
+    ps_launch_params lp = default_params(c);
+    if (params) {
+        lp = *params;
+    } else {
+        tune_load();
+        auto it = g_tuned.find(tune_key(c, polys, npoly, bw, kcase));
+        if (it != g_tuned.end()) lp = it->second;
+    }
+    if (lp.grid_cap < 1 || lp.chunk_tiles < 1 || (lp.g2_div != 1 && lp.g2_div != 2)) return PS_ERR_INVALID;
     for (int p = 0; p < npoly; ++p) {
         if (polys[p].n < 2 || polys[p].n > 3) return PS_ERR_INVALID;
         if (polys[p].slot < 0 || polys[p].slot >= c->nslots) return PS_ERR_INVALID;
@@ -509,6 +572,9 @@ extern "C" int ps_batch_create(ps_ctx *c, ps_polygon *polys, int npoly, double b
     ps_batch *b = new ps_batch();
     b->c = c;
     b->npoly = npoly;
+    b->grid_cap = lp.grid_cap;
+    b->chunk_tiles = lp.chunk_tiles;
+    b->g2_div = (tile_rows(c->N, c->fp64) / lp.g2_div >= 4) ? lp.g2_div : 1;
     // 1. legs of every polygon, de-duplicated by (slot, mask)
     std::vector<std::vector<int>> pleg(npoly);
     std::vector<std::pair<int, MaskParams>> keys;
@@ -549,7 +615,7 @@ extern "C" int ps_batch_create(ps_ctx *c, ps_polygon *polys, int npoly, double b
         L.g1_begin = b->g1_tiles;
         L.g2_begin = b->g2_tiles;
         b->g1_tiles += (int64_t)(K + 1) * ((L.Bx + tl - 1) / tl);
-        b->g2_tiles += 2 * (int64_t)(K + 1) * (N / tile_rows(N, c->fp64));
+        b->g2_tiles += 2 * (int64_t)(K + 1) * (N / (tile_rows(N, c->fp64) / b->g2_div));
         const size_t n1 = (size_t)(K + 1) * L.Bx * N, n2 = (size_t)(K + 1) * N * N;
         void *p1 = nullptr, *p2 = nullptr;
         if (hipMalloc(&p1, cs * n1 * 2) != hipSuccess || hipMalloc(&p2, cs * n2 * 2) != hipSuccess) {
@@ -579,9 +645,7 @@ extern "C" int ps_batch_create(ps_ctx *c, ps_polygon *polys, int npoly, double b
     }
     const int rows = fused_tile_rows(N, c->fp64);
     const int64_t tiles_per_group = (int64_t)N * (N / rows);
-    int chunk_tiles = 64;
-    if (const char *e = getenv("PS_CHUNK_TILES")) chunk_tiles = std::max(1, atoi(e));
-    if (const char *e = getenv("PS_GRID_CAP")) b->grid_cap = std::max(1, atoi(e));
+    const int chunk_tiles = b->chunk_tiles;
     const int nchunks = (int)((tiles_per_group + chunk_tiles - 1) / chunk_tiles);
     b->max_chunks = nchunks;
     b->poly_out.assign(npoly, -1);
@@ -958,6 +1022,62 @@ done:
 
 extern "C" double ps_batch_last_ms(const ps_batch *b) { return b ? b->last_ms : -1.0; }
 
+extern "C" int ps_batch_autotune(ps_ctx *c, ps_polygon *polys, int npoly, double bw, int kcase, int reps,
+                                 ps_launch_params *best, double *best_ms)
+{
+    if (!c || !polys || npoly < 1 || reps < 1) return PS_ERR_INVALID;
+    const int caps[] = {2 * c->cus, 4 * c->cus, 8 * c->cus, 16 * c->cus};
+    const int chunks[] = {16, 64, 256};
+    const int divs[] = {1, 2};
+    std::vector<ps_polygon> work(polys, polys + npoly);
+    std::vector<double> v(npoly);
+    ps_launch_params bp = default_params(c);
+    double bms = 1e300;
+    for (int cap : caps)
+        for (int ch : chunks)
+            for (int dv : divs) {
+                ps_launch_params lp{cap, ch, dv};
+                ps_batch *b = nullptr;
+                std::copy(polys, polys + npoly, work.begin());
+                int s = ps_batch_create_ex(c, work.data(), npoly, bw, kcase, &lp, &b);
+                if (s != PS_OK) return s;
+                double t = 0.0;
+                s = ps_batch_execute(b, v.data(), nullptr, nullptr);  // warm-up
+                for (int r = 0; r < reps && s == PS_OK; ++r) {
+                    s = ps_batch_execute(b, v.data(), nullptr, nullptr);
+                    t += b->last_ms;
+                }
+                ps_batch_destroy(b);
+                if (s != PS_OK) return s;
+                t /= reps;
+                if (t < bms) {
+                    bms = t;
+                    bp = lp;
+                }
+            }
+    const std::string key = tune_key(c, polys, npoly, bw, kcase);
+    tune_load();
+    g_tuned[key] = bp;
+    if (const char *f = getenv("PS_TUNE_FILE")) {
+        if (FILE *fp = fopen(f, "a")) {
+            fprintf(fp, "%s %d %d %d\n", key.c_str(), bp.grid_cap, bp.chunk_tiles, bp.g2_div);
+            fclose(fp);
+        }
+    }
+    if (best) *best = bp;
+    if (best_ms) *best_ms = bms;
+    return PS_OK;
+}
+
+extern "C" int ps_batch_params(const ps_batch *b, ps_launch_params *out)
+{
+    if (!b || !out) return PS_ERR_INVALID;
+    out->grid_cap = b->grid_cap;
+    out->chunk_tiles = b->chunk_tiles;
+    out->g2_div = b->g2_div;
+    return PS_OK;
+}
+
 extern "C" int ps_batch_profile(ps_batch *b, int reps, double *stage_ms, double *stage_bytes)
 {
     if (!b || reps < 1 || !stage_ms) return PS_ERR_INVALID;
@@ -971,11 +1091,12 @@ extern "C" int ps_batch_profile(ps_batch *b, int reps, double *stage_ms, double 
         (void)hipEventRecord(ev[0], c->st);
         if (b->dscale) st = run_leg_scale(c->fp64, c->N, b->dlegs, (int)b->legs.size(), c->st);
         if (st == PS_OK)
-            st = run_inverse_g(c->fp64, c->N, b->dlegs, (int)b->legs.size(), b->g1_tiles, 0, b->grid_cap, c->tw,
-                               c->st);
+            st = run_inverse_g(c->fp64, c->N, b->dlegs, (int)b->legs.size(), b->g1_tiles, 0, b->grid_cap, b->g2_div,
+                               c->tw, c->st);
         (void)hipEventRecord(ev[1], c->st);
         if (st == PS_OK)
-            st = run_inverse_g(c->fp64, c->N, b->dlegs, (int)b->legs.size(), 0, b->g2_tiles, b->grid_cap, c->tw, c->st);
+            st = run_inverse_g(c->fp64, c->N, b->dlegs, (int)b->legs.size(), 0, b->g2_tiles, b->grid_cap, b->g2_div,
+                               c->tw, c->st);
         (void)hipEventRecord(ev[2], c->st);
         if (st == PS_OK)
             st = run_zc_fused(c->fp64, c->N, b->dgroups, (int)b->groups.size(), b->dlegs, b->dsuffix, b->items,

@@ -54,7 +54,7 @@ int g_tile_lines(int N, int fp64);
 
 int run_forward(int fp64, int N, const void *field, int field_is_double, void *H, const void *tw, hipStream_t st);
 int run_inverse_g(int fp64, int N, const LegDev *legs, int nlegs, int64_t g1_tiles, int64_t g2_tiles, int grid_cap,
-                  const void *tw, hipStream_t st);
+                  int g2_div, const void *tw, hipStream_t st);
 int run_zc_plain(int fp64, int N, const void *gd, const void *gi, int Kz, double *accD, double *accI, int mode,
                  double2 *partials, const void *tw, hipStream_t st);
 int run_zc_fused(int fp64, int N, const GroupDev *groups, int ngroups, const LegDev *legs, const int *suffix,

@@ -188,11 +188,11 @@ __global__ void __launch_bounds__(NT) k_inv_g1(const LegDev *__restrict__ legs, 
 
 // G2: C2C along kx from the band to every x.  Tile = (leg, part, kz, y tile
 // of TLC adjacent columns), part 0 = D, 1 = I; stores are TLC-value segments.
-template <typename T, int N>
+template <typename T, int N, int WD>
 __global__ void __launch_bounds__(NT) k_inv_g2(const LegDev *__restrict__ legs, int nlegs, int64_t ntiles,
                                                const cplx<T> *__restrict__ tw)
 {
-    constexpr int TLC = tile_lines<T, N>();
+    constexpr int TLC = tile_lines<T, N>() / WD;  // WD: tile width divisor picked by the autotuner
     __shared__ cplx<T> lds[N * TLC];
     const int tid = threadIdx.x;
     constexpr int nyt = N / TLC;
@@ -723,7 +723,7 @@ static int launch_forward_n(const void *field, int field_is_double, void *H, con
 
 template <typename T, int N>
 static int launch_inverse_g_n(const LegDev *legs_dev, int nlegs, int64_t g1_tiles, int64_t g2_tiles, int grid_cap,
-                              const void *tw, hipStream_t st)
+                              int g2_div, const void *tw, hipStream_t st)
 {
     const cplx<T> *twp = reinterpret_cast<const cplx<T> *>(tw);
     if (g1_tiles > 0) {
@@ -732,7 +732,10 @@ static int launch_inverse_g_n(const LegDev *legs_dev, int nlegs, int64_t g1_tile
     }
     if (g2_tiles > 0) {
         const unsigned g = (unsigned)((g2_tiles < grid_cap) ? g2_tiles : grid_cap);
-        hipLaunchKernelGGL((k_inv_g2<T, N>), dim3(g), dim3(NT), 0, st, legs_dev, nlegs, g2_tiles, twp);
+        if (g2_div == 2)
+            hipLaunchKernelGGL((k_inv_g2<T, N, 2>), dim3(g), dim3(NT), 0, st, legs_dev, nlegs, g2_tiles, twp);
+        else
+            hipLaunchKernelGGL((k_inv_g2<T, N, 1>), dim3(g), dim3(NT), 0, st, legs_dev, nlegs, g2_tiles, twp);
     }
     return (int)hipGetLastError();
 }
@@ -779,11 +782,11 @@ static int dispatch_forward(int N, const void *field, int fd, void *H, const voi
 }
 
 template <typename T>
-static int dispatch_inverse_g(int N, const LegDev *legs, int nlegs, int64_t g1, int64_t g2, int cap, const void *tw,
-                              hipStream_t st)
+static int dispatch_inverse_g(int N, const LegDev *legs, int nlegs, int64_t g1, int64_t g2, int cap, int g2_div,
+                              const void *tw, hipStream_t st)
 {
     switch (N) {
-#define CASE(n) case n: return launch_inverse_g_n<T, n>(legs, nlegs, g1, g2, cap, tw, st);
+#define CASE(n) case n: return launch_inverse_g_n<T, n>(legs, nlegs, g1, g2, cap, g2_div, tw, st);
         PS_FOR_EACH_N(CASE)
 #undef CASE
     }
@@ -847,10 +850,10 @@ int run_forward(int fp64, int N, const void *field, int field_is_double, void *H
 }
 
 int run_inverse_g(int fp64, int N, const LegDev *legs, int nlegs, int64_t g1_tiles, int64_t g2_tiles, int grid_cap,
-                  const void *tw, hipStream_t st)
+                  int g2_div, const void *tw, hipStream_t st)
 {
-    return fp64 ? dispatch_inverse_g<double>(N, legs, nlegs, g1_tiles, g2_tiles, grid_cap, tw, st)
-                : dispatch_inverse_g<float>(N, legs, nlegs, g1_tiles, g2_tiles, grid_cap, tw, st);
+    return fp64 ? dispatch_inverse_g<double>(N, legs, nlegs, g1_tiles, g2_tiles, grid_cap, g2_div, tw, st)
+                : dispatch_inverse_g<float>(N, legs, nlegs, g1_tiles, g2_tiles, grid_cap, g2_div, tw, st);
 }
 
 int run_zc_plain(int fp64, int N, const void *gd, const void *gi, int Kz, double *accD, double *accI, int mode,

@@ -103,3 +103,25 @@ def test_batch_matches_plain_path(gpu_lib):
         v, _, _ = b.execute()
         b.close()
     np.testing.assert_allclose(v, plain, rtol=1e-11)
+
+
+def test_autotune_and_params(gpu_lib):
+    """Tuned launch shapes change only summation order, not results."""
+    P = gpu_lib
+    N, L = 32, 100.0
+    f = lognormal_field(N, 77)
+    kf = 2 * math.pi / L
+    polys = _polys_triangles(kf, [(4.0, 4.0)], [0.2, 1.0, 2.0], 0)
+    with P.Context(N, L, P.PREC_FP64) as ctx:
+        ctx.load_field(f)
+        b0 = ctx.batch(polys, 1.0, 0, params={"grid_cap": 64, "chunk_tiles": 1, "g2_div": 2})
+        v0, _, _ = b0.execute()
+        assert b0.params == {"grid_cap": 64, "chunk_tiles": 1, "g2_div": 2}
+        b0.close()
+        best, ms = ctx.autotune(polys, 1.0, 0, reps=1)
+        assert ms > 0 and best["grid_cap"] > 0
+        b1 = ctx.batch(polys, 1.0, 0)
+        assert b1.params == best
+        v1, _, _ = b1.execute()
+        b1.close()
+    np.testing.assert_allclose(v1, v0, rtol=1e-13)
