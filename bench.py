@@ -165,6 +165,14 @@ def main():
     names = ["G1 masked gather + ky inverse", "G2 kx inverse (band -> all x)", "Zc fused C2R + product-reduce",
              "final reduce"]
     dom = int(np.argmax(stage_ms))
+    kernel_of_stage = ["ps::k_inv_g1<float, 512>", "ps::k_inv_g2<float, 512, 1>", "ps::k_inv_zc_fused<float, 512>",
+                       "ps::k_reduce_partials"]
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "bispectrum512_hbm_traffic.json")
+    if N == 512 and os.path.exists(tfile):
+        # HBM bytes per launch from the committed FETCH_SIZE/WRITE_SIZE summary
+        t = json.load(open(tfile))["kernel_traffic_GB"].get(kernel_of_stage[dom])
+        traffic = None if t is None else round(t * 1e9)
     fwd_ms, fwd_bytes = ctx.forward_profile(reps=3)
     stages = {names[i]: {"ms": round(float(stage_ms[i]), 4), "GB": round(float(stage_bytes[i]) / 1e9, 4),
                          "GB/s": round(float(stage_bytes[i] / stage_ms[i] / 1e6), 1)} for i in range(4)}
@@ -210,7 +218,8 @@ def main():
                      "achieved": round(float(stage_bytes[dom] / stage_ms[dom] / 1e6), 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(float(stage_bytes[dom] / stage_ms[dom] / 1e6 / HBM_PEAK_GBS), 4),
-                     "traffic": None},
+                     "traffic": traffic, "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE+WRITE_SIZE, "
+                                                         "copy-kernel scaled; profiles/bispectrum512_hbm_traffic.md)"},
         "stages": stages,
         "forward_fft": {"ms": round(fwd_ms, 4), "GB": round(fwd_bytes / 1e9, 4),
                         "GB/s": round(fwd_bytes / fwd_ms / 1e6, 1),

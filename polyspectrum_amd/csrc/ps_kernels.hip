@@ -328,30 +328,33 @@ __global__ void __launch_bounds__(NT) k_inv_zc_plain(const cplx<T> *__restrict__
 // buffers of kz_cap planes (kz_cap = 1 + the largest Kz of the batch).
 template <typename T, int N> struct ZcCfg {
     static constexpr int LW = wave_lines<T, N>();
-    static constexpr int NWV = NT / 64;
+    // 4 wavefronts: measured faster at 512^3 than 8 (whose 16-row tiles read
+    // whole 128 B lines per plane but synchronise twice as many wavefronts)
+    static constexpr int NWV = 4;
+    static constexpr int BT = 64 * NWV;  // threads per workgroup
     static constexpr int ROWS = NWV * LW;
     static constexpr int SP = ROWS + 1;  // staging pitch
     static constexpr int NH = N / 2 + 1;
     // staged values per thread when prefetching (legs with more planes are
     // staged without overlap)
-    static constexpr int MPT = ((NH * ROWS + NT - 1) / NT < 4) ? (NH * ROWS + NT - 1) / NT : 4;
+    static constexpr int MPT = ((NH * ROWS + BT - 1) / BT < 4) ? (NH * ROWS + BT - 1) / BT : 4;
 };
 
 template <typename T, int N>
-__global__ void __launch_bounds__(NT) k_inv_zc_fused(const GroupDev *__restrict__ groups, int ngroups,
+__global__ void __launch_bounds__((ZcCfg<T, N>::BT)) k_inv_zc_fused(const GroupDev *__restrict__ groups, int ngroups,
                                                      const LegDev *__restrict__ legs,
                                                      const int *__restrict__ suffix, int64_t nitems,
                                                      double2 *__restrict__ partials, int max_chunks,
-                                                     const cplx<T> *__restrict__ twg, int kz_cap)
+                                                     const cplx<T> *__restrict__ twg, int kz_cap, int nbuf)
 {
     using C = ZcCfg<T, N>;
-    constexpr int LW = C::LW, NWV = C::NWV, ROWS = C::ROWS, SP = C::SP, MPT = C::MPT;
+    constexpr int LW = C::LW, NWV = C::NWV, ROWS = C::ROWS, SP = C::SP, MPT = C::MPT, BT = C::BT;
     constexpr int LP = wave_pitch<N>();
     using WO = WaveOut<T, N, LW>;
     extern __shared__ unsigned char dyn_lds[];
     __shared__ cplx<T> wbuf[NWV * LW * LP];
     __shared__ double acc[NWV][MAX_SUFFIX][4];
-    cplx<T> *stage_buf = reinterpret_cast<cplx<T> *>(dyn_lds);  // [2 buffers][2 parts][kz_cap * SP]
+    cplx<T> *stage_buf = reinterpret_cast<cplx<T> *>(dyn_lds);  // [nbuf buffers][2 parts][kz_cap * SP]
     const int part = kz_cap * SP;
     const int tid = threadIdx.x, w = tid / 64, lane = tid & 63;
     WaveTw<T, N, LW> tw;
@@ -367,7 +370,7 @@ __global__ void __launch_bounds__(NT) k_inv_zc_fused(const GroupDev *__restrict_
         const int64_t t_end = (t_begin + G.chunk_tiles < ntile) ? t_begin + G.chunk_tiles : ntile;
         const int nleg = G.npre + G.nsuf;
         const int64_t nsteps = (t_end - t_begin) * nleg;  // (tile, leg) steps of this item
-        for (int i = tid; i < NWV * MAX_SUFFIX * 4; i += NT) (&acc[0][0][0])[i] = 0.0;
+        for (int i = tid; i < NWV * MAX_SUFFIX * 4; i += BT) (&acc[0][0][0])[i] = 0.0;
         // staged copy of step s = (tile, leg) into buffer b
         auto leg_of = [&](int q) -> const LegDev & {
             return legs[q < G.npre ? (q == 0 ? G.preA : G.preB) : suffix[G.suf_begin + q - G.npre]];
@@ -383,7 +386,7 @@ __global__ void __launch_bounds__(NT) k_inv_zc_fused(const GroupDev *__restrict_
             const cplx<T> *gi = reinterpret_cast<const cplx<T> *>(L.g2i);
 #pragma unroll
             for (int m = 0; m < MPT; ++m) {
-                const int idx = tid + m * NT;
+                const int idx = tid + m * BT;
                 pfD[m] = mk<T>(0, 0);
                 pfI[m] = mk<T>(0, 0);
                 if (idx < cnt) {
@@ -411,12 +414,12 @@ __global__ void __launch_bounds__(NT) k_inv_zc_fused(const GroupDev *__restrict_
             };
 #pragma unroll
             for (int m = 0; m < MPT; ++m) {
-                const int idx = tid + m * NT;
+                const int idx = tid + m * BT;
                 if (idx < cnt) put(idx / ROWS, idx % ROWS, pfD[m], pfI[m]);
             }
             const cplx<T> *gd = reinterpret_cast<const cplx<T> *>(L.g2d);
             const cplx<T> *gi = reinterpret_cast<const cplx<T> *>(L.g2i);
-            for (int idx = tid + MPT * NT; idx < cnt; idx += NT) {
+            for (int idx = tid + MPT * BT; idx < cnt; idx += BT) {
                 const int row = idx % ROWS, kz = idx / ROWS;
                 const int64_t o = ((int64_t)kz * N + x) * N + y0 + row;
                 put(kz, row, gd ? gd[o] : mk<T>(0, 0), gi ? gi[o] : mk<T>(0, 0));
@@ -431,7 +434,7 @@ __global__ void __launch_bounds__(NT) k_inv_zc_fused(const GroupDev *__restrict_
         // of the transforms; each suffix product is then formed in FP64
         T pd[WO::IT][WO::R], pi[WO::IT][WO::R];
         for (int64_t s = 0; s < nsteps; ++s) {
-            const int b = (int)(s & 1);
+            const int b = (nbuf == 2) ? (int)(s & 1) : 0;
             const int qn = (q + 1 == nleg) ? 0 : q + 1;
             const int64_t tn = (qn == 0) ? tcur + 1 : tcur;
             if (s + 1 < nsteps) fetch(tn, qn);
@@ -477,12 +480,13 @@ __global__ void __launch_bounds__(NT) k_inv_zc_fused(const GroupDev *__restrict_
                     acc[w][o][0] = a.s; acc[w][o][1] = a.c; acc[w][o][2] = c.s; acc[w][o][3] = c.c;
                 }
             }
-            if (s + 1 < nsteps) store(tn, qn, b ^ 1);
+            if (nbuf == 1) __syncthreads();  // one buffer: every wavefront has read it
+            if (s + 1 < nsteps) store(tn, qn, (nbuf == 2) ? (b ^ 1) : 0);
             __syncthreads();
             q = qn;
             tcur = tn;
         }
-        for (int o = tid; o < G.nsuf; o += NT) {
+        for (int o = tid; o < G.nsuf; o += BT) {
             KSum a, c;
             a.init();
             c.init();
@@ -758,13 +762,27 @@ static int launch_zc_fused_n(const GroupDev *groups, int ngroups, const LegDev *
 {
     using C = ZcCfg<T, N>;
     if (kz_cap < 1 || kz_cap > C::NH) kz_cap = C::NH;
-    const size_t dyn = sizeof(cplx<T>) * 4 * (size_t)kz_cap * C::SP;
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_inv_zc_fused<T, N>),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
+    // two staging buffers (prefetch overlapped with the transforms) when they
+    // fit next to the static LDS, else one
+    hipFuncAttributes fa;
+    hipError_t e = hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(&k_inv_zc_fused<T, N>));
     if (e != hipSuccess) return (int)e;
-    const unsigned g = (unsigned)((nitems < grid_cap) ? nitems : grid_cap);
-    hipLaunchKernelGGL((k_inv_zc_fused<T, N>), dim3(g), dim3(NT), dyn, st, groups, ngroups, legs, suffix, nitems,
-                       partials, max_chunks, reinterpret_cast<const cplx<T> *>(tw), kz_cap);
+    int dev = 0, lds_max = 0;
+    e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+    if (e != hipSuccess) return (int)e;
+    const size_t per_buf = sizeof(cplx<T>) * 2 * (size_t)kz_cap * C::SP;
+    int nbuf = 2;
+    if (fa.sharedSizeBytes + 2 * per_buf > (size_t)lds_max) nbuf = 1;
+    if (fa.sharedSizeBytes + nbuf * per_buf > (size_t)lds_max) return PS_ERR_UNSUPPORTED_N;
+    const size_t dyn = nbuf * per_buf;
+    e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_inv_zc_fused<T, N>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
+    if (e != hipSuccess) return (int)e;
+    const int64_t cap = (C::NWV == 8) ? (grid_cap + 1) / 2 : grid_cap;  // same thread count in flight
+    const unsigned g = (unsigned)((nitems < cap) ? nitems : cap);
+    hipLaunchKernelGGL((k_inv_zc_fused<T, N>), dim3(g), dim3(C::BT), dyn, st, groups, ngroups, legs, suffix, nitems,
+                       partials, max_chunks, reinterpret_cast<const cplx<T> *>(tw), kz_cap, nbuf);
     return (int)hipGetLastError();
 }
 
@@ -835,7 +853,7 @@ int tile_rows(int N, int fp64) { return tile_lines_for(N, fp64 != 0); }
 int fused_tile_rows(int N, int fp64)
 {
     switch (N) {
-#define CASE(n) case n: return 4 * (fp64 ? wave_lines<double, n>() : wave_lines<float, n>());
+#define CASE(n) case n: return fp64 ? ZcCfg<double, n>::ROWS : ZcCfg<float, n>::ROWS;
         PS_FOR_EACH_N(CASE)
 #undef CASE
     }
