@@ -357,8 +357,9 @@ __global__ void __launch_bounds__((ZcCfg<T, N>::BT)) k_inv_zc_fused(const GroupD
     cplx<T> *stage_buf = reinterpret_cast<cplx<T> *>(dyn_lds);  // [nbuf buffers][2 parts][kz_cap * SP]
     const int part = kz_cap * SP;
     const int tid = threadIdx.x, w = tid / 64, lane = tid & 63;
-    WaveTw<T, N, LW> tw;
-    tw.template load<+1>(twg, lane);
+    WaveTw<T, N, LW> twr;
+    twr.template load<+1>(twg, lane);
+    const RegTw<T, N, LW> tw{twr};
     cplx<T> *wb = wbuf + w * LW * LP;
     constexpr int tiles_per_x = N / ROWS;
     const int64_t ntile = (int64_t)N * tiles_per_x;
@@ -453,24 +454,37 @@ __global__ void __launch_bounds__((ZcCfg<T, N>::BT)) k_inv_zc_fused(const GroupD
                 if (k > Kz) return mk<T>(0, 0);
                 return (lo ? sD : sI)[k * SP + row];
             };
+            // the squared-leg flags are uniform: each choice is its own
+            // instantiation, so no per-element selects enter the transforms
+            const bool edge = Kz < N / stage_radix(N, 0);
+            auto run = [&](auto &sink) {
+                if (edge) wave_fft<T, N, LW, +1, 0, true, RegTw<T, N, LW>>(wb, tw, lane, src, sink);
+                else wave_fft<T, N, LW, +1, 0, false, RegTw<T, N, LW>>(wb, tw, lane, src, sink);
+            };
             if (q < G.npre) {
-                const bool sq = G.pre_sq != 0;
-                auto sink = [&](int i, int r, int, int, cplx<T> v) {
-                    pd[i][r] *= sq ? v.x * v.x : v.x;
-                    pi[i][r] *= sq ? v.y * v.y : v.y;
+                auto sink1 = [&](int i, int r, int, int, cplx<T> v) {
+                    pd[i][r] *= v.x;
+                    pi[i][r] *= v.y;
                 };
-                if (Kz < N / stage_radix(N, 0)) wave_fft<T, N, LW, +1, 0, true>(wb, tw, lane, src, sink);
-                else wave_fft<T, N, LW, +1, 0, false>(wb, tw, lane, src, sink);
+                auto sink2 = [&](int i, int r, int, int, cplx<T> v) {
+                    pd[i][r] *= v.x * v.x;
+                    pi[i][r] *= v.y * v.y;
+                };
+                if (G.pre_sq) run(sink2);
+                else run(sink1);
             } else {
                 double sd = 0.0, si = 0.0;
-                const bool sq = G.suf_sq != 0;
-                auto sink = [&](int i, int r, int, int, cplx<T> v) {
-                    const double dv = (double)v.x, iv = (double)v.y;
-                    sd += (double)pd[i][r] * (sq ? dv * dv : dv);
-                    si += (double)pi[i][r] * (sq ? iv * iv : iv);
+                auto sink1 = [&](int i, int r, int, int, cplx<T> v) {
+                    sd += (double)pd[i][r] * (double)v.x;
+                    si += (double)pi[i][r] * (double)v.y;
                 };
-                if (Kz < N / stage_radix(N, 0)) wave_fft<T, N, LW, +1, 0, true>(wb, tw, lane, src, sink);
-                else wave_fft<T, N, LW, +1, 0, false>(wb, tw, lane, src, sink);
+                auto sink2 = [&](int i, int r, int, int, cplx<T> v) {
+                    const double dv = (double)v.x, iv = (double)v.y;
+                    sd += (double)pd[i][r] * (dv * dv);
+                    si += (double)pi[i][r] * (iv * iv);
+                };
+                if (G.suf_sq) run(sink2);
+                else run(sink1);
                 sd = wave_sum(sd);
                 si = wave_sum(si);
                 if (lane == 0) {

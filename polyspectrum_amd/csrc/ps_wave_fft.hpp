@@ -37,6 +37,10 @@ template <typename T, int N> constexpr int wave_lines()
 
 template <int N> constexpr int wave_pitch() { return N + N / 8 + 1; }
 __device__ __forceinline__ int wpad(int e) { return e + (e >> 3); }
+// wpad(e + s) = wpad(e) + s*9/8 when s is a multiple of 8, and = wpad(e) + s
+// when e is a multiple of 8 and s < 8: the stages below use that to turn the
+// per-r addresses into constant offsets of the LDS instructions
+
 
 template <typename T, int N, int LW, int S> struct WaveStage {
     static constexpr int R = stage_radix(N, S);
@@ -98,15 +102,32 @@ template <typename T, int N, int LW> struct WaveTw {
     }
 };
 
+// Twiddles read from a table exp(-2 pi i t/N) in LDS instead of registers.
+template <typename T, int N, int LW> struct LdsTw {
+    const cplx<T> *tab;
+    template <int S, int SIGN> __device__ __forceinline__ cplx<T> get(int i, int r, int j) const
+    {
+        using St = WaveStage<T, N, LW, S>;
+        return twiddle<T, SIGN>(tab, r * (j % St::NS) * (N / (St::NS * St::R)));
+    }
+};
+
+template <typename T, int N, int LW> struct RegTw {
+    const WaveTw<T, N, LW> &w;
+    template <int S, int SIGN> __device__ __forceinline__ cplx<T> get(int i, int r, int) const
+    {
+        return w.w[WaveTw<T, N, LW>::template at<S>(i, r)];
+    }
+};
+
 // Inverse (SIGN = +1) or forward (SIGN = -1) transform of LW lines.
 //   src(l, e)          -> input element e of line l (first stage)
 //   sink(i, r, l, z, v) <- output z of line l (held as v[i][r] of this thread)
 //   EDGE (first stage only): inputs e with NB <= e < N - NB are known to be
 //   zero (a spectrum band-limited to |k| < N/R_0), so only r = 0 and r = R-1
 //   are read and the butterfly is constant-folded.
-template <typename T, int N, int LW, int SIGN, int S, bool EDGE = false, class Src, class Sink>
-__device__ __forceinline__ void wave_fft(cplx<T> *__restrict__ wbuf, const WaveTw<T, N, LW> &tw, int lane,
-                                         Src &src, Sink &sink)
+template <typename T, int N, int LW, int SIGN, int S, bool EDGE = false, class Tw, class Src, class Sink>
+__device__ __forceinline__ void wave_fft(cplx<T> *__restrict__ wbuf, const Tw &tw, int lane, Src &src, Sink &sink)
 {
     using St = WaveStage<T, N, LW, S>;
     constexpr int R = St::R, NS = St::NS, NB = St::NB, TOT = St::TOT, IT = St::IT;
@@ -122,14 +143,15 @@ __device__ __forceinline__ void wave_fft(cplx<T> *__restrict__ wbuf, const WaveT
                 if constexpr (S == 0) {
                     if (EDGE && r != 0 && r != R - 1) v[i][r] = mk<T>(0, 0);
                     else v[i][r] = src(l, j + r * NB);
+                } else if constexpr (NB % 8 == 0) {
+                    v[i][r] = wbuf[l * LP + wpad(j) + r * (NB + NB / 8)];
                 } else {
                     v[i][r] = wbuf[l * LP + wpad(j + r * NB)];
                 }
             }
             if constexpr (NS > 1) {
 #pragma unroll
-                for (int r = 1; r < R; ++r)
-                    v[i][r] = cmul<T>(v[i][r], tw.w[WaveTw<T, N, LW>::template at<S>(i, r)]);
+                for (int r = 1; r < R; ++r) v[i][r] = cmul<T>(v[i][r], tw.template get<S, SIGN>(i, r, j));
             }
             dft<T, R, SIGN>(v[i]);
         }
@@ -152,12 +174,22 @@ __device__ __forceinline__ void wave_fft(cplx<T> *__restrict__ wbuf, const WaveT
             if (TOT % 64 == 0 || b < TOT) {
                 const int j = b % NB, l = b / NB;
                 const int d = (j / NS) * NS * R + j % NS;
+                if constexpr (NS % 8 == 0) {
+                    const int base = l * LP + wpad(d);
 #pragma unroll
-                for (int r = 0; r < R; ++r) wbuf[l * LP + wpad(d + r * NS)] = v[i][r];
+                    for (int r = 0; r < R; ++r) wbuf[base + r * (NS + NS / 8)] = v[i][r];
+                } else if constexpr (NS == 1 && R == 8) {
+                    const int base = l * LP + wpad(d);  // d = 8 j
+#pragma unroll
+                    for (int r = 0; r < R; ++r) wbuf[base + r] = v[i][r];
+                } else {
+#pragma unroll
+                    for (int r = 0; r < R; ++r) wbuf[l * LP + wpad(d + r * NS)] = v[i][r];
+                }
             }
         }
         wave_sync();
-        wave_fft<T, N, LW, SIGN, S + 1, false>(wbuf, tw, lane, src, sink);
+        wave_fft<T, N, LW, SIGN, S + 1, false, Tw>(wbuf, tw, lane, src, sink);
     }
 }
 
