@@ -1,0 +1,90 @@
+"""Parity at the benchmark configurations' own sizes.
+
+* 128^3 log-normal (sigma 0.8, P ~ k^-1.5), L = 100: n = 3, k1 = k2 = 0.5 h/Mpc,
+  10 theta bins (DEFAULT) and the n = 2 power spectrum in 64 bins.
+* 256^3 binary ionisation map (float64 input, 50% filling), L = 200: equilateral
+  triangles k = m k_F for m = 1..64, Npolygons written.
+
+The oracle (FP64, one core) is run on sampled configurations; every
+configuration is checked through size-independent properties: integer polygon
+counts, reference-shaped FP64 path = batched FP64 path, mixed within 1e-5 of
+FP64 (relative to the largest |B| of the sweep where B crosses zero).
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+from polyspectrum_amd.synthetic import ionisation_field, lognormal_field
+
+pytestmark = pytest.mark.gpu
+PI = math.acos(-1.0)
+
+
+def _tri(k1, k2, th):
+    return {"n": 3, "k": [k1, k2, math.sqrt(k1 * k1 + k2 * k2 - 2. * k1 * k2 * math.cos(th))],
+            "kmin": [0.0] * 3, "kmax": [0.0] * 3}
+
+
+def _batch(P, N, L, prec, field, polys):
+    with P.Context(N, L, prec) as ctx:
+        ctx.load_field(field)
+        b = ctx.batch(polys, 1.0, 0)
+        v, npg, _ = b.execute()
+        b.close()
+        plain = None
+        if prec == P.PREC_FP64:
+            plain = [ctx.polyspectrum(p["n"], p["k"], p["kmin"], p["kmax"], 1.0, 0)[0] for p in polys[:3]]
+    return v, npg, plain
+
+
+def test_config_128_bispectrum_and_power(gpu_lib):
+    P = gpu_lib
+    N, L = 128, 100.0
+    field = lognormal_field(N, 0, sigma=0.8, alpha=1.5)
+    tris = [_tri(0.5, 0.5, PI * i / 10) for i in range(10)]
+    kf = 2 * PI / L
+    pk = [{"n": 2, "k": [(i + 1) * kf] * 2, "kmin": [(i + 1) * kf] * 2, "kmax": [(i + 1) * kf] * 2}
+          for i in range(N // 2)]
+    polys = tris + pk
+    v64, n64, plain = _batch(P, N, L, P.PREC_FP64, field, polys)
+    v32, n32, _ = _batch(P, N, L, P.PREC_MIXED, field, polys)
+    spec = O.forward_spectrum(field)
+    for i in (0, 4, 9, 10, 20, 73):
+        p = polys[i]
+        ref = O.polyspectrum(spec, p["n"], p["k"], p["kmin"], p["kmax"], 1.0, 0, L)[0]
+        assert abs(v64[i] - ref) <= 1e-10 * abs(ref), (i, v64[i], ref)
+    np.testing.assert_allclose(plain, v64[:3], rtol=1e-11)
+    assert np.all(np.abs(n64 - np.round(n64)) < 1e-6 * np.maximum(1, n64))
+    tri_err = np.abs(v32[:10] - v64[:10]).max() / np.abs(v64[:10]).max()
+    pk_err = (np.abs(v32[10:] - v64[10:]) / np.abs(v64[10:])).max()
+    print("mixed: bispectrum", tri_err, "power", pk_err)
+    assert tri_err < 1e-5 and pk_err < 1e-5
+
+
+def test_config_256_equilateral_xhii(gpu_lib):
+    P = gpu_lib
+    N, L = 256, 200.0
+    field = ionisation_field(N, 1000)
+    assert field.dtype == np.float64 and abs(field.mean() - 0.5) < 1e-3
+    kf = 2 * PI / L
+    polys = [{"n": 3, "k": [m * kf] * 3, "kmin": [0.0] * 3, "kmax": [0.0] * 3} for m in range(1, 65)]
+    v64, n64, plain = _batch(P, N, L, P.PREC_FP64, field, polys)
+    spec = O.forward_spectrum(field)
+    for m in (3, 17):
+        p = polys[m - 1]
+        ref, _, _, sums = O.polyspectrum(spec, 3, p["k"], p["kmin"], p["kmax"], 1.0, 0, L)
+        assert abs(v64[m - 1] - ref) <= 1e-10 * abs(ref)
+        assert abs(n64[m - 1] - sums[1].real / N ** 3) < 1e-6 * n64[m - 1]
+    np.testing.assert_allclose(plain, v64[:3], rtol=1e-11)
+    assert np.all(n64 > 0) and np.all(np.abs(n64 - np.round(n64)) < 1e-6 * n64)
+    v32, n32, _ = _batch(P, N, L, P.PREC_MIXED, field, polys)
+    err = np.abs(v32 - v64) / np.abs(v64)
+    scale_err = np.abs(v32 - v64) / np.abs(v64).max()
+    worst = int(np.argmax(err))
+    print("mixed equilateral: max", err.max(), "at m =", worst + 1, "|B|/max|B| =",
+          abs(v64[worst]) / np.abs(v64).max(), "median", np.median(err))
+    # B(m) of a binary map is small at some m: bound those relative to the sweep
+    assert np.median(err) < 1e-6 and scale_err.max() < 1e-5
+    assert np.mean(err <= 1e-5) >= 0.95
