@@ -105,6 +105,76 @@ def cpu_baseline(N, L):
     }
 
 
+def realisation_batch(P, nper=16, sizes=(128, 192, 256)):
+    """Uneven batch of realisations: nper seeded log-normal fields (sigma 0.8) at
+    each size, boxes of 1 Mpc/h cells; per field the power spectrum in 32 bins
+    k_i = (i+1) k_F and the phase-only bispectrum at 16 theta bins for
+    k1 = k2 = 0.3 h/Mpc, DERIVED binning; every batch in one captured graph."""
+    from polyspectrum_amd.synthetic import lognormal_field
+    print("[bench] realisation batch", file=sys.stderr, flush=True)
+    ctxs, batches, npoly = [], [], 0
+    for N in sizes:
+        L = float(N)
+        kf = 2 * math.pi / L
+        ctx = P.Context(N, L, P.PREC_MIXED, nslots=nper)
+        for r in range(nper):
+            ctx.load_field(lognormal_field(N, 3000 + 100 * N + r, sigma=0.8, alpha=1.5), slot=r)
+        pk = [{"n": 2, "slot": r, "k": [(i + 1) * kf] * 2, "kmin": [(i + 1) * kf] * 2, "kmax": [(i + 1) * kf] * 2}
+              for r in range(nper) for i in range(32)]
+        k = 0.3
+        bs = []
+        for r in range(nper):
+            for i in range(16):
+                th = math.pi * i / 16
+                bs.append({"n": 3, "slot": r, "k": [k, k, math.sqrt(2 * k * k - 2 * k * k * math.cos(th))],
+                           "kmin": [0.0] * 3, "kmax": [0.0] * 3})
+        ctx.set_phase_only(0)
+        batches.append(ctx.batch(pk, 1.0, P.BINNING["DERIVED_BINNING"]))
+        ctx.set_phase_only(1)
+        batches.append(ctx.batch(bs, 1.0, P.BINNING["DERIVED_BINNING"]))
+        npoly += len(pk) + len(bs)
+        ctxs.append(ctx)
+    mb = P.MultiBatch(batches)
+    mb.execute()
+    t0 = time.perf_counter()
+    reps = 5
+    for _ in range(reps):
+        mb.execute()
+    ms = 1e3 * (time.perf_counter() - t0) / reps
+    out = {"fields": nper * len(sizes), "sizes": list(sizes), "configurations": npoly, "ms_per_replay": round(ms, 3),
+           "configurations_per_s": round(npoly / (ms / 1e3), 1), "fields_per_s": round(nper * len(sizes) / (ms / 1e3), 1)}
+    mb.close()
+    for b in batches:
+        b.close()
+    for c in ctxs:
+        c.close()
+    return out
+
+
+def adjoint_timing(P, N):
+    """dL/d delta for L = sum_t (B_t - B*_t)^2 over 16 triangles (k1 = k2 = 0.4
+    h/Mpc, 16 theta bins, DEFAULT) of a 512^3 log-normal field (sigma 0.8)."""
+    from polyspectrum_amd.synthetic import lognormal_field
+    print("[bench] adjoint", file=sys.stderr, flush=True)
+    L = 400.0 * N / 512
+    k = 0.4
+    polys = [{"n": 3, "k": [k, k, math.sqrt(2 * k * k - 2 * k * k * math.cos(math.pi * i / 16))],
+              "kmin": [0.0] * 3, "kmax": [0.0] * 3} for i in range(16)]
+    with P.Context(N, L, P.PREC_MIXED) as ctx:
+        ctx.load_field(lognormal_field(N, 4001, sigma=0.8, alpha=1.5))
+        b = ctx.batch(polys, 1.0, 0)
+        target, _, _ = b.execute()
+        ctx.load_field(lognormal_field(N, 4000, sigma=0.8, alpha=1.5))
+        v, _, _ = b.execute()
+        t0 = time.perf_counter()
+        g = b.adjoint(2.0 * (v - target))
+        ms = 1e3 * (time.perf_counter() - t0)
+        b.close()
+    return {"triangles": 16, "grid": N, "ms_per_gradient": round(ms, 2),
+            "includes": "5 full 512^3 transforms, mask applications and products + device-to-host copy of the float64 gradient",
+            "grad_rms": float(np.sqrt(np.mean(g ** 2)))}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -114,6 +184,8 @@ def main():
     ap.add_argument("--plain-samples", type=int, default=4)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--skip-extras", action="store_true",
+                    help="skip the realisation-batch and adjoint measurements")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy 5 times (scale for the FETCH/WRITE counters)")
     args = ap.parse_args()
@@ -234,6 +306,9 @@ def main():
         "launch_params": {"used": used_params, "autotuned": tuned is not None,
                           "autotune_best_ms": tuned_ms},
     }
+    if not args.skip_extras:
+        out["realisations"] = realisation_batch(P)
+        out["adjoint"] = adjoint_timing(P, N)
     if not args.skip_cpu:
         print("[bench] CPU baseline (oracle, 1 core)", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(N, L)
