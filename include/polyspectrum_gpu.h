@@ -123,12 +123,16 @@ int ps_batch_create(ps_ctx *ctx, ps_polygon *polys, int npoly, double kbinwidth,
                     ps_batch **out);
 
 /* Launch shape of a batch: persistent grid size of the pass kernels, Zc tiles
- * per work item (sets how many compensated partial sums each polygon has)
- * and the divisor of the G2 tile width (1: 16 FP32 / 8 FP64 columns, 2: half). */
+ * per work item (sets how many compensated partial sums each polygon has),
+ * the divisor of the G2 tile width (1: 16 FP32 / 8 FP64 columns, 2: half) and
+ * the schedule (overlap 0: each pass over all legs in turn on one stream;
+ * 1: per polygon group, the product pass of group g on a second stream beside
+ * the inverse transforms of the legs group g+1 introduces). */
 typedef struct ps_launch_params {
     int grid_cap;
     int chunk_tiles;
     int g2_div;
+    int overlap;
 } ps_launch_params;
 
 /* ps_batch_create with explicit launch parameters (NULL: the tuned ones for

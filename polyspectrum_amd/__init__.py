@@ -40,8 +40,14 @@ class Polygon(ctypes.Structure):
 
 
 class LaunchParams(ctypes.Structure):
-    """ps_launch_params: persistent grid size, Zc tiles per work item, G2 tile divisor."""
-    _fields_ = [("grid_cap", ctypes.c_int), ("chunk_tiles", ctypes.c_int), ("g2_div", ctypes.c_int)]
+    """ps_launch_params: persistent grid size, Zc tiles per work item, G2 tile
+    divisor, two-stream overlapped schedule (0/1)."""
+    _fields_ = [("grid_cap", ctypes.c_int), ("chunk_tiles", ctypes.c_int), ("g2_div", ctypes.c_int),
+                ("overlap", ctypes.c_int)]
+
+
+def _params_dict(lp):
+    return {"grid_cap": lp.grid_cap, "chunk_tiles": lp.chunk_tiles, "g2_div": lp.g2_div, "overlap": lp.overlap}
 
 
 def _polygon_array(polygons):
@@ -255,13 +261,13 @@ class Context:
 
     def autotune(self, polygons, kbinwidth=1.0, kbinning_case=0, reps=2):
         """Times candidate launch shapes; later batches of this shape use the best.
-        Returns ({grid_cap, chunk_tiles, g2_div}, ms)."""
+        Returns ({grid_cap, chunk_tiles, g2_div, overlap}, ms)."""
         arr = _polygon_array(polygons)
         lp = LaunchParams()
         ms = ctypes.c_double()
         _check(load_library().ps_batch_autotune(self._h, arr, len(polygons), float(kbinwidth), int(kbinning_case),
                                                 int(reps), ctypes.byref(lp), ctypes.byref(ms)))
-        return {"grid_cap": lp.grid_cap, "chunk_tiles": lp.chunk_tiles, "g2_div": lp.g2_div}, ms.value
+        return _params_dict(lp), ms.value
 
     def batch_for_field(self, polygons, field, kbinwidth=1.0, kbinning_case=0, slot=0):
         """Load a field into the slot, then plan a batch on it."""
@@ -277,7 +283,8 @@ class Batch:
         h = _vp()
         lp = None
         if params is not None:
-            lp = LaunchParams(int(params["grid_cap"]), int(params["chunk_tiles"]), int(params.get("g2_div", 1)))
+            lp = LaunchParams(int(params["grid_cap"]), int(params["chunk_tiles"]), int(params.get("g2_div", 1)),
+                              int(params.get("overlap", 0)))
         _check(load_library().ps_batch_create_ex(ctx._h, arr, len(polygons), float(kbinwidth), int(kbinning_case),
                                                  ctypes.byref(lp) if lp is not None else None, ctypes.byref(h)))
         self._h = h
@@ -315,7 +322,7 @@ class Batch:
     def params(self):
         lp = LaunchParams()
         _check(load_library().ps_batch_params(self._h, ctypes.byref(lp)))
-        return {"grid_cap": lp.grid_cap, "chunk_tiles": lp.chunk_tiles, "g2_div": lp.g2_div}
+        return _params_dict(lp)
 
     def results(self):
         """(values, npolygons, sums) of the last execution, no launch."""

@@ -60,6 +60,7 @@ struct ps_ctx {
     int device = 0;
     int cus = 256;
     hipStream_t st = nullptr;
+    hipStream_t st2 = nullptr;           // second stream: fused passes overlap the transforms
     void *tw = nullptr;                  // forward twiddles exp(-2 pi i t/N), N entries
     std::vector<void *> spec;            // half spectra per slot
     std::vector<int> loaded;
@@ -123,7 +124,9 @@ extern "C" int ps_ctx_destroy(ps_ctx *c)
     for (void *p : c->spec) if (p) (void)hipFree(p);
     void *bufs[] = {c->tw, c->staging, c->gd, c->gi, c->accD, c->accI, c->partials, c->dsum, c->dleg, c->dcorner};
     for (void *p : bufs) if (p) (void)hipFree(p);
+    if (c->st2) (void)hipStreamSynchronize(c->st2);
     if (c->st) (void)hipStreamDestroy(c->st);
+    if (c->st2) (void)hipStreamDestroy(c->st2);
     delete c;
     return PS_OK;
 }
@@ -145,7 +148,8 @@ extern "C" int ps_ctx_create(int nbins, double boxsize, int precision, int nslot
     int st = PS_OK;
     hipDeviceProp_t prop;
     if (hipGetDevice(&c->device) != hipSuccess || hipGetDeviceProperties(&prop, c->device) != hipSuccess ||
-        hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
+        hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->st2, hipStreamNonBlocking) != hipSuccess) {
         ps_ctx_destroy(c);
         return PS_ERR_HIP;
     }
@@ -369,7 +373,7 @@ static int plain_leg(ps_ctx *c, const MaskParams &mp, const void *spec, int mode
     const int tl = g_tile_lines(N, c->fp64);
     const int64_t g1 = (int64_t)(N / 2 + 1) * ((N + tl - 1) / tl);
     const int64_t g2 = 2 * (int64_t)(N / 2 + 1) * (N / tile_rows(N, c->fp64));
-    PSCK(run_inverse_g(c->fp64, N, c->dleg, 1, g1, g2, 1 << 30, 1, c->tw, c->st));
+    PSCK(run_inverse_g(c->fp64, N, c->dleg, 1, 0, g1, 0, g2, 1 << 30, 1, c->tw, c->st));
     PSCK(run_zc_plain(c->fp64, N, L.g2d, L.g2i, N / 2, spec ? c->accD : nullptr, c->accI, mode, c->partials, c->tw,
                       c->st));
     // the next leg overwrites the host-side descriptor buffer: keep order
@@ -430,6 +434,10 @@ extern "C" int ps_num_polygons(ps_ctx *c, int n, const double *k, double *kmin, 
 }
 
 // ---------------------------------------------------------------- batched path
+struct GroupRange {
+    int64_t g1a = 0, g1b = 0, g2a = 0, g2b = 0, i0 = 0, i1 = 0;
+};
+
 struct ps_batch {
     ps_ctx *c = nullptr;
     int npoly = 0;
@@ -446,6 +454,10 @@ struct ps_batch {
     int g2_div = 1;                      // G2 tile width divisor (1 or 2)
     int chunk_tiles = 64;                // Zc tiles per work item
     std::vector<void *> allocs;
+    int overlap = 0;                     // schedule, as ps_launch_params::overlap
+    std::vector<GroupRange> ranges;      // per group: tiles of the legs it introduces, its work items
+    std::vector<hipEvent_t> gev;         // per group: its legs are transformed (graph edges)
+    hipEvent_t zev = nullptr;            // every group's products are summed
     LegDev *dlegs = nullptr;
     GroupDev *dgroups = nullptr;
     int *dsuffix = nullptr;
@@ -472,6 +484,8 @@ extern "C" int ps_batch_destroy(ps_batch *b)
     if (b->graph) (void)hipGraphDestroy(b->graph);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
+    for (hipEvent_t e : b->gev) (void)hipEventDestroy(e);
+    if (b->zev) (void)hipEventDestroy(b->zev);
     for (void *p : b->allocs) (void)hipFree(p);
     void *bufs[] = {b->dlegs, b->dgroups, b->dsuffix, b->dcounts, b->dpartials, b->dsum, b->dscale};
     for (void *p : bufs) if (p) (void)hipFree(p);
@@ -484,19 +498,38 @@ static bool same_mask(const MaskParams &a, const MaskParams &b)
     return a.kind == b.kind && a.kkf == b.kkf && a.mmin == b.mmin && a.mmax == b.mmax && a.bw == b.bw;
 }
 
-// the launch sequence of one batch on a stream (captured into graphs)
-static int batch_record_on(ps_batch *b, hipStream_t s)
+// The launch sequence of one batch (captured into graphs).  Without overlap
+// each pass runs over every leg / group in turn on stream s.  With overlap the
+// HBM-bound G1/G2 passes of group g+1 run on stream s while the compute-bound
+// fused pass of group g runs on stream s2; events order each group's fused
+// pass after the transforms of the legs it reads.
+static int batch_record_on(ps_batch *b, hipStream_t s, hipStream_t s2)
 {
     ps_ctx *c = b->c;
-    if (b->dscale) PSCK(run_leg_scale(c->fp64, c->N, b->dlegs, (int)b->legs.size(), s));
-    PSCK(run_inverse_g(c->fp64, c->N, b->dlegs, (int)b->legs.size(), b->g1_tiles, b->g2_tiles, b->grid_cap, b->g2_div,
-                       c->tw, s));
-    PSCK(run_zc_fused(c->fp64, c->N, b->dgroups, (int)b->groups.size(), b->dlegs, b->dsuffix, b->items, b->dpartials,
-                      b->max_chunks, b->grid_cap, b->kz_cap, c->tw, s));
+    const int nl = (int)b->legs.size(), ng = (int)b->groups.size();
+    if (b->dscale) PSCK(run_leg_scale(c->fp64, c->N, b->dlegs, nl, s));
+    if (!b->overlap) {
+        PSCK(run_inverse_g(c->fp64, c->N, b->dlegs, nl, 0, b->g1_tiles, 0, b->g2_tiles, b->grid_cap, b->g2_div,
+                           c->tw, s));
+        PSCK(run_zc_fused(c->fp64, c->N, b->dgroups, ng, b->dlegs, b->dsuffix, 0, b->items, b->dpartials,
+                          b->max_chunks, b->grid_cap, b->kz_cap, c->tw, s));
+        PSCK(run_reduce_partials(b->dpartials, b->max_chunks, b->dcounts, 0, b->nout, b->dsum, b->dsum + b->nout, s));
+        return PS_OK;
+    }
+    for (int g = 0; g < ng; ++g) {
+        const GroupRange &R = b->ranges[g];
+        PSCK(run_inverse_g(c->fp64, c->N, b->dlegs, nl, R.g1a, R.g1b, R.g2a, R.g2b, b->grid_cap, b->g2_div, c->tw, s));
+        HIPCK(hipEventRecord(b->gev[g], s));
+        HIPCK(hipStreamWaitEvent(s2, b->gev[g], 0));
+        PSCK(run_zc_fused(c->fp64, c->N, b->dgroups, ng, b->dlegs, b->dsuffix, R.i0, R.i1, b->dpartials,
+                          b->max_chunks, b->grid_cap, b->kz_cap, c->tw, s2));
+    }
+    HIPCK(hipEventRecord(b->zev, s2));
+    HIPCK(hipStreamWaitEvent(s, b->zev, 0));
     PSCK(run_reduce_partials(b->dpartials, b->max_chunks, b->dcounts, 0, b->nout, b->dsum, b->dsum + b->nout, s));
     return PS_OK;
 }
-static int batch_record(ps_batch *b) { return batch_record_on(b, b->c->st); }
+static int batch_record(ps_batch *b) { return batch_record_on(b, b->c->st, b->c->st2); }
 
 // ---------------------------------------------------------------- launch-shape tuning
 // Best launch parameters found by ps_batch_autotune, per batch shape; kept for
@@ -528,7 +561,8 @@ static void tune_load()
     if (!fp) return;
     char key[256];
     ps_launch_params lp;
-    while (fscanf(fp, "%255s %d %d %d", key, &lp.grid_cap, &lp.chunk_tiles, &lp.g2_div) == 4) g_tuned[key] = lp;
+    while (fscanf(fp, "%255s %d %d %d %d", key, &lp.grid_cap, &lp.chunk_tiles, &lp.g2_div, &lp.overlap) == 5)
+        g_tuned[key] = lp;
     fclose(fp);
 }
 
@@ -538,6 +572,7 @@ static ps_launch_params default_params(const ps_ctx *c)
     lp.grid_cap = 8 * c->cus;
     lp.chunk_tiles = 64;
     lp.g2_div = 1;
+    lp.overlap = 0;
     if (const char *e = getenv("PS_CHUNK_TILES")) lp.chunk_tiles = std::max(1, atoi(e));
     if (const char *e = getenv("PS_GRID_CAP")) lp.grid_cap = std::max(1, atoi(e));
     return lp;
@@ -562,7 +597,9 @@ extern "C" int ps_batch_create_ex(ps_ctx *c, ps_polygon *polys, int npoly, doubl
         auto it = g_tuned.find(tune_key(c, polys, npoly, bw, kcase));
         if (it != g_tuned.end()) lp = it->second;
     }
-    if (lp.grid_cap < 1 || lp.chunk_tiles < 1 || (lp.g2_div != 1 && lp.g2_div != 2)) return PS_ERR_INVALID;
+    if (lp.grid_cap < 1 || lp.chunk_tiles < 1 || (lp.g2_div != 1 && lp.g2_div != 2) ||
+        (lp.overlap != 0 && lp.overlap != 1))
+        return PS_ERR_INVALID;
     for (int p = 0; p < npoly; ++p) {
         if (polys[p].n < 2 || polys[p].n > 3) return PS_ERR_INVALID;
         if (polys[p].slot < 0 || polys[p].slot >= c->nslots) return PS_ERR_INVALID;
@@ -574,6 +611,7 @@ extern "C" int ps_batch_create_ex(ps_ctx *c, ps_polygon *polys, int npoly, doubl
     b->npoly = npoly;
     b->grid_cap = lp.grid_cap;
     b->chunk_tiles = lp.chunk_tiles;
+    b->overlap = lp.overlap;
     b->g2_div = (tile_rows(c->N, c->fp64) / lp.g2_div >= 4) ? lp.g2_div : 1;
     // 1. legs of every polygon, de-duplicated by (slot, mask)
     std::vector<std::vector<int>> pleg(npoly);
@@ -595,6 +633,51 @@ extern "C" int ps_batch_create_ex(ps_ctx *c, ps_polygon *polys, int npoly, doubl
                 keys.push_back({P.slot, mp});
             }
             pleg[p].push_back(id);
+        }
+    }
+    // groups: polygons sharing their prefix legs, in a fixed order
+    std::map<std::tuple<int, int, int>, std::vector<int>> bykey;  // (npre-kind, a, b) -> polygons
+    for (int p = 0; p < npoly; ++p) {
+        const std::vector<int> &l = pleg[p];
+        if (polys[p].n == 3) bykey[{3, l[0], l[1]}].push_back(p);
+        else if (l[0] == l[1]) bykey[{1, -1, -1}].push_back(p);
+        else bykey[{2, l[0], -1}].push_back(p);
+    }
+    std::vector<std::pair<std::tuple<int, int, int>, std::vector<int>>> glist(bykey.begin(), bykey.end());
+    // order the legs by the first group that uses them, so that the legs a
+    // group introduces form one contiguous range of tiles
+    {
+        std::vector<int> newid(keys.size(), -1);
+        int nxt = 0;
+        auto take = [&](int id) {
+            if (id >= 0 && newid[id] < 0) newid[id] = nxt++;
+        };
+        for (auto &gkv : glist) {
+            const int kind = std::get<0>(gkv.first);
+            const std::vector<int> &ps_ = gkv.second;
+            for (size_t s0 = 0; s0 < ps_.size(); s0 += MAX_SUFFIX) {
+                if (kind == 3) {
+                    take(std::get<1>(gkv.first));
+                    take(std::get<2>(gkv.first));
+                } else if (kind == 2) {
+                    take(std::get<1>(gkv.first));
+                }
+                for (size_t q = s0; q < std::min(ps_.size(), s0 + MAX_SUFFIX); ++q) {
+                    const int pp = ps_[q];
+                    take(kind == 1 ? pleg[pp][0] : pleg[pp][polys[pp].n - 1]);
+                }
+            }
+        }
+        for (size_t i = 0; i < keys.size(); ++i) take((int)i);  // unused legs (none expected)
+        std::vector<std::pair<int, MaskParams>> k2(keys.size());
+        for (size_t i = 0; i < keys.size(); ++i) k2[newid[i]] = keys[i];
+        keys.swap(k2);
+        for (auto &l : pleg)
+            for (int &id : l) id = newid[id];
+        for (auto &gkv : glist) {
+            auto &t = gkv.first;
+            if (std::get<1>(t) >= 0) std::get<1>(t) = newid[std::get<1>(t)];
+            if (std::get<2>(t) >= 0) std::get<2>(t) = newid[std::get<2>(t)];
         }
     }
     for (auto &kv : keys) b->leg_slot.push_back(kv.first);
@@ -635,14 +718,7 @@ extern "C" int ps_batch_create_ex(ps_ctx *c, ps_polygon *polys, int npoly, doubl
         b->stage_bytes[0] += (double)cs * ((double)(K + 1) * L.Bx * std::min(2 * K + 1, N) + 2.0 * n1);
         b->stage_bytes[1] += (double)cs * (2.0 * n1 + 2.0 * n2);
     }
-    // 3. groups: polygons sharing their prefix legs
-    std::map<std::tuple<int, int, int>, std::vector<int>> bykey;  // (npre-kind, a, b) -> polygons
-    for (int p = 0; p < npoly; ++p) {
-        const std::vector<int> &l = pleg[p];
-        if (polys[p].n == 3) bykey[{3, l[0], l[1]}].push_back(p);
-        else if (l[0] == l[1]) bykey[{1, -1, -1}].push_back(p);
-        else bykey[{2, l[0], -1}].push_back(p);
-    }
+    // 3. group descriptors (legs renumbered above)
     const int rows = fused_tile_rows(N, c->fp64);
     const int64_t tiles_per_group = (int64_t)N * (N / rows);
     const int chunk_tiles = b->chunk_tiles;
@@ -651,7 +727,8 @@ extern "C" int ps_batch_create_ex(ps_ctx *c, ps_polygon *polys, int npoly, doubl
     b->poly_out.assign(npoly, -1);
     b->poly_n.assign(npoly, 0);
     std::vector<int> counts;
-    for (auto &kv : bykey) {
+    int legs_seen = 0;
+    for (auto &kv : glist) {
         const int kind = std::get<0>(kv.first);
         const std::vector<int> &ps_ = kv.second;
         for (size_t s0 = 0; s0 < ps_.size(); s0 += MAX_SUFFIX) {
@@ -685,6 +762,21 @@ extern "C" int ps_batch_create_ex(ps_ctx *c, ps_polygon *polys, int npoly, doubl
             }
             b->items += nchunks;
             b->groups.push_back(G);
+            // the legs this group introduces: [legs_seen, hi)
+            int hi = legs_seen;
+            if (G.npre >= 1) hi = std::max(hi, G.preA + 1);
+            if (G.npre == 2) hi = std::max(hi, G.preB + 1);
+            for (size_t q = s0; q < s1; ++q) hi = std::max(hi, b->suffix[G.suf_begin + (q - s0)] + 1);
+            GroupRange R;
+            R.g1a = legs_seen < (int)b->legs.size() ? b->legs[legs_seen].g1_begin : b->g1_tiles;
+            R.g1b = hi < (int)b->legs.size() ? b->legs[hi].g1_begin : b->g1_tiles;
+            R.g2a = legs_seen < (int)b->legs.size() ? b->legs[legs_seen].g2_begin : b->g2_tiles;
+            R.g2b = hi < (int)b->legs.size() ? b->legs[hi].g2_begin : b->g2_tiles;
+            if (hi <= legs_seen) R.g1b = R.g1a, R.g2b = R.g2a;
+            R.i0 = G.item_begin;
+            R.i1 = G.item_begin + nchunks;
+            b->ranges.push_back(R);
+            legs_seen = std::max(legs_seen, hi);
             // model bytes of the fused pass: every leg's D,I planes read once per group
             int64_t planes = 0;
             if (G.npre >= 1) planes += b->legs[G.preA].Kz + 1;
@@ -719,6 +811,9 @@ extern "C" int ps_batch_create_ex(ps_ctx *c, ps_polygon *polys, int npoly, doubl
     BCK(hipMemcpy(b->dcounts, counts.data(), sizeof(int) * counts.size(), hipMemcpyHostToDevice));
     BCK(hipEventCreate(&b->ev0));
     BCK(hipEventCreate(&b->ev1));
+    b->gev.assign(b->groups.size(), nullptr);
+    for (auto &e : b->gev) BCK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    BCK(hipEventCreateWithFlags(&b->zev, hipEventDisableTiming));
     // 5. capture the launch sequence once
     if (!getenv("PS_NO_GRAPH")) {
         BCK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
@@ -830,7 +925,7 @@ extern "C" int ps_multi_create(ps_batch **batches, int nbatch, ps_multi **out)
         st = PS_ERR_HIP;
     if (st == PS_OK && hipStreamBeginCapture(m->st, hipStreamCaptureModeThreadLocal) != hipSuccess) st = PS_ERR_HIP;
     if (st == PS_OK) {
-        for (int i = 0; i < nbatch && st == PS_OK; ++i) st = batch_record_on(batches[i], m->st);
+        for (int i = 0; i < nbatch && st == PS_OK; ++i) st = batch_record_on(batches[i], m->st, batches[i]->c->st2);
         hipGraph_t g = nullptr;
         const hipError_t e = hipStreamEndCapture(m->st, &g);
         if (e != hipSuccess && st == PS_OK) st = PS_ERR_HIP;
@@ -1036,7 +1131,7 @@ extern "C" int ps_batch_autotune(ps_ctx *c, ps_polygon *polys, int npoly, double
     for (int cap : caps)
         for (int ch : chunks)
             for (int dv : divs) {
-                ps_launch_params lp{cap, ch, dv};
+                ps_launch_params lp{cap, ch, dv, 0};
                 ps_batch *b = nullptr;
                 std::copy(polys, polys + npoly, work.begin());
                 int s = ps_batch_create_ex(c, work.data(), npoly, bw, kcase, &lp, &b);
@@ -1055,12 +1150,33 @@ extern "C" int ps_batch_autotune(ps_ctx *c, ps_polygon *polys, int npoly, double
                     bp = lp;
                 }
             }
+    {  // the overlapped schedule at the best shape
+        ps_launch_params lp = bp;
+        lp.overlap = 1;
+        ps_batch *b = nullptr;
+        std::copy(polys, polys + npoly, work.begin());
+        int s = ps_batch_create_ex(c, work.data(), npoly, bw, kcase, &lp, &b);
+        if (s != PS_OK) return s;
+        double t = 0.0;
+        s = ps_batch_execute(b, v.data(), nullptr, nullptr);
+        for (int r = 0; r < reps && s == PS_OK; ++r) {
+            s = ps_batch_execute(b, v.data(), nullptr, nullptr);
+            t += b->last_ms;
+        }
+        ps_batch_destroy(b);
+        if (s != PS_OK) return s;
+        t /= reps;
+        if (t < bms) {
+            bms = t;
+            bp = lp;
+        }
+    }
     const std::string key = tune_key(c, polys, npoly, bw, kcase);
     tune_load();
     g_tuned[key] = bp;
     if (const char *f = getenv("PS_TUNE_FILE")) {
         if (FILE *fp = fopen(f, "a")) {
-            fprintf(fp, "%s %d %d %d\n", key.c_str(), bp.grid_cap, bp.chunk_tiles, bp.g2_div);
+            fprintf(fp, "%s %d %d %d %d\n", key.c_str(), bp.grid_cap, bp.chunk_tiles, bp.g2_div, bp.overlap);
             fclose(fp);
         }
     }
@@ -1075,6 +1191,7 @@ extern "C" int ps_batch_params(const ps_batch *b, ps_launch_params *out)
     out->grid_cap = b->grid_cap;
     out->chunk_tiles = b->chunk_tiles;
     out->g2_div = b->g2_div;
+    out->overlap = b->overlap;
     return PS_OK;
 }
 
@@ -1091,15 +1208,15 @@ extern "C" int ps_batch_profile(ps_batch *b, int reps, double *stage_ms, double 
         (void)hipEventRecord(ev[0], c->st);
         if (b->dscale) st = run_leg_scale(c->fp64, c->N, b->dlegs, (int)b->legs.size(), c->st);
         if (st == PS_OK)
-            st = run_inverse_g(c->fp64, c->N, b->dlegs, (int)b->legs.size(), b->g1_tiles, 0, b->grid_cap, b->g2_div,
-                               c->tw, c->st);
+            st = run_inverse_g(c->fp64, c->N, b->dlegs, (int)b->legs.size(), 0, b->g1_tiles, 0, 0, b->grid_cap,
+                               b->g2_div, c->tw, c->st);
         (void)hipEventRecord(ev[1], c->st);
         if (st == PS_OK)
-            st = run_inverse_g(c->fp64, c->N, b->dlegs, (int)b->legs.size(), 0, b->g2_tiles, b->grid_cap, b->g2_div,
-                               c->tw, c->st);
+            st = run_inverse_g(c->fp64, c->N, b->dlegs, (int)b->legs.size(), 0, 0, 0, b->g2_tiles, b->grid_cap,
+                               b->g2_div, c->tw, c->st);
         (void)hipEventRecord(ev[2], c->st);
         if (st == PS_OK)
-            st = run_zc_fused(c->fp64, c->N, b->dgroups, (int)b->groups.size(), b->dlegs, b->dsuffix, b->items,
+            st = run_zc_fused(c->fp64, c->N, b->dgroups, (int)b->groups.size(), b->dlegs, b->dsuffix, 0, b->items,
                               b->dpartials, b->max_chunks, b->grid_cap, b->kz_cap, c->tw, c->st);
         (void)hipEventRecord(ev[3], c->st);
         if (st == PS_OK)

@@ -130,14 +130,14 @@ __device__ __forceinline__ int find_owner(const D *__restrict__ d, int n, int64_
 // G1: masked gather + C2C along ky.  Tile = (leg, kz, band tile of TLC kx
 // lines); LDS holds TLC D-lines (mask * spectrum) and TLC I-lines (mask).
 template <typename T, int N>
-__global__ void __launch_bounds__(NT) k_inv_g1(const LegDev *__restrict__ legs, int nlegs, int64_t ntiles,
+__global__ void __launch_bounds__(NT) k_inv_g1(const LegDev *__restrict__ legs, int nlegs, int64_t t0, int64_t t1,
                                                const cplx<T> *__restrict__ tw)
 {
     constexpr int TLC = tile_lines<T, N>() / 2;
     constexpr int TL2 = 2 * TLC;
     __shared__ cplx<T> lds[N * TL2];
     const int tid = threadIdx.x;
-    for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    for (int64_t tile = t0 + blockIdx.x; tile < t1; tile += gridDim.x) {
         const int li = find_owner(legs, nlegs, tile, &LegDev::g1_begin);
         const LegDev &L = legs[li];
         const int nbt = (L.Bx + TLC - 1) / TLC;
@@ -189,14 +189,14 @@ __global__ void __launch_bounds__(NT) k_inv_g1(const LegDev *__restrict__ legs, 
 // G2: C2C along kx from the band to every x.  Tile = (leg, part, kz, y tile
 // of TLC adjacent columns), part 0 = D, 1 = I; stores are TLC-value segments.
 template <typename T, int N, int WD>
-__global__ void __launch_bounds__(NT) k_inv_g2(const LegDev *__restrict__ legs, int nlegs, int64_t ntiles,
+__global__ void __launch_bounds__(NT) k_inv_g2(const LegDev *__restrict__ legs, int nlegs, int64_t t0, int64_t t1,
                                                const cplx<T> *__restrict__ tw)
 {
     constexpr int TLC = tile_lines<T, N>() / WD;  // WD: tile width divisor picked by the autotuner
     __shared__ cplx<T> lds[N * TLC];
     const int tid = threadIdx.x;
     constexpr int nyt = N / TLC;
-    for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    for (int64_t tile = t0 + blockIdx.x; tile < t1; tile += gridDim.x) {
         const int li = find_owner(legs, nlegs, tile, &LegDev::g2_begin);
         const LegDev &L = legs[li];
         const int64_t local = tile - L.g2_begin;
@@ -343,7 +343,7 @@ template <typename T, int N> struct ZcCfg {
 template <typename T, int N>
 __global__ void __launch_bounds__((ZcCfg<T, N>::BT)) k_inv_zc_fused(const GroupDev *__restrict__ groups, int ngroups,
                                                      const LegDev *__restrict__ legs,
-                                                     const int *__restrict__ suffix, int64_t nitems,
+                                                     const int *__restrict__ suffix, int64_t i0, int64_t i1,
                                                      double2 *__restrict__ partials, int max_chunks,
                                                      const cplx<T> *__restrict__ twg, int kz_cap, int nbuf)
 {
@@ -363,7 +363,7 @@ __global__ void __launch_bounds__((ZcCfg<T, N>::BT)) k_inv_zc_fused(const GroupD
     cplx<T> *wb = wbuf + w * LW * LP;
     constexpr int tiles_per_x = N / ROWS;
     const int64_t ntile = (int64_t)N * tiles_per_x;
-    for (int64_t item = blockIdx.x; item < nitems; item += gridDim.x) {
+    for (int64_t item = i0 + blockIdx.x; item < i1; item += gridDim.x) {
         const int gi = find_owner(groups, ngroups, item, &GroupDev::item_begin);
         const GroupDev &G = groups[gi];
         const int chunk = (int)(item - G.item_begin);
@@ -740,20 +740,20 @@ static int launch_forward_n(const void *field, int field_is_double, void *H, con
 }
 
 template <typename T, int N>
-static int launch_inverse_g_n(const LegDev *legs_dev, int nlegs, int64_t g1_tiles, int64_t g2_tiles, int grid_cap,
-                              int g2_div, const void *tw, hipStream_t st)
+static int launch_inverse_g_n(const LegDev *legs_dev, int nlegs, int64_t g1a, int64_t g1b, int64_t g2a, int64_t g2b,
+                              int grid_cap, int g2_div, const void *tw, hipStream_t st)
 {
     const cplx<T> *twp = reinterpret_cast<const cplx<T> *>(tw);
-    if (g1_tiles > 0) {
-        const unsigned g = (unsigned)((g1_tiles < grid_cap) ? g1_tiles : grid_cap);
-        hipLaunchKernelGGL((k_inv_g1<T, N>), dim3(g), dim3(NT), 0, st, legs_dev, nlegs, g1_tiles, twp);
+    if (g1b > g1a) {
+        const unsigned g = (unsigned)((g1b - g1a < grid_cap) ? g1b - g1a : grid_cap);
+        hipLaunchKernelGGL((k_inv_g1<T, N>), dim3(g), dim3(NT), 0, st, legs_dev, nlegs, g1a, g1b, twp);
     }
-    if (g2_tiles > 0) {
-        const unsigned g = (unsigned)((g2_tiles < grid_cap) ? g2_tiles : grid_cap);
+    if (g2b > g2a) {
+        const unsigned g = (unsigned)((g2b - g2a < grid_cap) ? g2b - g2a : grid_cap);
         if (g2_div == 2)
-            hipLaunchKernelGGL((k_inv_g2<T, N, 2>), dim3(g), dim3(NT), 0, st, legs_dev, nlegs, g2_tiles, twp);
+            hipLaunchKernelGGL((k_inv_g2<T, N, 2>), dim3(g), dim3(NT), 0, st, legs_dev, nlegs, g2a, g2b, twp);
         else
-            hipLaunchKernelGGL((k_inv_g2<T, N, 1>), dim3(g), dim3(NT), 0, st, legs_dev, nlegs, g2_tiles, twp);
+            hipLaunchKernelGGL((k_inv_g2<T, N, 1>), dim3(g), dim3(NT), 0, st, legs_dev, nlegs, g2a, g2b, twp);
     }
     return (int)hipGetLastError();
 }
@@ -771,7 +771,7 @@ static int launch_zc_plain_n(const void *gd, const void *gi, int Kz, double *acc
 
 template <typename T, int N>
 static int launch_zc_fused_n(const GroupDev *groups, int ngroups, const LegDev *legs, const int *suffix,
-                             int64_t nitems, double2 *partials, int max_chunks, int grid_cap, int kz_cap,
+                             int64_t i0, int64_t i1, double2 *partials, int max_chunks, int grid_cap, int kz_cap,
                              const void *tw, hipStream_t st)
 {
     using C = ZcCfg<T, N>;
@@ -793,9 +793,10 @@ static int launch_zc_fused_n(const GroupDev *groups, int ngroups, const LegDev *
     e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_inv_zc_fused<T, N>),
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
     if (e != hipSuccess) return (int)e;
+    if (i1 <= i0) return 0;
     const int64_t cap = (C::NWV == 8) ? (grid_cap + 1) / 2 : grid_cap;  // same thread count in flight
-    const unsigned g = (unsigned)((nitems < cap) ? nitems : cap);
-    hipLaunchKernelGGL((k_inv_zc_fused<T, N>), dim3(g), dim3(C::BT), dyn, st, groups, ngroups, legs, suffix, nitems,
+    const unsigned g = (unsigned)((i1 - i0 < cap) ? i1 - i0 : cap);
+    hipLaunchKernelGGL((k_inv_zc_fused<T, N>), dim3(g), dim3(C::BT), dyn, st, groups, ngroups, legs, suffix, i0, i1,
                        partials, max_chunks, reinterpret_cast<const cplx<T> *>(tw), kz_cap, nbuf);
     return (int)hipGetLastError();
 }
@@ -814,11 +815,11 @@ static int dispatch_forward(int N, const void *field, int fd, void *H, const voi
 }
 
 template <typename T>
-static int dispatch_inverse_g(int N, const LegDev *legs, int nlegs, int64_t g1, int64_t g2, int cap, int g2_div,
-                              const void *tw, hipStream_t st)
+static int dispatch_inverse_g(int N, const LegDev *legs, int nlegs, int64_t g1a, int64_t g1b, int64_t g2a, int64_t g2b,
+                              int cap, int g2_div, const void *tw, hipStream_t st)
 {
     switch (N) {
-#define CASE(n) case n: return launch_inverse_g_n<T, n>(legs, nlegs, g1, g2, cap, g2_div, tw, st);
+#define CASE(n) case n: return launch_inverse_g_n<T, n>(legs, nlegs, g1a, g1b, g2a, g2b, cap, g2_div, tw, st);
         PS_FOR_EACH_N(CASE)
 #undef CASE
     }
@@ -839,11 +840,11 @@ static int dispatch_zc_plain(int N, const void *gd, const void *gi, int Kz, doub
 
 template <typename T>
 static int dispatch_zc_fused(int N, const GroupDev *groups, int ngroups, const LegDev *legs, const int *suffix,
-                             int64_t nitems, double2 *partials, int max_chunks, int cap, int kz_cap, const void *tw,
-                             hipStream_t st)
+                             int64_t i0, int64_t i1, double2 *partials, int max_chunks, int cap, int kz_cap,
+                             const void *tw, hipStream_t st)
 {
     switch (N) {
-#define CASE(n) case n: return launch_zc_fused_n<T, n>(groups, ngroups, legs, suffix, nitems, partials, max_chunks, cap, kz_cap, tw, st);
+#define CASE(n) case n: return launch_zc_fused_n<T, n>(groups, ngroups, legs, suffix, i0, i1, partials, max_chunks, cap, kz_cap, tw, st);
         PS_FOR_EACH_N(CASE)
 #undef CASE
     }
@@ -881,11 +882,11 @@ int run_forward(int fp64, int N, const void *field, int field_is_double, void *H
                 : dispatch_forward<float>(N, field, field_is_double, H, tw, st);
 }
 
-int run_inverse_g(int fp64, int N, const LegDev *legs, int nlegs, int64_t g1_tiles, int64_t g2_tiles, int grid_cap,
-                  int g2_div, const void *tw, hipStream_t st)
+int run_inverse_g(int fp64, int N, const LegDev *legs, int nlegs, int64_t g1a, int64_t g1b, int64_t g2a, int64_t g2b,
+                  int grid_cap, int g2_div, const void *tw, hipStream_t st)
 {
-    return fp64 ? dispatch_inverse_g<double>(N, legs, nlegs, g1_tiles, g2_tiles, grid_cap, g2_div, tw, st)
-                : dispatch_inverse_g<float>(N, legs, nlegs, g1_tiles, g2_tiles, grid_cap, g2_div, tw, st);
+    return fp64 ? dispatch_inverse_g<double>(N, legs, nlegs, g1a, g1b, g2a, g2b, grid_cap, g2_div, tw, st)
+                : dispatch_inverse_g<float>(N, legs, nlegs, g1a, g1b, g2a, g2b, grid_cap, g2_div, tw, st);
 }
 
 int run_zc_plain(int fp64, int N, const void *gd, const void *gi, int Kz, double *accD, double *accI, int mode,
@@ -896,11 +897,11 @@ int run_zc_plain(int fp64, int N, const void *gd, const void *gi, int Kz, double
 }
 
 int run_zc_fused(int fp64, int N, const GroupDev *groups, int ngroups, const LegDev *legs, const int *suffix,
-                 int64_t nitems, double2 *partials, int max_chunks, int grid_cap, int kz_cap, const void *tw,
+                 int64_t i0, int64_t i1, double2 *partials, int max_chunks, int grid_cap, int kz_cap, const void *tw,
                  hipStream_t st)
 {
-    return fp64 ? dispatch_zc_fused<double>(N, groups, ngroups, legs, suffix, nitems, partials, max_chunks, grid_cap, kz_cap, tw, st)
-                : dispatch_zc_fused<float>(N, groups, ngroups, legs, suffix, nitems, partials, max_chunks, grid_cap, kz_cap, tw, st);
+    return fp64 ? dispatch_zc_fused<double>(N, groups, ngroups, legs, suffix, i0, i1, partials, max_chunks, grid_cap, kz_cap, tw, st)
+                : dispatch_zc_fused<float>(N, groups, ngroups, legs, suffix, i0, i1, partials, max_chunks, grid_cap, kz_cap, tw, st);
 }
 
 int run_reduce_partials(const double2 *partials, int64_t stride, const int *counts, int64_t uniform_count,

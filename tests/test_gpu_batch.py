@@ -116,8 +116,14 @@ def test_autotune_and_params(gpu_lib):
         ctx.load_field(f)
         b0 = ctx.batch(polys, 1.0, 0, params={"grid_cap": 64, "chunk_tiles": 1, "g2_div": 2})
         v0, _, _ = b0.execute()
-        assert b0.params == {"grid_cap": 64, "chunk_tiles": 1, "g2_div": 2}
+        assert b0.params == {"grid_cap": 64, "chunk_tiles": 1, "g2_div": 2, "overlap": 0}
         b0.close()
+        # the two-stream schedule sums in the same order
+        bo = ctx.batch(polys, 1.0, 0, params={"grid_cap": 64, "chunk_tiles": 1, "g2_div": 2, "overlap": 1})
+        vo, _, _ = bo.execute()
+        assert bo.params["overlap"] == 1
+        bo.close()
+        np.testing.assert_array_equal(vo, v0)
         best, ms = ctx.autotune(polys, 1.0, 0, reps=1)
         assert ms > 0 and best["grid_cap"] > 0
         b1 = ctx.batch(polys, 1.0, 0)
@@ -125,3 +131,26 @@ def test_autotune_and_params(gpu_lib):
         v1, _, _ = b1.execute()
         b1.close()
     np.testing.assert_allclose(v1, v0, rtol=1e-13)
+
+
+@pytest.mark.parametrize("prec_name", ["PREC_FP64", "PREC_MIXED"])
+def test_overlap_schedule_matches(gpu_lib, prec_name):
+    """Several polygon groups (two triangle prefixes, one power-spectrum group):
+    the two-stream schedule gives the single-stream values bit for bit."""
+    P = gpu_lib
+    N, L = 64, 100.0
+    f = lognormal_field(N, 41)
+    kf = 2 * math.pi / L
+    polys = _polys_triangles(kf, [(7.96, 7.96), (5.0, 9.0), (3.0, 3.0)], [0.4, 1.3, 2.2], 2)
+    polys += [{"n": 2, "k": [m * kf] * 2, "kmin": [m * kf] * 2, "kmax": [m * kf] * 2} for m in range(1, 12)]
+    out = {}
+    with P.Context(N, L, getattr(P, prec_name)) as ctx:
+        ctx.load_field(f)
+        for ov in (0, 1):
+            b = ctx.batch(polys, 1.0, 2, params={"grid_cap": 512, "chunk_tiles": 16, "g2_div": 1, "overlap": ov})
+            out[ov] = b.execute()
+            again = b.execute()
+            b.close()
+            np.testing.assert_array_equal(out[ov][0], again[0])
+    np.testing.assert_array_equal(out[1][0], out[0][0])
+    np.testing.assert_array_equal(out[1][1], out[0][1])
