@@ -50,6 +50,17 @@ extern "C" {
 #define PS_BINNING_GIVEN 2
 #define PS_BINNING_WATKINSON 3
 
+/* Evaluation of a batch (ps_launch_params::method).  Both give the
+ * reference's sum over the grid of the product of the filtered fields:
+ *   FOURIER: per group of polygons sharing their first legs, the product of
+ *            those legs is forward transformed once and each last leg is a
+ *            sum over its shell in Fourier space (Parseval); power spectra
+ *            need no transform.
+ *   DIRECT:  every leg inverse transformed, products and sums in real space. */
+#define PS_METHOD_FOURIER 0
+#define PS_METHOD_DIRECT 1
+#define PS_MAX_STAGES 6 /* passes ps_batch_profile reports at most */
+
 /* fftw_complex without <complex.h> (FFTW's own fallback typedef) */
 typedef double ps_fftw_complex[2];
 
@@ -123,16 +134,18 @@ int ps_batch_create(ps_ctx *ctx, ps_polygon *polys, int npoly, double kbinwidth,
                     ps_batch **out);
 
 /* Launch shape of a batch: persistent grid size of the pass kernels, Zc tiles
- * per work item (sets how many compensated partial sums each polygon has),
- * the divisor of the G2 tile width (1: 16 FP32 / 8 FP64 columns, 2: half) and
- * the schedule (overlap 0: each pass over all legs in turn on one stream;
- * 1: per polygon group, the product pass of group g on a second stream beside
- * the inverse transforms of the legs group g+1 introduces). */
+ * (DIRECT) or dot-pass steps (FOURIER) per work item (sets how many
+ * compensated partial sums each polygon has), the divisor of the G2/Fx tile
+ * width (1: 16 FP32 / 8 FP64 columns, 2: half), the DIRECT schedule (overlap
+ * 0: each pass over all legs in turn on one stream; 1: per polygon group, the
+ * product pass of group g on a second stream beside the inverse transforms of
+ * the legs group g+1 introduces) and the method (PS_METHOD_*). */
 typedef struct ps_launch_params {
     int grid_cap;
     int chunk_tiles;
     int g2_div;
     int overlap;
+    int method;
 } ps_launch_params;
 
 /* ps_batch_create with explicit launch parameters (NULL: the tuned ones for
@@ -169,11 +182,14 @@ double ps_batch_last_ms(const ps_batch *b);
 double ps_batch_model_bytes(const ps_batch *b);
 int ps_batch_destroy(ps_batch *b);
 
-/* Run the batch stage by stage (not from the graph) reps times with HIP
- * events between stages; stage_ms[4] receives the mean time of the G1, G2,
- * Zc (fused product-reduce) and final-reduce launches, stage_bytes[4]
- * (optional) the algorithmic HBM bytes of each. */
+/* Run the batch pass by pass (not from the graph) reps times with HIP events
+ * around every launch; stage_ms[PS_MAX_STAGES] receives the mean time of each
+ * pass (ps_batch_stage_count of them: DIRECT G1, G2, Zc, reduce; FOURIER G1,
+ * G2, Zf, Fx, dot, reduce), stage_bytes[PS_MAX_STAGES] (optional) the
+ * algorithmic HBM bytes of each. */
 int ps_batch_profile(ps_batch *b, int reps, double *stage_ms, double *stage_bytes);
+int ps_batch_stage_count(const ps_batch *b);
+const char *ps_batch_stage_name(const ps_batch *b, int stage);
 
 /* Mean time (ms) of reps forward transforms of a slot's last upload and the
  * algorithmic bytes of one (3 passes, each reading and writing its array). */

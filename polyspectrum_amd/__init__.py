@@ -40,14 +40,21 @@ class Polygon(ctypes.Structure):
 
 
 class LaunchParams(ctypes.Structure):
-    """ps_launch_params: persistent grid size, Zc tiles per work item, G2 tile
-    divisor, two-stream overlapped schedule (0/1)."""
+    """ps_launch_params: persistent grid size, work-item size, G2/Fx tile
+    divisor, two-stream overlapped schedule (0/1, direct method) and the
+    method (METHOD_FOURIER / METHOD_DIRECT)."""
     _fields_ = [("grid_cap", ctypes.c_int), ("chunk_tiles", ctypes.c_int), ("g2_div", ctypes.c_int),
-                ("overlap", ctypes.c_int)]
+                ("overlap", ctypes.c_int), ("method", ctypes.c_int)]
+
+
+METHOD_FOURIER = 0  # prefix product forward transformed once, shell sums per last leg
+METHOD_DIRECT = 1   # every leg inverse transformed, real-space products
+MAX_STAGES = 6
 
 
 def _params_dict(lp):
-    return {"grid_cap": lp.grid_cap, "chunk_tiles": lp.chunk_tiles, "g2_div": lp.g2_div, "overlap": lp.overlap}
+    return {"grid_cap": lp.grid_cap, "chunk_tiles": lp.chunk_tiles, "g2_div": lp.g2_div, "overlap": lp.overlap,
+            "method": lp.method}
 
 
 def _polygon_array(polygons):
@@ -118,6 +125,9 @@ def load_library():
     L.ps_forward_again.argtypes = [_vp, ctypes.c_int]
     L.ps_get_spectrum.argtypes = [_vp, ctypes.c_int, _vp]
     L.ps_batch_profile.argtypes = [_vp, ctypes.c_int, _dp, _dp]
+    L.ps_batch_stage_count.argtypes = [_vp]
+    L.ps_batch_stage_name.restype = ctypes.c_char_p
+    L.ps_batch_stage_name.argtypes = [_vp, ctypes.c_int]
     L.ps_batch_adjoint.argtypes = [_vp, ctypes.c_int, _dp, _vp, ctypes.c_int, ctypes.c_int]
     L.ps_forward_profile.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _dp, _dp]
     L.ps_batch_results.argtypes = [_vp, _dp, _dp, _dp]
@@ -284,7 +294,7 @@ class Batch:
         lp = None
         if params is not None:
             lp = LaunchParams(int(params["grid_cap"]), int(params["chunk_tiles"]), int(params.get("g2_div", 1)),
-                              int(params.get("overlap", 0)))
+                              int(params.get("overlap", 0)), int(params.get("method", METHOD_FOURIER)))
         _check(load_library().ps_batch_create_ex(ctx._h, arr, len(polygons), float(kbinwidth), int(kbinning_case),
                                                  ctypes.byref(lp) if lp is not None else None, ctypes.byref(h)))
         self._h = h
@@ -334,11 +344,15 @@ class Batch:
         return v, npol, s.reshape(self.n, 2)
 
     def profile(self, reps=3):
-        """Per-stage (G1, G2, Zc, reduce) mean ms and algorithmic bytes."""
-        ms = np.zeros(4)
-        by = np.zeros(4)
-        _check(load_library().ps_batch_profile(self._h, reps, ms.ctypes.data_as(_dp), by.ctypes.data_as(_dp)))
-        return ms, by
+        """Per-pass mean ms and algorithmic bytes, and the pass names
+        (direct: G1, G2, Zc, reduce; Fourier: G1, G2, Zf, Fx, dot, reduce)."""
+        L = load_library()
+        ms = np.zeros(MAX_STAGES)
+        by = np.zeros(MAX_STAGES)
+        _check(L.ps_batch_profile(self._h, reps, ms.ctypes.data_as(_dp), by.ctypes.data_as(_dp)))
+        n = L.ps_batch_stage_count(self._h)
+        names = [L.ps_batch_stage_name(self._h, i).decode() for i in range(n)]
+        return ms[:n], by[:n], names
 
     @property
     def last_ms(self):

@@ -438,9 +438,25 @@ struct GroupRange {
     int64_t g1a = 0, g1b = 0, g2a = 0, g2b = 0, i0 = 0, i1 = 0;
 };
 
+// Fourier-space method: the groups are evaluated in waves; the prefix legs a
+// wave transforms and the planes of its groups live in one buffer arena that
+// every wave reuses in turn.
+struct SpecWave {
+    std::vector<LegDev> legs;            // prefix legs transformed in this wave
+    std::vector<SpecGroupDev> groups;
+    std::vector<size_t> leg_off;         // arena offsets of the legs' G1/G2 buffers
+    std::vector<size_t> grp_off;         // arena offsets of the groups' Zf/Fx buffers
+    LegDev *dlegs = nullptr;
+    SpecGroupDev *dgroups = nullptr;
+    int64_t g1_tiles = 0, g2_tiles = 0, zf_tiles = 0, fx_tiles = 0, dot_items = 0;
+    int kz_cap = 1;                      // 1 + largest Kz of the wave's legs
+    size_t bytes = 0;
+};
+
 struct ps_batch {
     ps_ctx *c = nullptr;
     int npoly = 0;
+    int method = PS_METHOD_FOURIER;
     std::vector<int> poly_n;
     std::vector<int> poly_out;           // output slot of each polygon
     int nout = 0;
@@ -451,8 +467,8 @@ struct ps_batch {
     int max_chunks = 1;
     int grid_cap = 2048;
     int kz_cap = 1;                      // 1 + largest Kz of the legs
-    int g2_div = 1;                      // G2 tile width divisor (1 or 2)
-    int chunk_tiles = 64;                // Zc tiles per work item
+    int g2_div = 1;                      // G2 (and Fx) tile width divisor (1 or 2)
+    int chunk_tiles = 64;                // Zc tiles (direct) or dot steps (Fourier) per work item
     std::vector<void *> allocs;
     int overlap = 0;                     // schedule, as ps_launch_params::overlap
     std::vector<GroupRange> ranges;      // per group: tiles of the legs it introduces, its work items
@@ -465,6 +481,10 @@ struct ps_batch {
     double *dscale = nullptr;            // per-leg amplitude scales (mixed precision)
     double2 *dpartials = nullptr;
     double *dsum = nullptr;              // 2*nout
+    std::vector<SpecWave> waves;         // Fourier method
+    MaskParams *dmasks = nullptr;        // suffix masks in suffix order (Fourier method)
+    int2 *divl = nullptr;                // their m^2 intervals and counts
+    void *arena = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -473,8 +493,18 @@ struct ps_batch {
     std::vector<int> leg_slot;           // field slot of every leg
     std::vector<double> last_sumI;       // sum prod I of every output (last execute)
     int have_sums = 0;
-    double stage_bytes[4] = {0, 0, 0, 0};  // G1, G2, Zc, reduce
+    int nstages = 4;
+    double stage_bytes[PS_MAX_STAGES] = {};
 };
+
+static const char *const kStageDirect[] = {"G1 masked gather + ky inverse", "G2 kx inverse (band -> all x)",
+                                           "Zc fused C2R + product-reduce", "final reduce"};
+static const char *const kStageFourier[] = {"G1 masked gather + ky inverse (prefix legs)",
+                                            "G2 kx inverse (band -> all x)",
+                                            "Zf C2R + product + forward z",
+                                            "Fx forward x (all x -> band)",
+                                            "dot forward y + shell sums",
+                                            "final reduce"};
 
 extern "C" int ps_batch_destroy(ps_batch *b)
 {
@@ -487,7 +517,12 @@ extern "C" int ps_batch_destroy(ps_batch *b)
     for (hipEvent_t e : b->gev) (void)hipEventDestroy(e);
     if (b->zev) (void)hipEventDestroy(b->zev);
     for (void *p : b->allocs) (void)hipFree(p);
-    void *bufs[] = {b->dlegs, b->dgroups, b->dsuffix, b->dcounts, b->dpartials, b->dsum, b->dscale};
+    for (SpecWave &w : b->waves) {
+        if (w.dlegs) (void)hipFree(w.dlegs);
+        if (w.dgroups) (void)hipFree(w.dgroups);
+    }
+    void *bufs[] = {b->dlegs, b->dgroups, b->dsuffix, b->dcounts, b->dpartials, b->dsum,
+                    b->dscale, b->dmasks, b->divl, b->arena};
     for (void *p : bufs) if (p) (void)hipFree(p);
     delete b;
     return PS_OK;
@@ -498,30 +533,50 @@ static bool same_mask(const MaskParams &a, const MaskParams &b)
     return a.kind == b.kind && a.kkf == b.kkf && a.mmin == b.mmin && a.mmax == b.mmax && a.bw == b.bw;
 }
 
-// The launch sequence of one batch (captured into graphs).  Without overlap
-// each pass runs over every leg / group in turn on stream s.  With overlap the
-// HBM-bound G1/G2 passes of group g+1 run on stream s while the compute-bound
-// fused pass of group g runs on stream s2; events order each group's fused
-// pass after the transforms of the legs it reads.
+// The launch sequence of one batch (captured into graphs).
+// Fourier method: per wave, the inverse transforms of the prefix legs, then
+// the Zf, Fx and dot passes of the wave's groups.
+// Direct method without overlap: each pass over every leg / group in turn on
+// stream s.  With overlap the HBM-bound G1/G2 passes of group g+1 run on
+// stream s while the compute-bound fused pass of group g runs on stream s2;
+// events order each group's fused pass after the transforms of the legs it
+// reads.
 static int batch_record_on(ps_batch *b, hipStream_t s, hipStream_t s2)
 {
     ps_ctx *c = b->c;
+    const int N = c->N;
+    if (b->method == PS_METHOD_FOURIER) {
+        for (SpecWave &w : b->waves) {
+            const int nl = (int)w.legs.size(), ng = (int)w.groups.size();
+            if (nl > 0) {
+                if (b->dscale) PSCK(run_leg_scale(c->fp64, N, w.dlegs, nl, s));
+                PSCK(run_inverse_g(c->fp64, N, w.dlegs, nl, 0, w.g1_tiles, 0, w.g2_tiles, b->grid_cap, b->g2_div,
+                                   c->tw, s));
+            }
+            PSCK(run_spec_zf(c->fp64, N, w.dgroups, ng, w.dlegs, 0, w.zf_tiles, b->grid_cap, w.kz_cap, c->tw, s));
+            PSCK(run_spec_fx(c->fp64, N, w.dgroups, ng, 0, w.fx_tiles, b->grid_cap, b->g2_div, c->tw, s));
+            PSCK(run_spec_dot(c->fp64, N, w.dgroups, ng, b->dmasks, b->divl, 0, w.dot_items, b->dpartials,
+                              b->max_chunks, b->grid_cap, c->tw, s));
+        }
+        PSCK(run_reduce_partials(b->dpartials, b->max_chunks, b->dcounts, 0, b->nout, b->dsum, b->dsum + b->nout, s));
+        return PS_OK;
+    }
     const int nl = (int)b->legs.size(), ng = (int)b->groups.size();
-    if (b->dscale) PSCK(run_leg_scale(c->fp64, c->N, b->dlegs, nl, s));
+    if (b->dscale) PSCK(run_leg_scale(c->fp64, N, b->dlegs, nl, s));
     if (!b->overlap) {
-        PSCK(run_inverse_g(c->fp64, c->N, b->dlegs, nl, 0, b->g1_tiles, 0, b->g2_tiles, b->grid_cap, b->g2_div,
+        PSCK(run_inverse_g(c->fp64, N, b->dlegs, nl, 0, b->g1_tiles, 0, b->g2_tiles, b->grid_cap, b->g2_div,
                            c->tw, s));
-        PSCK(run_zc_fused(c->fp64, c->N, b->dgroups, ng, b->dlegs, b->dsuffix, 0, b->items, b->dpartials,
+        PSCK(run_zc_fused(c->fp64, N, b->dgroups, ng, b->dlegs, b->dsuffix, 0, b->items, b->dpartials,
                           b->max_chunks, b->grid_cap, b->kz_cap, c->tw, s));
         PSCK(run_reduce_partials(b->dpartials, b->max_chunks, b->dcounts, 0, b->nout, b->dsum, b->dsum + b->nout, s));
         return PS_OK;
     }
     for (int g = 0; g < ng; ++g) {
         const GroupRange &R = b->ranges[g];
-        PSCK(run_inverse_g(c->fp64, c->N, b->dlegs, nl, R.g1a, R.g1b, R.g2a, R.g2b, b->grid_cap, b->g2_div, c->tw, s));
+        PSCK(run_inverse_g(c->fp64, N, b->dlegs, nl, R.g1a, R.g1b, R.g2a, R.g2b, b->grid_cap, b->g2_div, c->tw, s));
         HIPCK(hipEventRecord(b->gev[g], s));
         HIPCK(hipStreamWaitEvent(s2, b->gev[g], 0));
-        PSCK(run_zc_fused(c->fp64, c->N, b->dgroups, ng, b->dlegs, b->dsuffix, R.i0, R.i1, b->dpartials,
+        PSCK(run_zc_fused(c->fp64, N, b->dgroups, ng, b->dlegs, b->dsuffix, R.i0, R.i1, b->dpartials,
                           b->max_chunks, b->grid_cap, b->kz_cap, c->tw, s2));
     }
     HIPCK(hipEventRecord(b->zev, s2));
@@ -561,7 +616,8 @@ static void tune_load()
     if (!fp) return;
     char key[256];
     ps_launch_params lp;
-    while (fscanf(fp, "%255s %d %d %d %d", key, &lp.grid_cap, &lp.chunk_tiles, &lp.g2_div, &lp.overlap) == 5)
+    while (fscanf(fp, "%255s %d %d %d %d %d", key, &lp.grid_cap, &lp.chunk_tiles, &lp.g2_div, &lp.overlap,
+                  &lp.method) == 6)
         g_tuned[key] = lp;
     fclose(fp);
 }
@@ -573,6 +629,7 @@ static ps_launch_params default_params(const ps_ctx *c)
     lp.chunk_tiles = 64;
     lp.g2_div = 1;
     lp.overlap = 0;
+    lp.method = PS_METHOD_FOURIER;
     if (const char *e = getenv("PS_CHUNK_TILES")) lp.chunk_tiles = std::max(1, atoi(e));
     if (const char *e = getenv("PS_GRID_CAP")) lp.grid_cap = std::max(1, atoi(e));
     return lp;
@@ -582,6 +639,283 @@ extern "C" int ps_batch_create(ps_ctx *c, ps_polygon *polys, int npoly, double b
 {
     return ps_batch_create_ex(c, polys, npoly, bw, kcase, nullptr, out);
 }
+
+#define PCK(call)                                                                            \
+    do {                                                                                     \
+        hipError_t e_ = (call);                                                              \
+        if (e_ != hipSuccess) return (e_ == hipErrorOutOfMemory) ? PS_ERR_NOMEM : PS_ERR_HIP; \
+    } while (0)
+
+// Direct method: every leg inverse transformed (G1, G2), the products and
+// sums in real space in the fused Zc pass.
+static int plan_direct(ps_batch *b, std::vector<int> &counts)
+{
+    ps_ctx *c = b->c;
+    const int N = c->N;
+    const size_t cs = c->csize();
+    const int tl = g_tile_lines(N, c->fp64);
+    for (LegDev &L : b->legs) {
+        const int K = L.Kz;
+        b->kz_cap = std::max(b->kz_cap, K + 1);
+        L.g1_begin = b->g1_tiles;
+        L.g2_begin = b->g2_tiles;
+        b->g1_tiles += (int64_t)(K + 1) * ((L.Bx + tl - 1) / tl);
+        b->g2_tiles += 2 * (int64_t)(K + 1) * (N / (tile_rows(N, c->fp64) / b->g2_div));
+        const size_t n1 = (size_t)(K + 1) * L.Bx * N, n2 = (size_t)(K + 1) * N * N;
+        void *p1 = nullptr, *p2 = nullptr;
+        PCK(hipMalloc(&p1, cs * n1 * 2));
+        b->allocs.push_back(p1);
+        PCK(hipMalloc(&p2, cs * n2 * 2));
+        b->allocs.push_back(p2);
+        L.g1d = p1;
+        L.g1i = (char *)p1 + cs * n1;
+        L.g2d = p2;
+        L.g2i = (char *)p2 + cs * n2;
+        // model bytes: G1 reads the spectrum cells of the band box and writes
+        // the D,I band lines; G2 reads them and writes the D,I planes
+        b->stage_bytes[0] += (double)cs * ((double)(K + 1) * L.Bx * std::min(2 * K + 1, N) + 2.0 * n1);
+        b->stage_bytes[1] += (double)cs * (2.0 * n1 + 2.0 * n2);
+    }
+    const int rows = fused_tile_rows(N, c->fp64);
+    const int64_t tiles_per_group = (int64_t)N * (N / rows);
+    const int nchunks = (int)((tiles_per_group + b->chunk_tiles - 1) / b->chunk_tiles);
+    b->max_chunks = nchunks;
+    int legs_seen = 0;
+    for (GroupDev &G : b->groups) {
+        G.item_begin = b->items;
+        b->items += nchunks;
+        for (int o = 0; o < G.nsuf; ++o) counts[G.out_begin + o] = nchunks;
+        // the legs this group introduces: [legs_seen, hi)
+        int hi = legs_seen;
+        if (G.npre >= 1) hi = std::max(hi, G.preA + 1);
+        if (G.npre == 2) hi = std::max(hi, G.preB + 1);
+        for (int o = 0; o < G.nsuf; ++o) hi = std::max(hi, b->suffix[G.suf_begin + o] + 1);
+        GroupRange R;
+        const int nl = (int)b->legs.size();
+        R.g1a = legs_seen < nl ? b->legs[legs_seen].g1_begin : b->g1_tiles;
+        R.g1b = hi < nl ? b->legs[hi].g1_begin : b->g1_tiles;
+        R.g2a = legs_seen < nl ? b->legs[legs_seen].g2_begin : b->g2_tiles;
+        R.g2b = hi < nl ? b->legs[hi].g2_begin : b->g2_tiles;
+        if (hi <= legs_seen) R.g1b = R.g1a, R.g2b = R.g2a;
+        R.i0 = G.item_begin;
+        R.i1 = G.item_begin + nchunks;
+        b->ranges.push_back(R);
+        legs_seen = std::max(legs_seen, hi);
+        // model bytes of the fused pass: every leg's D,I planes read once per group
+        int64_t planes = 0;
+        if (G.npre >= 1) planes += b->legs[G.preA].Kz + 1;
+        if (G.npre == 2) planes += b->legs[G.preB].Kz + 1;
+        for (int o = 0; o < G.nsuf; ++o) planes += b->legs[b->suffix[G.suf_begin + o]].Kz + 1;
+        b->stage_bytes[2] += 2.0 * cs * (double)planes * N * N;
+    }
+    b->nstages = 4;
+    b->stage_bytes[3] = 32.0 * (double)b->nout * b->max_chunks;
+    b->stage_bytes[2] += b->stage_bytes[3];  // the fused pass writes the partial sums
+    if (!c->fp64) {
+        PCK(hipMalloc(&b->dscale, sizeof(double) * b->legs.size()));
+        for (size_t i = 0; i < b->legs.size(); ++i) b->legs[i].scale = b->dscale + i;
+    }
+    PCK(hipMalloc(&b->dlegs, sizeof(LegDev) * b->legs.size()));
+    PCK(hipMalloc(&b->dgroups, sizeof(GroupDev) * b->groups.size()));
+    PCK(hipMalloc(&b->dsuffix, sizeof(int) * b->suffix.size()));
+    PCK(hipMemcpy(b->dlegs, b->legs.data(), sizeof(LegDev) * b->legs.size(), hipMemcpyHostToDevice));
+    PCK(hipMemcpy(b->dgroups, b->groups.data(), sizeof(GroupDev) * b->groups.size(), hipMemcpyHostToDevice));
+    PCK(hipMemcpy(b->dsuffix, b->suffix.data(), sizeof(int) * b->suffix.size(), hipMemcpyHostToDevice));
+    return PS_OK;
+}
+
+// Fourier method: only the prefix legs of triangle groups are inverse
+// transformed; each group's prefix product is forward transformed once and
+// every suffix leg is a sum over its shell (power spectra: no transform).
+static int plan_fourier(ps_batch *b, std::vector<int> &counts)
+{
+    ps_ctx *c = b->c;
+    const int N = c->N, fp64 = c->fp64;
+    const size_t cs = c->csize();
+    const int tl = g_tile_lines(N, fp64);
+    const int zrows = spec_zf_rows(N, fp64);
+    const int fxcols = tile_rows(N, fp64) / b->g2_div;
+    const int dot_lines = b->chunk_tiles * spec_dot_lines(N, fp64);
+    // suffix masks and their m^2 intervals
+    const size_t ns = b->suffix.size();
+    std::vector<MaskParams> sm(ns);
+    std::vector<int> skmax(ns);
+    for (size_t s = 0; s < ns; ++s) {
+        sm[s] = b->legs[b->suffix[s]].mask;
+        skmax[s] = b->legs[b->suffix[s]].Kz;
+    }
+    PCK(hipMalloc(&b->dmasks, sizeof(MaskParams) * ns));
+    PCK(hipMalloc(&b->divl, sizeof(int2) * 2 * ns));
+    {
+        int *dk = nullptr;
+        PCK(hipMalloc(&dk, sizeof(int) * ns));
+        b->allocs.push_back(dk);
+        PCK(hipMemcpy(b->dmasks, sm.data(), sizeof(MaskParams) * ns, hipMemcpyHostToDevice));
+        PCK(hipMemcpy(dk, skmax.data(), sizeof(int) * ns, hipMemcpyHostToDevice));
+        PSCK(run_shell_intervals(b->dmasks, dk, (int)ns, b->divl, c->st));
+        std::vector<int2> hv(2 * ns);
+        PCK(hipMemcpyAsync(hv.data(), b->divl, sizeof(int2) * 2 * ns, hipMemcpyDeviceToHost, c->st));
+        PCK(hipStreamSynchronize(c->st));
+        bool patched = false;
+        for (size_t s = 0; s < ns; ++s) {
+            // a radial shell is one m^2 interval; anything else is tested per cell
+            const int2 iv = hv[2 * s];
+            const int cnt = hv[2 * s + 1].x;
+            const bool ok = mask_is_radial(sm[s].kind) && (iv.y < iv.x ? cnt == 0 : cnt == iv.y - iv.x + 1);
+            if (!ok && mask_is_radial(sm[s].kind)) {
+                hv[2 * s] = make_int2(-1, -1);
+                patched = true;
+            }
+        }
+        if (patched) PCK(hipMemcpy(b->divl, hv.data(), sizeof(int2) * 2 * ns, hipMemcpyHostToDevice));
+    }
+    // waves of groups within the arena budget
+    size_t budget = (size_t)16 << 30;
+    size_t mfree = 0, mtot = 0;
+    if (hipMemGetInfo(&mfree, &mtot) == hipSuccess && mfree / 2 < budget) budget = mfree / 2;
+    if (const char *e = getenv("PS_ARENA_MB")) budget = (size_t)std::max(1, atoi(e)) << 20;
+    auto leg_bytes = [&](const LegDev &L) -> size_t {
+        return 2 * cs * ((size_t)(L.Kz + 1) * L.Bx * N + (size_t)(L.Kz + 1) * N * N);
+    };
+    std::vector<SpecWave> &W = b->waves;
+    std::map<int, int> wave_leg;  // leg id -> index in the current wave
+    W.emplace_back();
+    for (const GroupDev &G : b->groups) {
+        SpecGroupDev S{};
+        S.fft = (G.npre == 2 || G.pre_sq) ? 1 : 0;
+        int Kt = 0;
+        for (int o = 0; o < G.nsuf; ++o) Kt = std::max(Kt, b->legs[b->suffix[G.suf_begin + o]].Kz);
+        int Kq = Kt;
+        if (!S.fft && G.npre == 1) Kq = std::min(Kq, b->legs[G.preA].Kz);
+        Kq = std::min(Kq, N / 2);
+        S.Kq = Kq;
+        S.Kqz = std::min(Kq, N / 2);
+        S.Bq = std::min(2 * Kq + 1, N);
+        S.npre = S.fft ? G.npre : 0;
+        S.pre_sq = G.pre_sq;
+        S.nsuf = G.nsuf;
+        S.suf_begin = G.suf_begin;
+        S.out_begin = G.out_begin;
+        S.has_A = (!S.fft && G.npre == 1) ? 1 : 0;
+        if (S.has_A) S.mA = b->legs[G.preA].mask;
+        const LegDev &L0 = b->legs[b->suffix[G.suf_begin]];
+        S.spec = L0.spec;
+        S.phase_only = L0.phase_only;
+        S.dot_lines = dot_lines;
+        const size_t gbytes = S.fft ? 2 * cs * ((size_t)(S.Kqz + 1) * N * N + (size_t)(S.Kqz + 1) * S.Bq * N) : 0;
+        std::vector<int> pre;
+        if (S.fft) {
+            pre.push_back(G.preA);
+            if (G.npre == 2) pre.push_back(G.preB);
+        }
+        auto need_of = [&]() {
+            size_t need = gbytes;
+            for (int p : pre)
+                if (!wave_leg.count(p)) need += leg_bytes(b->legs[p]);
+            return need;
+        };
+        if (!W.back().groups.empty() && W.back().bytes + need_of() > budget) {
+            W.emplace_back();
+            wave_leg.clear();
+        }
+        SpecWave &w = W.back();
+        for (int p : pre) {
+            if (wave_leg.count(p)) continue;
+            wave_leg[p] = (int)w.legs.size();
+            w.leg_off.push_back(w.bytes);
+            w.bytes += leg_bytes(b->legs[p]);
+            w.legs.push_back(b->legs[p]);
+        }
+        if (S.fft) {
+            S.preA = wave_leg[G.preA];
+            S.preB = (G.npre == 2) ? wave_leg[G.preB] : S.preA;
+        }
+        w.grp_off.push_back(S.fft ? w.bytes : 0);
+        w.bytes += gbytes;
+        w.groups.push_back(S);
+    }
+    // tiles, work items and model bytes
+    size_t arena = 0, nscale = 0;
+    for (SpecWave &w : W) {
+        for (LegDev &L : w.legs) {
+            const int K = L.Kz;
+            w.kz_cap = std::max(w.kz_cap, K + 1);
+            L.g1_begin = w.g1_tiles;
+            L.g2_begin = w.g2_tiles;
+            w.g1_tiles += (int64_t)(K + 1) * ((L.Bx + tl - 1) / tl);
+            w.g2_tiles += 2 * (int64_t)(K + 1) * (N / (tile_rows(N, fp64) / b->g2_div));
+            const double n1 = (double)(K + 1) * L.Bx * N, n2 = (double)(K + 1) * N * N;
+            b->stage_bytes[0] += (double)cs * ((double)(K + 1) * L.Bx * std::min(2 * K + 1, N) + 2.0 * n1);
+            b->stage_bytes[1] += (double)cs * (2.0 * n1 + 2.0 * n2);
+        }
+        for (SpecGroupDev &S : w.groups) {
+            S.zf_begin = w.zf_tiles;
+            S.fx_begin = w.fx_tiles;
+            S.dot_begin = w.dot_items;
+            const double planes = (double)(S.Kqz + 1) * N * N, band = (double)(S.Kqz + 1) * S.Bq * N;
+            if (S.fft) {
+                w.zf_tiles += (int64_t)N * (N / zrows);
+                w.fx_tiles += 2 * (int64_t)(S.Kqz + 1) * (N / fxcols);
+                double pre = (double)(w.legs[S.preA].Kz + 1);
+                if (S.npre == 2) pre += (double)(w.legs[S.preB].Kz + 1);
+                b->stage_bytes[2] += 2.0 * cs * (pre * N * N + planes);
+                b->stage_bytes[3] += 2.0 * cs * (planes + band);
+                b->stage_bytes[4] += 2.0 * cs * band;
+            }
+            b->stage_bytes[4] += (double)cs * (S.Kqz + 1) * (double)S.Bq * S.Bq;  // spectrum cells of the band
+            const int64_t nlines = 2 * (int64_t)(S.Kqz + 1) * S.Bq;
+            const int items = (int)((nlines + dot_lines - 1) / dot_lines);
+            w.dot_items += items;
+            for (int o = 0; o < S.nsuf; ++o) counts[S.out_begin + o] = items;
+            b->max_chunks = std::max(b->max_chunks, items);
+        }
+        arena = std::max(arena, w.bytes);
+        nscale += w.legs.size();
+    }
+    b->nstages = 6;
+    b->stage_bytes[5] = 32.0 * (double)b->nout * b->max_chunks;
+    b->stage_bytes[4] += b->stage_bytes[5];  // the dot pass writes the partial sums
+    // buffers and device tables
+    if (arena > 0) PCK(hipMalloc(&b->arena, arena));
+    // amplitude scales in both precisions: the packed forward transform of
+    // P_D + i P_I needs parts of like size (P_D = D_A D_B is ~|H|^2 larger)
+    if (nscale > 0) PCK(hipMalloc(&b->dscale, sizeof(double) * nscale));
+    size_t si = 0;
+    char *base = (char *)b->arena;
+    for (SpecWave &w : W) {
+        for (size_t i = 0; i < w.legs.size(); ++i) {
+            LegDev &L = w.legs[i];
+            const size_t n1 = (size_t)(L.Kz + 1) * L.Bx * N, n2 = (size_t)(L.Kz + 1) * N * N;
+            char *p = base + w.leg_off[i];
+            L.g1d = p;
+            L.g1i = p + cs * n1;
+            L.g2d = p + 2 * cs * n1;
+            L.g2i = p + 2 * cs * n1 + cs * n2;
+            L.scale = b->dscale ? b->dscale + si++ : nullptr;
+        }
+        for (size_t g = 0; g < w.groups.size(); ++g) {
+            SpecGroupDev &S = w.groups[g];
+            if (S.fft) {
+                const size_t np = (size_t)(S.Kqz + 1) * N * N, nb = (size_t)(S.Kqz + 1) * S.Bq * N;
+                char *p = base + w.grp_off[g];
+                S.qd = p;
+                S.qi = p + cs * np;
+                S.bd = p + 2 * cs * np;
+                S.bi = p + 2 * cs * np + cs * nb;
+                S.scaleA = w.legs[S.preA].scale;
+                S.scaleB = (S.npre == 2) ? w.legs[S.preB].scale : nullptr;
+            }
+        }
+        if (!w.legs.empty()) {
+            PCK(hipMalloc(&w.dlegs, sizeof(LegDev) * w.legs.size()));
+            PCK(hipMemcpy(w.dlegs, w.legs.data(), sizeof(LegDev) * w.legs.size(), hipMemcpyHostToDevice));
+        }
+        PCK(hipMalloc(&w.dgroups, sizeof(SpecGroupDev) * w.groups.size()));
+        PCK(hipMemcpy(w.dgroups, w.groups.data(), sizeof(SpecGroupDev) * w.groups.size(), hipMemcpyHostToDevice));
+    }
+    return PS_OK;
+}
+#undef PCK
 
 extern "C" int ps_batch_create_ex(ps_ctx *c, ps_polygon *polys, int npoly, double bw, int kcase,
                                   const ps_launch_params *params, ps_batch **out)
@@ -598,7 +932,7 @@ extern "C" int ps_batch_create_ex(ps_ctx *c, ps_polygon *polys, int npoly, doubl
         if (it != g_tuned.end()) lp = it->second;
     }
     if (lp.grid_cap < 1 || lp.chunk_tiles < 1 || (lp.g2_div != 1 && lp.g2_div != 2) ||
-        (lp.overlap != 0 && lp.overlap != 1))
+        (lp.overlap != 0 && lp.overlap != 1) || (lp.method != PS_METHOD_FOURIER && lp.method != PS_METHOD_DIRECT))
         return PS_ERR_INVALID;
     for (int p = 0; p < npoly; ++p) {
         if (polys[p].n < 2 || polys[p].n > 3) return PS_ERR_INVALID;
@@ -609,9 +943,10 @@ extern "C" int ps_batch_create_ex(ps_ctx *c, ps_polygon *polys, int npoly, doubl
     ps_batch *b = new ps_batch();
     b->c = c;
     b->npoly = npoly;
+    b->method = lp.method;
     b->grid_cap = lp.grid_cap;
     b->chunk_tiles = lp.chunk_tiles;
-    b->overlap = lp.overlap;
+    b->overlap = (lp.method == PS_METHOD_DIRECT) ? lp.overlap : 0;
     b->g2_div = (tile_rows(c->N, c->fp64) / lp.g2_div >= 4) ? lp.g2_div : 1;
     // 1. legs of every polygon, de-duplicated by (slot, mask)
     std::vector<std::vector<int>> pleg(npoly);
@@ -635,15 +970,17 @@ extern "C" int ps_batch_create_ex(ps_ctx *c, ps_polygon *polys, int npoly, doubl
             pleg[p].push_back(id);
         }
     }
-    // groups: polygons sharing their prefix legs, in a fixed order
-    std::map<std::tuple<int, int, int>, std::vector<int>> bykey;  // (npre-kind, a, b) -> polygons
+    // groups: polygons of one field slot sharing their prefix legs, in a fixed order
+    typedef std::tuple<int, int, int, int> GKey;  // (kind, leg a, leg b, slot)
+    std::map<GKey, std::vector<int>> bykey;
     for (int p = 0; p < npoly; ++p) {
         const std::vector<int> &l = pleg[p];
-        if (polys[p].n == 3) bykey[{3, l[0], l[1]}].push_back(p);
-        else if (l[0] == l[1]) bykey[{1, -1, -1}].push_back(p);
-        else bykey[{2, l[0], -1}].push_back(p);
+        const int sl = polys[p].slot;
+        if (polys[p].n == 3) bykey[GKey{3, l[0], l[1], sl}].push_back(p);
+        else if (l[0] == l[1]) bykey[GKey{1, -1, -1, sl}].push_back(p);
+        else bykey[GKey{2, l[0], -1, sl}].push_back(p);
     }
-    std::vector<std::pair<std::tuple<int, int, int>, std::vector<int>>> glist(bykey.begin(), bykey.end());
+    std::vector<std::pair<GKey, std::vector<int>>> glist(bykey.begin(), bykey.end());
     // order the legs by the first group that uses them, so that the legs a
     // group introduces form one contiguous range of tiles
     {
@@ -680,12 +1017,9 @@ extern "C" int ps_batch_create_ex(ps_ctx *c, ps_polygon *polys, int npoly, doubl
             if (std::get<2>(t) >= 0) std::get<2>(t) = newid[std::get<2>(t)];
         }
     }
-    for (auto &kv : keys) b->leg_slot.push_back(kv.first);
-    // 2. leg descriptors, buffers and tile ranges
-    const size_t cs = c->csize();
-    const int tl = g_tile_lines(N, c->fp64);
-    double bytes = 0.0;
+    // 2. host leg table (buffers are the method's)
     for (auto &kv : keys) {
+        b->leg_slot.push_back(kv.first);
         LegDev L{};
         L.mask = kv.second;
         L.spec = c->spec[kv.first];
@@ -693,41 +1027,12 @@ extern "C" int ps_batch_create_ex(ps_ctx *c, ps_polygon *polys, int npoly, doubl
         const int K = shell_band(kv.second, N);
         L.Kz = K;
         L.Kx = K;
-        b->kz_cap = std::max(b->kz_cap, K + 1);
         L.Bx = std::min(2 * K + 1, N);
-        L.g1_begin = b->g1_tiles;
-        L.g2_begin = b->g2_tiles;
-        b->g1_tiles += (int64_t)(K + 1) * ((L.Bx + tl - 1) / tl);
-        b->g2_tiles += 2 * (int64_t)(K + 1) * (N / (tile_rows(N, c->fp64) / b->g2_div));
-        const size_t n1 = (size_t)(K + 1) * L.Bx * N, n2 = (size_t)(K + 1) * N * N;
-        void *p1 = nullptr, *p2 = nullptr;
-        if (hipMalloc(&p1, cs * n1 * 2) != hipSuccess || hipMalloc(&p2, cs * n2 * 2) != hipSuccess) {
-            if (p1) (void)hipFree(p1);
-            ps_batch_destroy(b);
-            return PS_ERR_NOMEM;
-        }
-        b->allocs.push_back(p1);
-        b->allocs.push_back(p2);
-        L.g1d = p1;
-        L.g1i = (char *)p1 + cs * n1;
-        L.g2d = p2;
-        L.g2i = (char *)p2 + cs * n2;
         b->legs.push_back(L);
-        // model bytes: G1 reads the spectrum cells of the band box and writes
-        // the D,I band lines; G2 reads them and writes the D,I planes
-        b->stage_bytes[0] += (double)cs * ((double)(K + 1) * L.Bx * std::min(2 * K + 1, N) + 2.0 * n1);
-        b->stage_bytes[1] += (double)cs * (2.0 * n1 + 2.0 * n2);
     }
-    // 3. group descriptors (legs renumbered above)
-    const int rows = fused_tile_rows(N, c->fp64);
-    const int64_t tiles_per_group = (int64_t)N * (N / rows);
-    const int chunk_tiles = b->chunk_tiles;
-    const int nchunks = (int)((tiles_per_group + chunk_tiles - 1) / chunk_tiles);
-    b->max_chunks = nchunks;
+    // 3. group descriptors
     b->poly_out.assign(npoly, -1);
     b->poly_n.assign(npoly, 0);
-    std::vector<int> counts;
-    int legs_seen = 0;
     for (auto &kv : glist) {
         const int kind = std::get<0>(kv.first);
         const std::vector<int> &ps_ = kv.second;
@@ -749,8 +1054,7 @@ extern "C" int ps_batch_create_ex(ps_ctx *c, ps_polygon *polys, int npoly, doubl
             G.nsuf = (int)(s1 - s0);
             G.suf_begin = (int)b->suffix.size();
             G.out_begin = b->nout;
-            G.chunk_tiles = chunk_tiles;
-            G.item_begin = b->items;
+            G.chunk_tiles = b->chunk_tiles;
             for (size_t q = s0; q < s1; ++q) {
                 const int p = ps_[q];
                 const std::vector<int> &l = pleg[p];
@@ -758,56 +1062,26 @@ extern "C" int ps_batch_create_ex(ps_ctx *c, ps_polygon *polys, int npoly, doubl
                 b->poly_out[p] = b->nout;
                 b->poly_n[p] = polys[p].n;
                 b->nout++;
-                counts.push_back(nchunks);
             }
-            b->items += nchunks;
             b->groups.push_back(G);
-            // the legs this group introduces: [legs_seen, hi)
-            int hi = legs_seen;
-            if (G.npre >= 1) hi = std::max(hi, G.preA + 1);
-            if (G.npre == 2) hi = std::max(hi, G.preB + 1);
-            for (size_t q = s0; q < s1; ++q) hi = std::max(hi, b->suffix[G.suf_begin + (q - s0)] + 1);
-            GroupRange R;
-            R.g1a = legs_seen < (int)b->legs.size() ? b->legs[legs_seen].g1_begin : b->g1_tiles;
-            R.g1b = hi < (int)b->legs.size() ? b->legs[hi].g1_begin : b->g1_tiles;
-            R.g2a = legs_seen < (int)b->legs.size() ? b->legs[legs_seen].g2_begin : b->g2_tiles;
-            R.g2b = hi < (int)b->legs.size() ? b->legs[hi].g2_begin : b->g2_tiles;
-            if (hi <= legs_seen) R.g1b = R.g1a, R.g2b = R.g2a;
-            R.i0 = G.item_begin;
-            R.i1 = G.item_begin + nchunks;
-            b->ranges.push_back(R);
-            legs_seen = std::max(legs_seen, hi);
-            // model bytes of the fused pass: every leg's D,I planes read once per group
-            int64_t planes = 0;
-            if (G.npre >= 1) planes += b->legs[G.preA].Kz + 1;
-            if (G.npre == 2) planes += b->legs[G.preB].Kz + 1;
-            for (size_t q = s0; q < s1; ++q) planes += b->legs[b->suffix[G.suf_begin + (q - s0)]].Kz + 1;
-            b->stage_bytes[2] += 2.0 * cs * (double)planes * N * N;
         }
     }
-    b->stage_bytes[3] = 32.0 * (double)b->nout * b->max_chunks;
-    b->stage_bytes[2] += b->stage_bytes[3];
-    b->model_bytes = b->stage_bytes[0] + b->stage_bytes[1] + b->stage_bytes[2] + b->stage_bytes[3];
-    (void)bytes;
-    // 4. device tables
-    int st = PS_OK;
+    // 4. the method's buffers, tiles and device tables
+    std::vector<int> counts(b->nout, 0);
+    int st = (b->method == PS_METHOD_DIRECT) ? plan_direct(b, counts) : plan_fourier(b, counts);
+    if (st != PS_OK) {
+        ps_batch_destroy(b);
+        return st;
+    }
+    b->model_bytes = 0.0;
+    for (int i = 0; i < b->nstages; ++i) b->model_bytes += b->stage_bytes[i];
 #define BCK(call)                                       \
     do {                                                \
         if ((call) != hipSuccess) { st = PS_ERR_HIP; goto fail; } \
     } while (0)
-    if (!c->fp64) {
-        BCK(hipMalloc(&b->dscale, sizeof(double) * b->legs.size()));
-        for (size_t i = 0; i < b->legs.size(); ++i) b->legs[i].scale = b->dscale + i;
-    }
-    BCK(hipMalloc(&b->dlegs, sizeof(LegDev) * b->legs.size()));
-    BCK(hipMalloc(&b->dgroups, sizeof(GroupDev) * b->groups.size()));
-    BCK(hipMalloc(&b->dsuffix, sizeof(int) * b->suffix.size()));
     BCK(hipMalloc(&b->dcounts, sizeof(int) * counts.size()));
     BCK(hipMalloc(&b->dpartials, sizeof(double2) * 2 * (size_t)b->nout * b->max_chunks));
     BCK(hipMalloc(&b->dsum, sizeof(double) * 2 * (size_t)b->nout));
-    BCK(hipMemcpy(b->dlegs, b->legs.data(), sizeof(LegDev) * b->legs.size(), hipMemcpyHostToDevice));
-    BCK(hipMemcpy(b->dgroups, b->groups.data(), sizeof(GroupDev) * b->groups.size(), hipMemcpyHostToDevice));
-    BCK(hipMemcpy(b->dsuffix, b->suffix.data(), sizeof(int) * b->suffix.size(), hipMemcpyHostToDevice));
     BCK(hipMemcpy(b->dcounts, counts.data(), sizeof(int) * counts.size(), hipMemcpyHostToDevice));
     BCK(hipEventCreate(&b->ev0));
     BCK(hipEventCreate(&b->ev1));
@@ -1126,33 +1400,8 @@ extern "C" int ps_batch_autotune(ps_ctx *c, ps_polygon *polys, int npoly, double
     const int divs[] = {1, 2};
     std::vector<ps_polygon> work(polys, polys + npoly);
     std::vector<double> v(npoly);
-    ps_launch_params bp = default_params(c);
-    double bms = 1e300;
-    for (int cap : caps)
-        for (int ch : chunks)
-            for (int dv : divs) {
-                ps_launch_params lp{cap, ch, dv, 0};
-                ps_batch *b = nullptr;
-                std::copy(polys, polys + npoly, work.begin());
-                int s = ps_batch_create_ex(c, work.data(), npoly, bw, kcase, &lp, &b);
-                if (s != PS_OK) return s;
-                double t = 0.0;
-                s = ps_batch_execute(b, v.data(), nullptr, nullptr);  // warm-up
-                for (int r = 0; r < reps && s == PS_OK; ++r) {
-                    s = ps_batch_execute(b, v.data(), nullptr, nullptr);
-                    t += b->last_ms;
-                }
-                ps_batch_destroy(b);
-                if (s != PS_OK) return s;
-                t /= reps;
-                if (t < bms) {
-                    bms = t;
-                    bp = lp;
-                }
-            }
-    {  // the overlapped schedule at the best shape
-        ps_launch_params lp = bp;
-        lp.overlap = 1;
+    // mean replay time of one candidate (after a warm-up replay)
+    auto trial = [&](const ps_launch_params &lp, double *ms) -> int {
         ps_batch *b = nullptr;
         std::copy(polys, polys + npoly, work.begin());
         int s = ps_batch_create_ex(c, work.data(), npoly, bw, kcase, &lp, &b);
@@ -1164,8 +1413,36 @@ extern "C" int ps_batch_autotune(ps_ctx *c, ps_polygon *polys, int npoly, double
             t += b->last_ms;
         }
         ps_batch_destroy(b);
+        *ms = t / reps;
+        return s;
+    };
+    ps_launch_params bp = default_params(c);
+    double bms = 1e300;
+    ps_launch_params bdirect = bp;
+    double bdms = 1e300;
+    for (int method : {PS_METHOD_FOURIER, PS_METHOD_DIRECT})
+        for (int cap : caps)
+            for (int ch : chunks)
+                for (int dv : divs) {
+                    const ps_launch_params lp{cap, ch, dv, 0, method};
+                    double t = 0.0;
+                    const int s = trial(lp, &t);
+                    if (s != PS_OK) return s;
+                    if (t < bms) {
+                        bms = t;
+                        bp = lp;
+                    }
+                    if (method == PS_METHOD_DIRECT && t < bdms) {
+                        bdms = t;
+                        bdirect = lp;
+                    }
+                }
+    {  // the overlapped schedule at the best direct shape
+        ps_launch_params lp = bdirect;
+        lp.overlap = 1;
+        double t = 0.0;
+        const int s = trial(lp, &t);
         if (s != PS_OK) return s;
-        t /= reps;
         if (t < bms) {
             bms = t;
             bp = lp;
@@ -1176,7 +1453,8 @@ extern "C" int ps_batch_autotune(ps_ctx *c, ps_polygon *polys, int npoly, double
     g_tuned[key] = bp;
     if (const char *f = getenv("PS_TUNE_FILE")) {
         if (FILE *fp = fopen(f, "a")) {
-            fprintf(fp, "%s %d %d %d %d\n", key.c_str(), bp.grid_cap, bp.chunk_tiles, bp.g2_div, bp.overlap);
+            fprintf(fp, "%s %d %d %d %d %d\n", key.c_str(), bp.grid_cap, bp.chunk_tiles, bp.g2_div, bp.overlap,
+                    bp.method);
             fclose(fp);
         }
     }
@@ -1192,6 +1470,7 @@ extern "C" int ps_batch_params(const ps_batch *b, ps_launch_params *out)
     out->chunk_tiles = b->chunk_tiles;
     out->g2_div = b->g2_div;
     out->overlap = b->overlap;
+    out->method = b->method;
     return PS_OK;
 }
 
@@ -1200,43 +1479,88 @@ extern "C" int ps_batch_profile(ps_batch *b, int reps, double *stage_ms, double 
     if (!b || reps < 1 || !stage_ms) return PS_ERR_INVALID;
     ps_ctx *c = b->c;
     HIPCK(hipSetDevice(c->device));
-    hipEvent_t ev[5];
-    for (auto &e : ev) HIPCK(hipEventCreate(&e));
-    double acc[4] = {0, 0, 0, 0};
+    const int N = c->N, fp = c->fp64;
+    const hipStream_t s = c->st;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> marks;
     int st = PS_OK;
-    for (int r = 0; r < reps && st == PS_OK; ++r) {
-        (void)hipEventRecord(ev[0], c->st);
-        if (b->dscale) st = run_leg_scale(c->fp64, c->N, b->dlegs, (int)b->legs.size(), c->st);
-        if (st == PS_OK)
-            st = run_inverse_g(c->fp64, c->N, b->dlegs, (int)b->legs.size(), 0, b->g1_tiles, 0, 0, b->grid_cap,
-                               b->g2_div, c->tw, c->st);
-        (void)hipEventRecord(ev[1], c->st);
-        if (st == PS_OK)
-            st = run_inverse_g(c->fp64, c->N, b->dlegs, (int)b->legs.size(), 0, 0, 0, b->g2_tiles, b->grid_cap,
-                               b->g2_div, c->tw, c->st);
-        (void)hipEventRecord(ev[2], c->st);
-        if (st == PS_OK)
-            st = run_zc_fused(c->fp64, c->N, b->dgroups, (int)b->groups.size(), b->dlegs, b->dsuffix, 0, b->items,
-                              b->dpartials, b->max_chunks, b->grid_cap, b->kz_cap, c->tw, c->st);
-        (void)hipEventRecord(ev[3], c->st);
-        if (st == PS_OK)
-            st = run_reduce_partials(b->dpartials, b->max_chunks, b->dcounts, 0, b->nout, b->dsum,
-                                     b->dsum + b->nout, c->st);
-        (void)hipEventRecord(ev[4], c->st);
-        if (hipEventSynchronize(ev[4]) != hipSuccess) st = PS_ERR_HIP;
-        for (int i = 0; i < 4 && st == PS_OK; ++i) {
-            float ms = 0.f;
-            (void)hipEventElapsedTime(&ms, ev[i], ev[i + 1]);
-            acc[i] += ms;
+    // one pass (over the whole batch, or one wave of it) between two events
+    auto timed = [&](int stage, auto &&launch) {
+        if (st != PS_OK) return;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+            if (e0) (void)hipEventDestroy(e0);
+            st = PS_ERR_HIP;
+            return;
         }
+        marks.push_back({stage, {e0, e1}});
+        (void)hipEventRecord(e0, s);
+        st = launch();
+        (void)hipEventRecord(e1, s);
+    };
+    for (int r = 0; r < reps && st == PS_OK; ++r) {
+        if (b->method == PS_METHOD_FOURIER) {
+            for (SpecWave &w : b->waves) {
+                const int nl = (int)w.legs.size(), ng = (int)w.groups.size();
+                if (nl > 0) {
+                    timed(0, [&] {
+                        const int x = b->dscale ? run_leg_scale(fp, N, w.dlegs, nl, s) : PS_OK;
+                        return x != PS_OK ? x
+                                          : run_inverse_g(fp, N, w.dlegs, nl, 0, w.g1_tiles, 0, 0, b->grid_cap,
+                                                          b->g2_div, c->tw, s);
+                    });
+                    timed(1, [&] {
+                        return run_inverse_g(fp, N, w.dlegs, nl, 0, 0, 0, w.g2_tiles, b->grid_cap, b->g2_div, c->tw, s);
+                    });
+                }
+                timed(2, [&] {
+                    return run_spec_zf(fp, N, w.dgroups, ng, w.dlegs, 0, w.zf_tiles, b->grid_cap, w.kz_cap, c->tw, s);
+                });
+                timed(3, [&] { return run_spec_fx(fp, N, w.dgroups, ng, 0, w.fx_tiles, b->grid_cap, b->g2_div, c->tw, s); });
+                timed(4, [&] {
+                    return run_spec_dot(fp, N, w.dgroups, ng, b->dmasks, b->divl, 0, w.dot_items, b->dpartials,
+                                        b->max_chunks, b->grid_cap, c->tw, s);
+                });
+            }
+        } else {
+            const int nl = (int)b->legs.size();
+            timed(0, [&] {
+                const int x = b->dscale ? run_leg_scale(fp, N, b->dlegs, nl, s) : PS_OK;
+                return x != PS_OK ? x
+                                  : run_inverse_g(fp, N, b->dlegs, nl, 0, b->g1_tiles, 0, 0, b->grid_cap, b->g2_div,
+                                                  c->tw, s);
+            });
+            timed(1, [&] { return run_inverse_g(fp, N, b->dlegs, nl, 0, 0, 0, b->g2_tiles, b->grid_cap, b->g2_div, c->tw, s); });
+            timed(2, [&] {
+                return run_zc_fused(fp, N, b->dgroups, (int)b->groups.size(), b->dlegs, b->dsuffix, 0, b->items,
+                                    b->dpartials, b->max_chunks, b->grid_cap, b->kz_cap, c->tw, s);
+            });
+        }
+        timed(b->nstages - 1, [&] {
+            return run_reduce_partials(b->dpartials, b->max_chunks, b->dcounts, 0, b->nout, b->dsum, b->dsum + b->nout, s);
+        });
     }
-    for (auto &e : ev) (void)hipEventDestroy(e);
+    if (hipStreamSynchronize(s) != hipSuccess && st == PS_OK) st = PS_ERR_HIP;
+    double acc[PS_MAX_STAGES] = {};
+    for (auto &m : marks) {
+        float ms = 0.f;
+        if (st == PS_OK && hipEventElapsedTime(&ms, m.second.first, m.second.second) == hipSuccess) acc[m.first] += ms;
+        (void)hipEventDestroy(m.second.first);
+        (void)hipEventDestroy(m.second.second);
+    }
     if (st != PS_OK) return st > 0 ? PS_ERR_HIP : st;
-    for (int i = 0; i < 4; ++i) {
-        stage_ms[i] = acc[i] / reps;
-        if (stage_bytes) stage_bytes[i] = b->stage_bytes[i];
+    for (int i = 0; i < PS_MAX_STAGES; ++i) {
+        stage_ms[i] = (i < b->nstages) ? acc[i] / reps : 0.0;
+        if (stage_bytes) stage_bytes[i] = (i < b->nstages) ? b->stage_bytes[i] : 0.0;
     }
     return PS_OK;
+}
+
+extern "C" int ps_batch_stage_count(const ps_batch *b) { return b ? b->nstages : 0; }
+
+extern "C" const char *ps_batch_stage_name(const ps_batch *b, int stage)
+{
+    if (!b || stage < 0 || stage >= b->nstages) return nullptr;
+    return b->method == PS_METHOD_FOURIER ? kStageFourier[stage] : kStageDirect[stage];
 }
 
 extern "C" int ps_forward_profile(ps_ctx *c, int slot, int reps, double *ms, double *bytes)

@@ -1,0 +1,37 @@
+// ps_device.hpp -- small device helpers shared by the kernel files.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "ps_fft_core.hpp"
+#include "ps_internal.h"
+
+namespace ps {
+
+// rows per tile of the z passes and lines per tile of the other passes
+template <typename T, int N> constexpr int tile_lines() { return tile_lines_for(N, sizeof(T) == 8); }
+
+// band index j in [0, B) <-> grid index k of the band |m| <= K (B = min(2K+1, N))
+__device__ __forceinline__ int band_to_k(int j, int K, int B, int N) { return (B == N) ? j : (j <= K ? j : j + N - B); }
+__device__ __forceinline__ int k_to_band(int k, int K, int B, int N)
+{
+    if (B == N) return k;
+    if (k <= K) return k;
+    if (k >= N - K) return k - N + B;
+    return -1;
+}
+
+// find the leg (or group) owning a tile: largest i with begin[i] <= t
+template <typename D>
+__device__ __forceinline__ int find_owner(const D *__restrict__ d, int n, int64_t t, int64_t D::*field)
+{
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (d[mid].*field <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+}  // namespace ps

@@ -47,6 +47,37 @@ struct GroupDev {
     int64_t item_begin;  // first work item of this group
 };
 
+// A polygon group evaluated in Fourier space.  Output o of the group is the
+// real-space sum  sum_x P(x) D_o(x)  written as a sum over the half spectrum
+//   S_o = sum_k w(kz) Re(conj Q(k) V(k)) m_o(k),   w = 1 on kz = 0, N/2, else 2,
+// with V the field's spectrum (D sums) or 1 (I sums), m_o the suffix shell and
+// Q = F[P] the forward transform of the prefix product (Parseval).
+//   fft = 1 (triangles): P = product of the prefix legs in real space, taken
+//     through G1, G2 and the Zf pass (inverse), then Zf, Fx and the dot pass
+//     (forward), kept on the band |mx|, |my| <= Kq, kz <= Kqz.
+//   fft = 0 (power spectra): Q = N^3 m_A H read from the spectrum (has_A = 0:
+//     Q = N^3 H, the suffix leg is its own partner).
+struct SpecGroupDev {
+    int fft;
+    int npre, pre_sq;        // prefix legs (fft = 1)
+    int preA, preB;          // indices into the pass's leg table
+    int nsuf, suf_begin;     // suffix masks: masks[suf_begin + o], o < nsuf
+    int out_begin;           // first output slot
+    int Kq, Kqz, Bq;         // band of Q; Bq = min(2 Kq + 1, N)
+    int has_A;               // fft = 0: Q carries the mask mA
+    int phase_only;          // V = H / |H| on the selected cells
+    int dot_lines;           // lines (D and I lines counted apart) per dot work item
+    MaskParams mA;
+    const void *spec;        // half spectrum of the group's field slot
+    void *qd, *qi;           // Zf output planes [Kqz+1][N][N] (D, I)
+    void *bd, *bi;           // Fx output band [Kqz+1][Bq][N]
+    const double *scaleA;    // amplitude scales of the prefix legs (mixed precision), may be null
+    const double *scaleB;
+    int64_t zf_begin;        // first Zf tile of the group
+    int64_t fx_begin;        // first Fx tile
+    int64_t dot_begin;       // first dot work item
+};
+
 int supported_grid(int N);
 int tile_rows(int N, int fp64);
 int fused_tile_rows(int N, int fp64);
@@ -73,5 +104,20 @@ int run_apply_masks(int fp64, int N, void *H, const MaskParams *masks, const dou
 int run_add_spec(int fp64, void *a, const void *b, int64_t n, hipStream_t st);
 int run_mul_fields(double *out, const double *a, const double *b, double alpha, int64_t n, hipStream_t st);
 int run_corner_extremes(const MaskParams &mp, int N, int K, double bw, double2 *partials, int nblocks, hipStream_t st);
+
+// Fourier-space groups: tiles [t0, t1) of the Zf and Fx passes, work items
+// [i0, i1) of the dot pass.  run_shell_intervals writes, per mask s,
+// ivl[2s] = (lo, hi), the m^2 interval the mask selects ((1, 0): empty or not
+// radial), and ivl[2s+1].x = the number of m^2 values it selects.
+int spec_zf_rows(int N, int fp64);
+int spec_dot_lines(int N, int fp64);  // lines one dot workgroup transforms per step
+int run_spec_zf(int fp64, int N, const SpecGroupDev *groups, int ngroups, const LegDev *legs, int64_t t0, int64_t t1,
+                int grid_cap, int kz_cap, const void *tw, hipStream_t st);
+int run_spec_fx(int fp64, int N, const SpecGroupDev *groups, int ngroups, int64_t t0, int64_t t1, int grid_cap,
+                int div, const void *tw, hipStream_t st);
+int run_spec_dot(int fp64, int N, const SpecGroupDev *groups, int ngroups, const MaskParams *masks, const int2 *ivl,
+                 int64_t i0, int64_t i1, double2 *partials, int max_chunks, int grid_cap, const void *tw,
+                 hipStream_t st);
+int run_shell_intervals(const MaskParams *masks, const int *kmax, int n, int2 *out, hipStream_t st);
 
 }  // namespace ps

@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "ps_device.hpp"
 #include "ps_fft_core.hpp"
 #include "ps_internal.h"
 #include "ps_mask.hpp"
@@ -32,9 +33,6 @@ namespace ps {
 // ---------------------------------------------------------------- helpers
 template <typename T> __device__ __forceinline__ T ld_real(const float *p, int64_t i) { return (T)p[i]; }
 template <typename T> __device__ __forceinline__ T ld_real(const double *p, int64_t i) { return (T)p[i]; }
-
-// rows per tile of the z passes and lines per tile of the other passes
-template <typename T, int N> constexpr int tile_lines() { return tile_lines_for(N, sizeof(T) == 8); }
 
 // ================================================================ forward
 // Z pass: R2C along z.  Two real rows share one complex FFT (z = a + i b);
@@ -105,27 +103,6 @@ __global__ void __launch_bounds__(NT) k_c2c_strided(cplx<T> *__restrict__ data, 
 }
 
 // ================================================================ inverse
-__device__ __forceinline__ int band_to_k(int j, int K, int B, int N) { return (B == N) ? j : (j <= K ? j : j + N - B); }
-__device__ __forceinline__ int k_to_band(int k, int K, int B, int N)
-{
-    if (B == N) return k;
-    if (k <= K) return k;
-    if (k >= N - K) return k - N + B;
-    return -1;
-}
-
-// find the leg (or group) owning a tile: largest i with begin[i] <= t
-template <typename D>
-__device__ __forceinline__ int find_owner(const D *__restrict__ d, int n, int64_t t, int64_t D::*field)
-{
-    int lo = 0, hi = n - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (d[mid].*field <= t) lo = mid;
-        else hi = mid - 1;
-    }
-    return lo;
-}
 
 // G1: masked gather + C2C along ky.  Tile = (leg, kz, band tile of TLC kx
 // lines); LDS holds TLC D-lines (mask * spectrum) and TLC I-lines (mask).

@@ -88,4 +88,23 @@ __device__ __forceinline__ bool mask_eval(const MaskParams &p, int mx, int my, i
     }
 }
 
+// DEFAULT, DETERMINING and RANGE membership depend on the cell only through
+// the integer m2 = mx^2 + my^2 + mz^2 (the same int sum and FP64 square root
+// as mask_eval), and monotonically: a shell is an interval [lo, hi] of m2.
+__host__ __device__ __forceinline__ bool mask_is_radial(int kind)
+{
+    return kind != MASK_WATKINSON && kind != MASK_WATKINSON_KN;
+}
+
+__device__ __forceinline__ bool mask_eval_m2(const MaskParams &p, int m2)
+{
+#pragma clang fp contract(off)
+    const double m = sqrt((double)m2);
+    if (p.kind == MASK_RANGE) {
+        const double emin = p.mmin - m, emax = m - p.mmax;
+        return (emin <= 0.5 * p.bw && emax <= 0.5 * p.bw);
+    }
+    return fabs(p.kkf - m) <= 0.5 * p.bw;
+}
+
 }  // namespace ps

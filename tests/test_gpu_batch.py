@@ -116,14 +116,18 @@ def test_autotune_and_params(gpu_lib):
         ctx.load_field(f)
         b0 = ctx.batch(polys, 1.0, 0, params={"grid_cap": 64, "chunk_tiles": 1, "g2_div": 2})
         v0, _, _ = b0.execute()
-        assert b0.params == {"grid_cap": 64, "chunk_tiles": 1, "g2_div": 2, "overlap": 0}
+        assert b0.params == {"grid_cap": 64, "chunk_tiles": 1, "g2_div": 2, "overlap": 0, "method": 0}
         b0.close()
-        # the two-stream schedule sums in the same order
-        bo = ctx.batch(polys, 1.0, 0, params={"grid_cap": 64, "chunk_tiles": 1, "g2_div": 2, "overlap": 1})
-        vo, _, _ = bo.execute()
-        assert bo.params["overlap"] == 1
-        bo.close()
-        np.testing.assert_array_equal(vo, v0)
+        # the direct method with the two-stream schedule sums in the same order as without
+        pd = {"grid_cap": 64, "chunk_tiles": 1, "g2_div": 2, "method": P.METHOD_DIRECT}
+        vd = []
+        for ov in (0, 1):
+            bo = ctx.batch(polys, 1.0, 0, params=dict(pd, overlap=ov))
+            vd.append(bo.execute()[0])
+            assert bo.params["overlap"] == ov and bo.params["method"] == P.METHOD_DIRECT
+            bo.close()
+        np.testing.assert_array_equal(vd[1], vd[0])
+        np.testing.assert_allclose(vd[0], v0, rtol=1e-11)
         best, ms = ctx.autotune(polys, 1.0, 0, reps=1)
         assert ms > 0 and best["grid_cap"] > 0
         b1 = ctx.batch(polys, 1.0, 0)
@@ -135,8 +139,9 @@ def test_autotune_and_params(gpu_lib):
 
 @pytest.mark.parametrize("prec_name", ["PREC_FP64", "PREC_MIXED"])
 def test_overlap_schedule_matches(gpu_lib, prec_name):
-    """Several polygon groups (two triangle prefixes, one power-spectrum group):
-    the two-stream schedule gives the single-stream values bit for bit."""
+    """Several polygon groups (two triangle prefixes, one power-spectrum group),
+    direct method: the two-stream schedule gives the single-stream values bit
+    for bit."""
     P = gpu_lib
     N, L = 64, 100.0
     f = lognormal_field(N, 41)
@@ -147,7 +152,8 @@ def test_overlap_schedule_matches(gpu_lib, prec_name):
     with P.Context(N, L, getattr(P, prec_name)) as ctx:
         ctx.load_field(f)
         for ov in (0, 1):
-            b = ctx.batch(polys, 1.0, 2, params={"grid_cap": 512, "chunk_tiles": 16, "g2_div": 1, "overlap": ov})
+            b = ctx.batch(polys, 1.0, 2, params={"grid_cap": 512, "chunk_tiles": 16, "g2_div": 1, "overlap": ov,
+                                                 "method": P.METHOD_DIRECT})
             out[ov] = b.execute()
             again = b.execute()
             b.close()

