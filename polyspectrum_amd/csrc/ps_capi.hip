@@ -62,6 +62,7 @@ struct ps_ctx {
     hipStream_t st = nullptr;
     hipStream_t st2 = nullptr;           // second stream: fused passes overlap the transforms
     void *tw = nullptr;                  // forward twiddles exp(-2 pi i t/N), N entries
+    std::map<int, void *> twm;           // twiddle tables of smaller transform grids
     std::vector<void *> spec;            // half spectra per slot
     std::vector<int> loaded;
     void *staging = nullptr;             // field upload buffer (N^3 doubles)
@@ -122,6 +123,7 @@ extern "C" int ps_ctx_destroy(ps_ctx *c)
     (void)hipSetDevice(c->device);
     if (c->st) (void)hipStreamSynchronize(c->st);
     for (void *p : c->spec) if (p) (void)hipFree(p);
+    for (auto &kv : c->twm) (void)hipFree(kv.second);
     void *bufs[] = {c->tw, c->staging, c->gd, c->gi, c->accD, c->accI, c->partials, c->dsum, c->dleg, c->dcorner};
     for (void *p : bufs) if (p) (void)hipFree(p);
     if (c->st2) (void)hipStreamSynchronize(c->st2);
@@ -363,6 +365,7 @@ static int plain_leg(ps_ctx *c, const MaskParams &mp, const void *spec, int mode
     L.Kz = N / 2;
     L.Kx = N / 2;
     L.Bx = N;
+    L.ns = N;
     L.g1_begin = 0;
     L.g2_begin = 0;
     L.g1d = spec ? c->gd : nullptr;
@@ -442,6 +445,8 @@ struct GroupRange {
 // wave transforms and the planes of its groups live in one buffer arena that
 // every wave reuses in turn.
 struct SpecWave {
+    int M = 0;                           // transform grid of the wave (<= the spectrum's N)
+    const void *tw = nullptr;            // its twiddle table
     std::vector<LegDev> legs;            // prefix legs transformed in this wave
     std::vector<SpecGroupDev> groups;
     std::vector<size_t> leg_off;         // arena offsets of the legs' G1/G2 buffers
@@ -547,16 +552,16 @@ static int batch_record_on(ps_batch *b, hipStream_t s, hipStream_t s2)
     const int N = c->N;
     if (b->method == PS_METHOD_FOURIER) {
         for (SpecWave &w : b->waves) {
-            const int nl = (int)w.legs.size(), ng = (int)w.groups.size();
+            const int nl = (int)w.legs.size(), ng = (int)w.groups.size(), M = w.M;
             if (nl > 0) {
-                if (b->dscale) PSCK(run_leg_scale(c->fp64, N, w.dlegs, nl, s));
-                PSCK(run_inverse_g(c->fp64, N, w.dlegs, nl, 0, w.g1_tiles, 0, w.g2_tiles, b->grid_cap, b->g2_div,
-                                   c->tw, s));
+                if (b->dscale) PSCK(run_leg_scale(c->fp64, M, w.dlegs, nl, s));
+                PSCK(run_inverse_g(c->fp64, M, w.dlegs, nl, 0, w.g1_tiles, 0, w.g2_tiles, b->grid_cap, b->g2_div,
+                                   w.tw, s));
             }
-            PSCK(run_spec_zf(c->fp64, N, w.dgroups, ng, w.dlegs, 0, w.zf_tiles, b->grid_cap, w.kz_cap, c->tw, s));
-            PSCK(run_spec_fx(c->fp64, N, w.dgroups, ng, 0, w.fx_tiles, b->grid_cap, b->g2_div, c->tw, s));
-            PSCK(run_spec_dot(c->fp64, N, w.dgroups, ng, b->dmasks, b->divl, 0, w.dot_items, b->dpartials,
-                              b->max_chunks, b->grid_cap, c->tw, s));
+            PSCK(run_spec_zf(c->fp64, M, w.dgroups, ng, w.dlegs, 0, w.zf_tiles, b->grid_cap, w.kz_cap, w.tw, s));
+            PSCK(run_spec_fx(c->fp64, M, w.dgroups, ng, 0, w.fx_tiles, b->grid_cap, b->g2_div, w.tw, s));
+            PSCK(run_spec_dot(c->fp64, M, w.dgroups, ng, b->dmasks, b->divl, 0, w.dot_items, b->dpartials,
+                              b->max_chunks, b->grid_cap, w.tw, s));
         }
         PSCK(run_reduce_partials(b->dpartials, b->max_chunks, b->dcounts, 0, b->nout, b->dsum, b->dsum + b->nout, s));
         return PS_OK;
@@ -732,10 +737,6 @@ static int plan_fourier(ps_batch *b, std::vector<int> &counts)
     ps_ctx *c = b->c;
     const int N = c->N, fp64 = c->fp64;
     const size_t cs = c->csize();
-    const int tl = g_tile_lines(N, fp64);
-    const int zrows = spec_zf_rows(N, fp64);
-    const int fxcols = tile_rows(N, fp64) / b->g2_div;
-    const int dot_lines = b->chunk_tiles * spec_dot_lines(N, fp64);
     // suffix masks and their m^2 intervals
     const size_t ns = b->suffix.size();
     std::vector<MaskParams> sm(ns);
@@ -769,30 +770,40 @@ static int plan_fourier(ps_batch *b, std::vector<int> &counts)
         }
         if (patched) PCK(hipMemcpy(b->divl, hv.data(), sizeof(int2) * 2 * ns, hipMemcpyHostToDevice));
     }
-    // waves of groups within the arena budget
-    size_t budget = (size_t)16 << 30;
-    size_t mfree = 0, mtot = 0;
-    if (hipMemGetInfo(&mfree, &mtot) == hipSuccess && mfree / 2 < budget) budget = mfree / 2;
-    if (const char *e = getenv("PS_ARENA_MB")) budget = (size_t)std::max(1, atoi(e)) << 20;
-    auto leg_bytes = [&](const LegDev &L) -> size_t {
-        return 2 * cs * ((size_t)(L.Kz + 1) * L.Bx * N + (size_t)(L.Kz + 1) * N * N);
+    // per group: its band and the transform grid M (smallest supported size
+    // above K_A + K_B + Kq, where the product is alias-free; N at most)
+    static const int kGrids[] = {16, 24, 32, 48, 64, 96, 128, 192, 256, 384, 512};
+    auto grid_for = [&](int need) {
+        for (int m : kGrids)
+            if (m >= need && m <= N) return m;
+        return N;
     };
-    std::vector<SpecWave> &W = b->waves;
-    std::map<int, int> wave_leg;  // leg id -> index in the current wave
-    W.emplace_back();
+    std::vector<SpecGroupDev> sg;
+    std::vector<int> sgM;
     for (const GroupDev &G : b->groups) {
         SpecGroupDev S{};
         S.fft = (G.npre == 2 || G.pre_sq) ? 1 : 0;
         int Kt = 0;
         for (int o = 0; o < G.nsuf; ++o) Kt = std::max(Kt, b->legs[b->suffix[G.suf_begin + o]].Kz);
-        int Kq = Kt;
-        if (!S.fft && G.npre == 1) Kq = std::min(Kq, b->legs[G.preA].Kz);
+        int Kq = Kt, need = 0;
+        if (S.fft) {
+            // the product of the prefix legs vanishes beyond |m_i| = K_A + K_B
+            const int KA = b->legs[G.preA].Kz, KB = b->legs[G.npre == 2 ? G.preB : G.preA].Kz;
+            Kq = std::min(Kq, KA + KB);
+            need = KA + KB + Kq + 1;
+        } else {
+            if (G.npre == 1) Kq = std::min(Kq, b->legs[G.preA].Kz);
+            need = 2 * Kq + 1;
+        }
         Kq = std::min(Kq, N / 2);
+        const int M = grid_for(need);
         S.Kq = Kq;
-        S.Kqz = std::min(Kq, N / 2);
-        S.Bq = std::min(2 * Kq + 1, N);
+        S.Kqz = std::min(Kq, M / 2);
+        S.Bq = std::min(2 * Kq + 1, M);
         S.npre = S.fft ? G.npre : 0;
         S.pre_sq = G.pre_sq;
+        S.preA = G.preA;  // global leg ids here, wave-local below
+        S.preB = (G.npre == 2) ? G.preB : G.preA;
         S.nsuf = G.nsuf;
         S.suf_begin = G.suf_begin;
         S.out_begin = G.out_begin;
@@ -801,64 +812,109 @@ static int plan_fourier(ps_batch *b, std::vector<int> &counts)
         const LegDev &L0 = b->legs[b->suffix[G.suf_begin]];
         S.spec = L0.spec;
         S.phase_only = L0.phase_only;
-        S.dot_lines = dot_lines;
-        const size_t gbytes = S.fft ? 2 * cs * ((size_t)(S.Kqz + 1) * N * N + (size_t)(S.Kqz + 1) * S.Bq * N) : 0;
-        std::vector<int> pre;
-        if (S.fft) {
-            pre.push_back(G.preA);
-            if (G.npre == 2) pre.push_back(G.preB);
-        }
-        auto need_of = [&]() {
+        S.ns = N;
+        S.fscale = S.fft ? pow((double)N / (double)M, 3) : 1.0;
+        sg.push_back(S);
+        sgM.push_back(M);
+    }
+    // waves: one transform grid each, within the arena budget; grids in
+    // increasing order, groups in their order within a grid
+    size_t budget = (size_t)16 << 30;
+    size_t mfree = 0, mtot = 0;
+    if (hipMemGetInfo(&mfree, &mtot) == hipSuccess && mfree / 2 < budget) budget = mfree / 2;
+    if (const char *e = getenv("PS_ARENA_MB")) budget = (size_t)std::max(1, atoi(e)) << 20;
+    std::vector<SpecWave> &W = b->waves;
+    std::map<int, int> wave_leg;  // leg id -> index in the current wave
+    std::vector<int> grids(sgM.begin(), sgM.end());
+    std::sort(grids.begin(), grids.end());
+    grids.erase(std::unique(grids.begin(), grids.end()), grids.end());
+    for (int M : grids) {
+        auto leg_bytes = [&](const LegDev &L) -> size_t {
+            const int Bx = std::min(2 * L.Kz + 1, M);
+            return 2 * cs * ((size_t)(L.Kz + 1) * Bx * M + (size_t)(L.Kz + 1) * M * M);
+        };
+        bool fresh = true;
+        for (size_t gi = 0; gi < sg.size(); ++gi) {
+            if (sgM[gi] != M) continue;
+            SpecGroupDev S = sg[gi];
+            const size_t gbytes = S.fft ? 2 * cs * ((size_t)(S.Kqz + 1) * M * M + (size_t)(S.Kqz + 1) * S.Bq * M) : 0;
+            std::vector<int> pre;
+            if (S.fft) {
+                pre.push_back(S.preA);
+                if (S.npre == 2) pre.push_back(S.preB);
+            }
             size_t need = gbytes;
             for (int p : pre)
-                if (!wave_leg.count(p)) need += leg_bytes(b->legs[p]);
-            return need;
-        };
-        if (!W.back().groups.empty() && W.back().bytes + need_of() > budget) {
-            W.emplace_back();
-            wave_leg.clear();
+                if (fresh || !wave_leg.count(p)) need += leg_bytes(b->legs[p]);
+            if (fresh || (!W.back().groups.empty() && W.back().bytes + need > budget)) {
+                W.emplace_back();
+                W.back().M = M;
+                wave_leg.clear();
+                fresh = false;
+            }
+            SpecWave &w = W.back();
+            for (int p : pre) {
+                if (wave_leg.count(p)) continue;
+                wave_leg[p] = (int)w.legs.size();
+                LegDev L = b->legs[p];
+                L.Bx = std::min(2 * L.Kz + 1, M);
+                w.leg_off.push_back(w.bytes);
+                w.bytes += leg_bytes(L);
+                w.legs.push_back(L);
+            }
+            if (S.fft) {
+                S.preA = wave_leg[S.preA];
+                S.preB = wave_leg[S.preB];
+            } else {
+                S.preA = S.preB = 0;
+            }
+            w.grp_off.push_back(S.fft ? w.bytes : 0);
+            w.bytes += gbytes;
+            w.groups.push_back(S);
         }
-        SpecWave &w = W.back();
-        for (int p : pre) {
-            if (wave_leg.count(p)) continue;
-            wave_leg[p] = (int)w.legs.size();
-            w.leg_off.push_back(w.bytes);
-            w.bytes += leg_bytes(b->legs[p]);
-            w.legs.push_back(b->legs[p]);
-        }
-        if (S.fft) {
-            S.preA = wave_leg[G.preA];
-            S.preB = (G.npre == 2) ? wave_leg[G.preB] : S.preA;
-        }
-        w.grp_off.push_back(S.fft ? w.bytes : 0);
-        w.bytes += gbytes;
-        w.groups.push_back(S);
     }
     // tiles, work items and model bytes
     size_t arena = 0, nscale = 0;
     for (SpecWave &w : W) {
+        const int M = w.M;
+        const int tl = g_tile_lines(M, fp64);
+        const int zrows = spec_zf_rows(M, fp64);
+        const int fxcols = tile_rows(M, fp64) / b->g2_div;
+        const int dot_lines = b->chunk_tiles * spec_dot_lines(M, fp64);
+        if (M == N) {
+            w.tw = c->tw;
+        } else {
+            auto it = c->twm.find(M);
+            if (it == c->twm.end()) {
+                void *t = nullptr;
+                PSCK(fp64 ? make_twiddles<double>(M, &t) : make_twiddles<float>(M, &t));
+                it = c->twm.emplace(M, t).first;
+            }
+            w.tw = it->second;
+        }
         for (LegDev &L : w.legs) {
             const int K = L.Kz;
             w.kz_cap = std::max(w.kz_cap, K + 1);
             L.g1_begin = w.g1_tiles;
             L.g2_begin = w.g2_tiles;
             w.g1_tiles += (int64_t)(K + 1) * ((L.Bx + tl - 1) / tl);
-            w.g2_tiles += 2 * (int64_t)(K + 1) * (N / (tile_rows(N, fp64) / b->g2_div));
-            const double n1 = (double)(K + 1) * L.Bx * N, n2 = (double)(K + 1) * N * N;
-            b->stage_bytes[0] += (double)cs * ((double)(K + 1) * L.Bx * std::min(2 * K + 1, N) + 2.0 * n1);
+            w.g2_tiles += 2 * (int64_t)(K + 1) * (M / (tile_rows(M, fp64) / b->g2_div));
+            const double n1 = (double)(K + 1) * L.Bx * M, n2 = (double)(K + 1) * M * M;
+            b->stage_bytes[0] += (double)cs * ((double)(K + 1) * L.Bx * L.Bx + 2.0 * n1);
             b->stage_bytes[1] += (double)cs * (2.0 * n1 + 2.0 * n2);
         }
         for (SpecGroupDev &S : w.groups) {
+            S.dot_lines = dot_lines;
             S.zf_begin = w.zf_tiles;
             S.fx_begin = w.fx_tiles;
             S.dot_begin = w.dot_items;
-            const double planes = (double)(S.Kqz + 1) * N * N, band = (double)(S.Kqz + 1) * S.Bq * N;
+            const double planes = (double)(S.Kqz + 1) * M * M, band = (double)(S.Kqz + 1) * S.Bq * M;
             if (S.fft) {
-                w.zf_tiles += (int64_t)N * (N / zrows);
-                w.fx_tiles += 2 * (int64_t)(S.Kqz + 1) * (N / fxcols);
+                w.zf_tiles += (int64_t)M * (M / zrows);
+                w.fx_tiles += 2 * (int64_t)(S.Kqz + 1) * (M / fxcols);
                 double pre = (double)(w.legs[S.preA].Kz + 1);
                 if (S.npre == 2) pre += (double)(w.legs[S.preB].Kz + 1);
-                b->stage_bytes[2] += 2.0 * cs * (pre * N * N + planes);
+                b->stage_bytes[2] += 2.0 * cs * (pre * M * M + planes);
                 b->stage_bytes[3] += 2.0 * cs * (planes + band);
                 b->stage_bytes[4] += 2.0 * cs * band;
             }
@@ -883,9 +939,10 @@ static int plan_fourier(ps_batch *b, std::vector<int> &counts)
     size_t si = 0;
     char *base = (char *)b->arena;
     for (SpecWave &w : W) {
+        const int M = w.M;
         for (size_t i = 0; i < w.legs.size(); ++i) {
             LegDev &L = w.legs[i];
-            const size_t n1 = (size_t)(L.Kz + 1) * L.Bx * N, n2 = (size_t)(L.Kz + 1) * N * N;
+            const size_t n1 = (size_t)(L.Kz + 1) * L.Bx * M, n2 = (size_t)(L.Kz + 1) * M * M;
             char *p = base + w.leg_off[i];
             L.g1d = p;
             L.g1i = p + cs * n1;
@@ -896,7 +953,7 @@ static int plan_fourier(ps_batch *b, std::vector<int> &counts)
         for (size_t g = 0; g < w.groups.size(); ++g) {
             SpecGroupDev &S = w.groups[g];
             if (S.fft) {
-                const size_t np = (size_t)(S.Kqz + 1) * N * N, nb = (size_t)(S.Kqz + 1) * S.Bq * N;
+                const size_t np = (size_t)(S.Kqz + 1) * M * M, nb = (size_t)(S.Kqz + 1) * S.Bq * M;
                 char *p = base + w.grp_off[g];
                 S.qd = p;
                 S.qi = p + cs * np;
@@ -1028,6 +1085,7 @@ extern "C" int ps_batch_create_ex(ps_ctx *c, ps_polygon *polys, int npoly, doubl
         L.Kz = K;
         L.Kx = K;
         L.Bx = std::min(2 * K + 1, N);
+        L.ns = N;
         b->legs.push_back(L);
     }
     // 3. group descriptors
@@ -1500,25 +1558,25 @@ extern "C" int ps_batch_profile(ps_batch *b, int reps, double *stage_ms, double 
     for (int r = 0; r < reps && st == PS_OK; ++r) {
         if (b->method == PS_METHOD_FOURIER) {
             for (SpecWave &w : b->waves) {
-                const int nl = (int)w.legs.size(), ng = (int)w.groups.size();
+                const int nl = (int)w.legs.size(), ng = (int)w.groups.size(), M = w.M;
                 if (nl > 0) {
                     timed(0, [&] {
-                        const int x = b->dscale ? run_leg_scale(fp, N, w.dlegs, nl, s) : PS_OK;
+                        const int x = b->dscale ? run_leg_scale(fp, M, w.dlegs, nl, s) : PS_OK;
                         return x != PS_OK ? x
-                                          : run_inverse_g(fp, N, w.dlegs, nl, 0, w.g1_tiles, 0, 0, b->grid_cap,
-                                                          b->g2_div, c->tw, s);
+                                          : run_inverse_g(fp, M, w.dlegs, nl, 0, w.g1_tiles, 0, 0, b->grid_cap,
+                                                          b->g2_div, w.tw, s);
                     });
                     timed(1, [&] {
-                        return run_inverse_g(fp, N, w.dlegs, nl, 0, 0, 0, w.g2_tiles, b->grid_cap, b->g2_div, c->tw, s);
+                        return run_inverse_g(fp, M, w.dlegs, nl, 0, 0, 0, w.g2_tiles, b->grid_cap, b->g2_div, w.tw, s);
                     });
                 }
                 timed(2, [&] {
-                    return run_spec_zf(fp, N, w.dgroups, ng, w.dlegs, 0, w.zf_tiles, b->grid_cap, w.kz_cap, c->tw, s);
+                    return run_spec_zf(fp, M, w.dgroups, ng, w.dlegs, 0, w.zf_tiles, b->grid_cap, w.kz_cap, w.tw, s);
                 });
-                timed(3, [&] { return run_spec_fx(fp, N, w.dgroups, ng, 0, w.fx_tiles, b->grid_cap, b->g2_div, c->tw, s); });
+                timed(3, [&] { return run_spec_fx(fp, M, w.dgroups, ng, 0, w.fx_tiles, b->grid_cap, b->g2_div, w.tw, s); });
                 timed(4, [&] {
-                    return run_spec_dot(fp, N, w.dgroups, ng, b->dmasks, b->divl, 0, w.dot_items, b->dpartials,
-                                        b->max_chunks, b->grid_cap, c->tw, s);
+                    return run_spec_dot(fp, M, w.dgroups, ng, b->dmasks, b->divl, 0, w.dot_items, b->dpartials,
+                                        b->max_chunks, b->grid_cap, w.tw, s);
                 });
             }
         } else {

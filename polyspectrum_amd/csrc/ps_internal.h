@@ -25,6 +25,7 @@ struct LegDev {
     int Kz;            // planes kz in [0, Kz] can be non-zero
     int Kx;            // |mx| <= Kx can be non-zero
     int Bx;            // band width min(2*Kx+1, N)
+    int ns;            // grid size of the spectrum (the transforms may run on a smaller grid)
     int64_t g1_begin;  // first tile of this leg in the G1 pass
     int64_t g2_begin;  // first tile of this leg in the G2 pass
     void *g1d, *g1i;   // band buffers [Kz+1][Bx][N] (D part may be null)
@@ -54,7 +55,9 @@ struct GroupDev {
 // Q = F[P] the forward transform of the prefix product (Parseval).
 //   fft = 1 (triangles): P = product of the prefix legs in real space, taken
 //     through G1, G2 and the Zf pass (inverse), then Zf, Fx and the dot pass
-//     (forward), kept on the band |mx|, |my| <= Kq, kz <= Kqz.
+//     (forward), kept on the band |mx|, |my| <= Kq, kz <= Kqz.  P is band
+//     limited to K_A + K_B, so a grid M > K_A + K_B + Kq carries these sums
+//     without aliasing; the passes run on the smallest supported such M.
 //   fft = 0 (power spectra): Q = N^3 m_A H read from the spectrum (has_A = 0:
 //     Q = N^3 H, the suffix leg is its own partner).
 struct SpecGroupDev {
@@ -67,6 +70,8 @@ struct SpecGroupDev {
     int has_A;               // fft = 0: Q carries the mask mA
     int phase_only;          // V = H / |H| on the selected cells
     int dot_lines;           // lines (D and I lines counted apart) per dot work item
+    int ns;                  // grid size of the spectrum; the passes run on the kernels' grid M <= ns
+    double fscale;           // (ns / M)^3 for fft groups (sum over the ns grid = (ns/M)^3 x the M grid's), else 1
     MaskParams mA;
     const void *spec;        // half spectrum of the group's field slot
     void *qd, *qi;           // Zf output planes [Kqz+1][N][N] (D, I)

@@ -129,11 +129,13 @@ __global__ void __launch_bounds__(NT) k_inv_g1(const LegDev *__restrict__ legs, 
             const int j = j0 + l;
             cplx<T> d = mk<T>(0, 0), u = mk<T>(0, 0);
             if (j < L.Bx) {
-                const int kx = band_to_k(j, L.Kx, L.Bx, N);
-                if (mask_eval(L.mask, lattice(kx, N), lattice(ky, N), mz)) {
+                const int mx = lattice(band_to_k(j, L.Kx, L.Bx, N), N), my = lattice(ky, N);
+                if (mask_eval(L.mask, mx, my, mz)) {
                     u.x = (T)1;
                     if (spec) {
-                        d = spec[((int64_t)kz * N + kx) * N + ky];
+                        // the cell's index on the spectrum's grid (ns >= N)
+                        const int ns = L.ns;
+                        d = spec[((int64_t)kz * ns + (mx < 0 ? mx + ns : mx)) * ns + (my < 0 ? my + ns : my)];
                         if (L.phase_only) {
                             // z / |z| (utils_fftw.c:78-93), applied to the cells a shell selects
                             const T a = hypot(d.x, d.y);
@@ -523,9 +525,10 @@ __global__ void __launch_bounds__(NT) k_leg_scale(const LegDev *__restrict__ leg
         for (int64_t c = (int64_t)tid * step; c < total; c += (int64_t)NT * step) {
             const int kz = (int)(c / ((int64_t)B * B));
             const int jx = (int)((c / B) % B), jy = (int)(c % B);
-            const int kx = band_to_k(jx, L.Kx, B, N), ky = band_to_k(jy, L.Kx, B, N);
-            if (mask_eval(L.mask, lattice(kx, N), lattice(ky, N), lattice(kz, N))) {
-                const cplx<T> d = spec[((int64_t)kz * N + kx) * N + ky];
+            const int mx = lattice(band_to_k(jx, L.Kx, B, N), N), my = lattice(band_to_k(jy, L.Kx, B, N), N);
+            if (mask_eval(L.mask, mx, my, lattice(kz, N))) {
+                const int ns = L.ns;
+                const cplx<T> d = spec[((int64_t)kz * ns + (mx < 0 ? mx + ns : mx)) * ns + (my < 0 ? my + ns : my)];
                 pw += (double)d.x * (double)d.x + (double)d.y * (double)d.y;
                 cnt += 1.0;
             }

@@ -122,15 +122,31 @@ __global__ void __launch_bounds__((ZfCfg<T, N>::BT)) k_spec_zf(const SpecGroupDe
             }
         }
         __syncthreads();  // every wavefront has read the staged planes (qb aliases them)
-        // conj(Q) in natural z order; the forward transform is then
-        // F[Q] = conj(IFFT(conj Q)) with the inverse twiddles
+        // Lines to transform, conjugated and in natural z order (the forward
+        // transform is F[q] = conj(IFFT(conj q)) with the inverse twiddles).
+        // PAIRED (an even number of rows per wavefront): rows a, b = a + 1
+        // share lines as q_D = P_D(a) + i P_D(b) and q_I = P_I(a) + i P_I(b),
+        // so that only like parts share a transform (P_I = I_A I_B peaks at
+        // x = 0 and would dominate the rounding of P_D).  Otherwise one line
+        // per row: q = P_D + i P_I.
+        constexpr bool PAIRED = (LW % 2 == 0);
 #pragma unroll
         for (int i = 0; i < WO::IT; ++i) {
             const int b = lane + 64 * i;
             if (WO::TOT % 64 == 0 || b < WO::TOT) {
                 const int j = b % WO::NB, l = b / WO::NB;
 #pragma unroll
-                for (int r = 0; r < WO::R; ++r) qb[l * LP + wpad(j + r * WO::NB)] = mk<T>(pd[i][r], -pi[i][r]);
+                for (int r = 0; r < WO::R; ++r) {
+                    const int e = wpad(j + r * WO::NB);
+                    if constexpr (PAIRED) {
+                        T *qd_ = reinterpret_cast<T *>(&qb[(l & ~1) * LP + e]);
+                        T *qi_ = reinterpret_cast<T *>(&qb[(l | 1) * LP + e]);
+                        qd_[l & 1] = (l & 1) ? -pd[i][r] : pd[i][r];
+                        qi_[l & 1] = (l & 1) ? -pi[i][r] : pi[i][r];
+                    } else {
+                        qb[l * LP + e] = mk<T>(pd[i][r], -pi[i][r]);
+                    }
+                }
             }
         }
         wave_sync();
@@ -153,18 +169,29 @@ __global__ void __launch_bounds__((ZfCfg<T, N>::BT)) k_spec_zf(const SpecGroupDe
             }
         }
         __syncthreads();
-        // qb holds a = conj F[Q]; with A = F[Q](k), B = conj F[Q](N-k):
-        // F_z[P_D](k) = (A + B) / 2, F_z[P_I](k) = (A - B) / (2i)
+        // a line holds a = conj F[q]; with A = F[q](k), B = conj F[q](N-k), the
+        // real-part field of q has F_z = (A + B) / 2, the imaginary-part field
+        // F_z = (A - B) / (2i)
         cplx<T> *od = reinterpret_cast<cplx<T> *>(G.qd);
         cplx<T> *oi = reinterpret_cast<cplx<T> *>(G.qi);
         const T h = (T)0.5;
+        auto re_part = [&](cplx<T> a, cplx<T> c) { return mk<T>((a.x + c.x) * h, (c.y - a.y) * h); };
+        auto im_part = [&](cplx<T> a, cplx<T> c) { return mk<T>(-(a.y + c.y) * h, (c.x - a.x) * h); };
         for (int idx = tid; idx < (Kqz + 1) * ROWS; idx += BT) {
-            const int row = idx % ROWS, kz = idx / ROWS;
-            const cplx<T> *line = dyn + row * LP;  // row = wavefront * LW + line
-            const cplx<T> a = line[wpad(kz)], c = line[wpad((N - kz) % N)];
+            const int row = idx % ROWS, kz = idx / ROWS;  // row = wavefront * LW + line
+            const int kn = wpad(kz), km = wpad((N - kz) % N);
             const int64_t o = ((int64_t)kz * N + x) * N + y0 + row;
-            od[o] = mk<T>((a.x + c.x) * h, (c.y - a.y) * h);
-            oi[o] = mk<T>(-(a.y + c.y) * h, (c.x - a.x) * h);
+            if constexpr (PAIRED) {
+                const cplx<T> *ld = dyn + (row & ~1) * LP, *li = dyn + (row | 1) * LP;
+                const cplx<T> ad = ld[kn], cd = ld[km], ai = li[kn], ci = li[km];
+                od[o] = (row & 1) ? im_part(ad, cd) : re_part(ad, cd);
+                oi[o] = (row & 1) ? im_part(ai, ci) : re_part(ai, ci);
+            } else {
+                const cplx<T> *line = dyn + row * LP;
+                const cplx<T> a = line[kn], c = line[km];
+                od[o] = re_part(a, c);
+                oi[o] = im_part(a, c);
+            }
         }
     }
 }
@@ -243,10 +270,11 @@ __global__ void __launch_bounds__((DotCfg<T, N>::BT)) k_spec_dot(const SpecGroup
     twr.template load<+1>(twg, lane);
     const RegTw<T, N, LW> tw{twr};
     cplx<T> *wb = wbuf + w * LW * LP;
-    const double n3 = (double)N * (double)N * (double)N;
     for (int64_t item = i0 + blockIdx.x; item < i1; item += gridDim.x) {
         const int gi = find_owner(groups, ngroups, item, &SpecGroupDev::dot_begin);
         const SpecGroupDev &G = groups[gi];
+        const int ns = G.ns;
+        const double n3 = (double)ns * (double)ns * (double)ns;
         const int chunk = (int)(item - G.dot_begin);
         const int Bq = G.Bq, Kq = G.Kq;
         const int64_t nlines = 2 * (int64_t)(G.Kqz + 1) * Bq;
@@ -301,7 +329,7 @@ __global__ void __launch_bounds__((DotCfg<T, N>::BT)) k_spec_dot(const SpecGroup
                     if (inb) {
                         cplx<T> V = mk<T>(0, 0);
                         if (part == 0) {
-                            V = spec[((int64_t)kz * N + kx) * N + ky];
+                            V = spec[((int64_t)kz * ns + (mx < 0 ? mx + ns : mx)) * ns + (my < 0 ? my + ns : my)];
                             if (G.phase_only) {
                                 // z / |z| (utils_fftw.c:78-93) on the cells a shell selects
                                 const T a = hypot(V.x, V.y);
@@ -379,12 +407,14 @@ __global__ void __launch_bounds__((DotCfg<T, N>::BT)) k_spec_dot(const SpecGroup
                 c.add(acc[ww][o][3]);
             }
             // undo the prefix legs' amplitude scales (their D parts entered G1 divided by them)
-            double f = 1.0;
+            // and the grid ratio (ns/M)^3 of a transform on the reduced grid
+            const double g = G.fscale;
+            double f = g;
             if (G.scaleA) f *= G.pre_sq ? (*G.scaleA) * (*G.scaleA) : *G.scaleA;
             if (G.scaleB) f *= *G.scaleB;
             const int64_t po = (int64_t)(G.out_begin + o) * max_chunks + chunk;
             partials[2 * po + 0] = make_double2(a.s * f, a.c * f);
-            partials[2 * po + 1] = make_double2(c.s, c.c);
+            partials[2 * po + 1] = make_double2(c.s * g, c.c * g);
         }
         __syncthreads();
     }
