@@ -8,9 +8,12 @@ Mpc/h; 4 pairs k1 = k2 in {0.1, 0.4, 0.7, 1.0} h/Mpc x 32 theta bins
 cosThetaBinwidth = 0.05 (run.c:68-81), kbinwidth = 1.
 
 A step is one evaluation of all 128 triangles (B and Npolygons) by the batched
-mixed-precision path (FP32 FFTs, FP64 compensated sums), replayed from its
-captured hipGraph, with the field's spectrum already resident in HBM.  Beside
-it the script measures:
+mixed-precision path (FP32 FFTs, FP64 compensated sums; the autotuned method,
+normally the Fourier-space one: per k1 = k2 pair the squared leg is inverse
+transformed, squared and forward transformed once on an alias-free reduced
+grid, and each triangle is a sum over its k3 shell), replayed from its captured
+hipGraph, with the field's spectrum already resident in HBM.  Beside it the
+script measures:
   * the reference-shaped FP64 path (ps_polyspectrum, one triangle per call)
     on a sample of triangles, and the batched FP64 path on all 128 (the
     accuracy reference for the mixed path);
@@ -231,23 +234,21 @@ def main():
     ms_per_step = 1e3 * elapsed / args.steps
     value = ntri * args.steps / elapsed
 
-    # ---- per-stage HIP-event times (dominant kernel) and the forward FFT
+    # ---- per-pass HIP-event times (dominant pass) and the forward FFT
     used_params = batch.params
-    stage_ms, stage_bytes = batch.profile(reps=3)
-    names = ["G1 masked gather + ky inverse", "G2 kx inverse (band -> all x)", "Zc fused C2R + product-reduce",
-             "final reduce"]
+    stage_ms, stage_bytes, names = batch.profile(reps=3)
     dom = int(np.argmax(stage_ms))
-    kernel_of_stage = ["ps::k_inv_g1<float, 512>", "ps::k_inv_g2<float, 512, 1>", "ps::k_inv_zc_fused<float, 512>",
-                       "ps::k_reduce_partials"]
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "bispectrum512_hbm_traffic.json")
     if N == 512 and os.path.exists(tfile):
-        # HBM bytes per launch from the committed FETCH_SIZE/WRITE_SIZE summary
-        t = json.load(open(tfile))["kernel_traffic_GB"].get(kernel_of_stage[dom])
+        # HBM bytes per step of the dominant pass (all its launches) from the
+        # committed FETCH_SIZE/WRITE_SIZE summary
+        t = json.load(open(tfile)).get("pass_traffic_GB", {}).get(names[dom])
         traffic = None if t is None else round(t * 1e9)
     fwd_ms, fwd_bytes = ctx.forward_profile(reps=3)
     stages = {names[i]: {"ms": round(float(stage_ms[i]), 4), "GB": round(float(stage_bytes[i]) / 1e9, 4),
-                         "GB/s": round(float(stage_bytes[i] / stage_ms[i] / 1e6), 1)} for i in range(4)}
+                         "GB/s": round(float(stage_bytes[i] / stage_ms[i] / 1e6), 1) if stage_ms[i] > 0 else None}
+              for i in range(len(names))}
     batch.close()
     ctx.close()
 
@@ -290,8 +291,12 @@ def main():
                      "achieved": round(float(stage_bytes[dom] / stage_ms[dom] / 1e6), 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(float(stage_bytes[dom] / stage_ms[dom] / 1e6 / HBM_PEAK_GBS), 4),
-                     "traffic": traffic, "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE+WRITE_SIZE, "
-                                                         "copy-kernel scaled; profiles/bispectrum512_hbm_traffic.md)"},
+                     "traffic": traffic, "traffic_unit": "bytes per step over the pass's launches (rocprofv3 "
+                                                         "FETCH_SIZE+WRITE_SIZE, copy-kernel scaled; "
+                                                         "profiles/bispectrum512_hbm_traffic.md)"},
+        "step_roofline": {"model_GB": round(float(np.sum(stage_bytes)) / 1e9, 4),
+                          "GB/s": round(float(np.sum(stage_bytes)) / (ms_per_step * 1e6), 1),
+                          "frac": round(float(np.sum(stage_bytes)) / (ms_per_step * 1e6) / HBM_PEAK_GBS, 4)},
         "stages": stages,
         "forward_fft": {"ms": round(fwd_ms, 4), "GB": round(fwd_bytes / 1e9, 4),
                         "GB/s": round(fwd_bytes / fwd_ms / 1e6, 1),

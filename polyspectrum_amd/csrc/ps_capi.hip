@@ -455,6 +455,7 @@ struct SpecWave {
     SpecGroupDev *dgroups = nullptr;
     int64_t g1_tiles = 0, g2_tiles = 0, zf_tiles = 0, fx_tiles = 0, dot_items = 0;
     int kz_cap = 1;                      // 1 + largest Kz of the wave's legs
+    int kinds = 0;                       // dot pass instantiations needed: 1 radial, 2 cube-corner masks
     size_t bytes = 0;
 };
 
@@ -561,7 +562,7 @@ static int batch_record_on(ps_batch *b, hipStream_t s, hipStream_t s2)
             PSCK(run_spec_zf(c->fp64, M, w.dgroups, ng, w.dlegs, 0, w.zf_tiles, b->grid_cap, w.kz_cap, w.tw, s));
             PSCK(run_spec_fx(c->fp64, M, w.dgroups, ng, 0, w.fx_tiles, b->grid_cap, b->g2_div, w.tw, s));
             PSCK(run_spec_dot(c->fp64, M, w.dgroups, ng, b->dmasks, b->divl, 0, w.dot_items, b->dpartials,
-                              b->max_chunks, b->grid_cap, w.tw, s));
+                              b->max_chunks, b->grid_cap, w.kinds, w.tw, s));
         }
         PSCK(run_reduce_partials(b->dpartials, b->max_chunks, b->dcounts, 0, b->nout, b->dsum, b->dsum + b->nout, s));
         return PS_OK;
@@ -631,7 +632,7 @@ static ps_launch_params default_params(const ps_ctx *c)
 {
     ps_launch_params lp;
     lp.grid_cap = 8 * c->cus;
-    lp.chunk_tiles = 64;
+    lp.chunk_tiles = 1;  // dot steps per work item (the direct method's Zc runs best near 64 tiles)
     lp.g2_div = 1;
     lp.overlap = 0;
     lp.method = PS_METHOD_FOURIER;
@@ -745,6 +746,7 @@ static int plan_fourier(ps_batch *b, std::vector<int> &counts)
         sm[s] = b->legs[b->suffix[s]].mask;
         skmax[s] = b->legs[b->suffix[s]].Kz;
     }
+    std::vector<char> radial_ok(ns, 0);
     PCK(hipMalloc(&b->dmasks, sizeof(MaskParams) * ns));
     PCK(hipMalloc(&b->divl, sizeof(int2) * 2 * ns));
     {
@@ -769,6 +771,7 @@ static int plan_fourier(ps_batch *b, std::vector<int> &counts)
             }
         }
         if (patched) PCK(hipMemcpy(b->divl, hv.data(), sizeof(int2) * 2 * ns, hipMemcpyHostToDevice));
+        for (size_t s = 0; s < ns; ++s) radial_ok[s] = mask_is_radial(sm[s].kind) && hv[2 * s].x >= 0;
     }
     // per group: its band and the transform grid M (smallest supported size
     // above K_A + K_B + Kq, where the product is alias-free; N at most)
@@ -814,6 +817,8 @@ static int plan_fourier(ps_batch *b, std::vector<int> &counts)
         S.phase_only = L0.phase_only;
         S.ns = N;
         S.fscale = S.fft ? pow((double)N / (double)M, 3) : 1.0;
+        S.radial = 1;
+        for (int o = 0; o < G.nsuf; ++o) S.radial &= radial_ok[G.suf_begin + o] ? 1 : 0;
         sg.push_back(S);
         sgM.push_back(M);
     }
@@ -904,6 +909,7 @@ static int plan_fourier(ps_batch *b, std::vector<int> &counts)
             b->stage_bytes[1] += (double)cs * (2.0 * n1 + 2.0 * n2);
         }
         for (SpecGroupDev &S : w.groups) {
+            w.kinds |= S.radial ? 1 : 2;
             S.dot_lines = dot_lines;
             S.zf_begin = w.zf_tiles;
             S.fx_begin = w.fx_tiles;
@@ -1454,7 +1460,9 @@ extern "C" int ps_batch_autotune(ps_ctx *c, ps_polygon *polys, int npoly, double
 {
     if (!c || !polys || npoly < 1 || reps < 1) return PS_ERR_INVALID;
     const int caps[] = {2 * c->cus, 4 * c->cus, 8 * c->cus, 16 * c->cus};
-    const int chunks[] = {16, 64, 256};
+    // work-item sizes: Zc tiles (direct) or dot steps (Fourier) per item
+    const int chunks_direct[] = {16, 64, 256};
+    const int chunks_fourier[] = {1, 2, 4};
     const int divs[] = {1, 2};
     std::vector<ps_polygon> work(polys, polys + npoly);
     std::vector<double> v(npoly);
@@ -1480,7 +1488,10 @@ extern "C" int ps_batch_autotune(ps_ctx *c, ps_polygon *polys, int npoly, double
     double bdms = 1e300;
     for (int method : {PS_METHOD_FOURIER, PS_METHOD_DIRECT})
         for (int cap : caps)
-            for (int ch : chunks)
+            for (int ch : (method == PS_METHOD_FOURIER ? std::vector<int>(std::begin(chunks_fourier),
+                                                                       std::end(chunks_fourier))
+                                                       : std::vector<int>(std::begin(chunks_direct),
+                                                                          std::end(chunks_direct))))
                 for (int dv : divs) {
                     const ps_launch_params lp{cap, ch, dv, 0, method};
                     double t = 0.0;
@@ -1576,7 +1587,7 @@ extern "C" int ps_batch_profile(ps_batch *b, int reps, double *stage_ms, double 
                 timed(3, [&] { return run_spec_fx(fp, M, w.dgroups, ng, 0, w.fx_tiles, b->grid_cap, b->g2_div, w.tw, s); });
                 timed(4, [&] {
                     return run_spec_dot(fp, M, w.dgroups, ng, b->dmasks, b->divl, 0, w.dot_items, b->dpartials,
-                                        b->max_chunks, b->grid_cap, w.tw, s);
+                                        b->max_chunks, b->grid_cap, w.kinds, w.tw, s);
                 });
             }
         } else {

@@ -70,6 +70,7 @@ struct SpecGroupDev {
     int has_A;               // fft = 0: Q carries the mask mA
     int phase_only;          // V = H / |H| on the selected cells
     int dot_lines;           // lines (D and I lines counted apart) per dot work item
+    int radial;              // every suffix mask is one m^2 interval (per-line |my| ranges)
     int ns;                  // grid size of the spectrum; the passes run on the kernels' grid M <= ns
     double fscale;           // (ns / M)^3 for fft groups (sum over the ns grid = (ns/M)^3 x the M grid's), else 1
     MaskParams mA;
@@ -120,8 +121,9 @@ int run_spec_zf(int fp64, int N, const SpecGroupDev *groups, int ngroups, const 
                 int grid_cap, int kz_cap, const void *tw, hipStream_t st);
 int run_spec_fx(int fp64, int N, const SpecGroupDev *groups, int ngroups, int64_t t0, int64_t t1, int grid_cap,
                 int div, const void *tw, hipStream_t st);
+// kinds: bit 0 = the wave has radial groups, bit 1 = groups with cube-corner masks
 int run_spec_dot(int fp64, int N, const SpecGroupDev *groups, int ngroups, const MaskParams *masks, const int2 *ivl,
-                 int64_t i0, int64_t i1, double2 *partials, int max_chunks, int grid_cap, const void *tw,
+                 int64_t i0, int64_t i1, double2 *partials, int max_chunks, int grid_cap, int kinds, const void *tw,
                  hipStream_t st);
 int run_shell_intervals(const MaskParams *masks, const int *kmax, int n, int2 *out, hipStream_t st);
 

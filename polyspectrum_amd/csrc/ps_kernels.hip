@@ -505,8 +505,8 @@ __global__ void __launch_bounds__((ZcCfg<T, N>::BT)) k_inv_zc_fused(const GroupD
     }
 }
 
-// Amplitude scale of each leg: the rms of |spectrum| over (a sample of at
-// most ~16k) cells of its shell.  Packing D + i I into one complex transform
+// Amplitude scale of each leg: the rms of |spectrum| over (a sample of about
+// 4k) cells of its shell.  Packing D + i I into one complex transform
 // in the last pass gives both parts the rounding noise of the larger one;
 // dividing the D part by this scale keeps both near unit amplitude.
 template <typename T, int N>
@@ -519,12 +519,13 @@ __global__ void __launch_bounds__(NT) k_leg_scale(const LegDev *__restrict__ leg
     const cplx<T> *spec = reinterpret_cast<const cplx<T> *>(L.spec);
     double pw = 0.0, cnt = 0.0;
     if (spec && !L.phase_only) {
-        const int B = L.Bx;
-        const int64_t total = (int64_t)(L.Kz + 1) * B * B;
-        const int64_t step = (total > 16384) ? total / 16384 : 1;
-        for (int64_t c = (int64_t)tid * step; c < total; c += (int64_t)NT * step) {
-            const int kz = (int)(c / ((int64_t)B * B));
-            const int jx = (int)((c / B) % B), jy = (int)(c % B);
+        // a sample of about 4096 cells of the band box (< 2^31 cells)
+        const int B = L.Bx, BB = B * B;
+        const int total = (L.Kz + 1) * BB;
+        const int step = (total > 4096) ? total / 4096 : 1;
+        for (int c = tid * step; c < total; c += NT * step) {
+            const int kz = c / BB, rem = c - kz * BB;
+            const int jx = rem / B, jy = rem - jx * B;
             const int mx = lattice(band_to_k(jx, L.Kx, B, N), N), my = lattice(band_to_k(jy, L.Kx, B, N), N);
             if (mask_eval(L.mask, mx, my, lattice(kz, N))) {
                 const int ns = L.ns;

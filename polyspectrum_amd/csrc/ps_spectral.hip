@@ -245,11 +245,16 @@ template <typename T, int N> struct DotCfg {
 // Work item = (group, chunk of dot_lines lines).  Line 2p + part of a group is
 // the y line of band cell p = kz * Bq + jx, part 0 = D, 1 = I.  Each
 // wavefront transforms LW lines (fft groups) or reads Q from the spectrum
-// (power spectra); every thread then holds cells (line, ky) and, for each
-// suffix leg, adds w Re(conj Q V) (D) or w Re Q (I) over the cells of its
-// shell; the sums are tree-summed over the wavefront and accumulated with
-// compensation per wavefront.  Partial sums per (output, chunk).
-template <typename T, int N>
+// (power spectra); every thread then holds cells (line, ky) with their
+// contribution w Re(conj Q V) (D) or w Re Q (I).
+//   radial groups: along a line m^2 = mx^2 + mz^2 + my^2 grows with |my|, so
+//     a shell [lo, hi] of m^2 is one range a <= |my| <= b of the line.  The
+//     contributions go to LDS; lane (t, part) adds, line by line, the range
+//     of suffix t and keeps a compensated sum in registers for the item.
+//   other masks (cube-corner tests): per suffix, every cell tested, the sums
+//     tree-summed over the wavefront and accumulated per wavefront.
+// Partial sums per (output, chunk).
+template <typename T, int N, bool RADIAL>
 __global__ void __launch_bounds__((DotCfg<T, N>::BT)) k_spec_dot(const SpecGroupDev *__restrict__ groups, int ngroups,
                                                                  const MaskParams *__restrict__ masks,
                                                                  const int2 *__restrict__ ivl, int64_t i0, int64_t i1,
@@ -258,6 +263,7 @@ __global__ void __launch_bounds__((DotCfg<T, N>::BT)) k_spec_dot(const SpecGroup
 {
     using C = DotCfg<T, N>;
     constexpr int LW = C::LW, NWV = C::NWV, BT = C::BT, LINES = C::LINES;
+    static_assert(MAX_SUFFIX <= 32, "lanes (t, part) of a wavefront hold the radial sums");
     constexpr int LP = wave_pitch<N>();
     using WO = WaveOut<T, N, LW>;
     constexpr int IT = WO::IT, R = WO::R, NB = WO::NB;
@@ -273,10 +279,10 @@ __global__ void __launch_bounds__((DotCfg<T, N>::BT)) k_spec_dot(const SpecGroup
     for (int64_t item = i0 + blockIdx.x; item < i1; item += gridDim.x) {
         const int gi = find_owner(groups, ngroups, item, &SpecGroupDev::dot_begin);
         const SpecGroupDev &G = groups[gi];
-        const int ns = G.ns;
-        const double n3 = (double)ns * (double)ns * (double)ns;
+        if ((G.radial != 0) != RADIAL) continue;  // the other instantiation's item
         const int chunk = (int)(item - G.dot_begin);
-        const int Bq = G.Bq, Kq = G.Kq;
+        const int Bq = G.Bq, Kq = G.Kq, ns = G.ns;
+        const double n3 = (double)ns * (double)ns * (double)ns;
         const int64_t nlines = 2 * (int64_t)(G.Kqz + 1) * Bq;
         const int64_t l0 = (int64_t)chunk * G.dot_lines;
         const int64_t l1 = (l0 + G.dot_lines < nlines) ? l0 + G.dot_lines : nlines;
@@ -287,6 +293,8 @@ __global__ void __launch_bounds__((DotCfg<T, N>::BT)) k_spec_dot(const SpecGroup
         }
         __syncthreads();
         const cplx<T> *spec = reinterpret_cast<const cplx<T> *>(G.spec);
+        KSum ks;  // radial: lane (t, part)'s sum over the item
+        ks.init();
         for (int64_t base = l0; base < l1; base += LINES) {
             const int64_t wl0 = base + w * LW;  // this wavefront's first line
             cplx<T> qo[IT][R];
@@ -302,97 +310,145 @@ __global__ void __launch_bounds__((DotCfg<T, N>::BT)) k_spec_dot(const SpecGroup
                 auto sink = [&](int i, int r, int, int, cplx<T> v) { qo[i][r] = mk<T>(v.x, -v.y); };
                 wave_fft<T, N, LW, +1, 0, false, RegTw<T, N, LW>>(wb, tw, lane, src, sink);
             }
-            // per cell: contribution, m^2 (-1: no cell) and packed (mx, my, mz)
-            double cv[IT][R];
-            int m2v[IT][R], crd[IT][R], prt[IT];
-#pragma unroll
-            for (int i = 0; i < IT; ++i) {
-                const int b = lane + 64 * i;
-                const int j = b % NB, l = b / NB;
+            // contribution of cell (line, ky): w Re(conj Q V) (D) or w Re Q (I)
+            auto contribution = [&](int i, int r, int l, int ky, int &m2, int &mx_, int &my_, int &mz_) -> double {
                 const int64_t line = wl0 + l;
-                const bool okl = (WO::TOT % 64 == 0 || b < WO::TOT) && line < l1;
+                m2 = -1;
+                if (line >= l1) return 0.0;
                 const int part = (int)(line & 1);
-                prt[i] = part;
                 const int64_t pair = line >> 1;
-                const int kz = (int)(pair / Bq), jx = (int)(pair % Bq);
-                const int kx = band_to_k(jx, Kq, Bq, N);
-                const int mx = lattice(kx, N), mz = lattice(kz, N);
+                const int kz = (int)(pair / Bq), jx = (int)(pair - (int64_t)kz * Bq);
+                if (k_to_band(ky, Kq, Bq, N) < 0) return 0.0;
+                const int mx = lattice(band_to_k(jx, Kq, Bq, N), N), my = lattice(ky, N), mz = lattice(kz, N);
+                m2 = mx * mx + my * my + mz * mz;
+                mx_ = mx;
+                my_ = my;
+                mz_ = mz;
                 const double wz = (kz == 0 || 2 * kz == N) ? 1.0 : 2.0;
-#pragma unroll
-                for (int r = 0; r < R; ++r) {
-                    const int ky = j + r * NB;
-                    const bool inb = okl && k_to_band(ky, Kq, Bq, N) >= 0;
-                    const int my = lattice(ky, N);
-                    m2v[i][r] = inb ? mx * mx + my * my + mz * mz : -1;
-                    crd[i][r] = ((mx + 512) << 20) | ((my + 512) << 10) | (mz + 512);
-                    double c = 0.0;
-                    if (inb) {
-                        cplx<T> V = mk<T>(0, 0);
-                        if (part == 0) {
-                            V = spec[((int64_t)kz * ns + (mx < 0 ? mx + ns : mx)) * ns + (my < 0 ? my + ns : my)];
-                            if (G.phase_only) {
-                                // z / |z| (utils_fftw.c:78-93) on the cells a shell selects
-                                const T a = hypot(V.x, V.y);
-                                V = mk<T>(V.x / a, V.y / a);
-                            }
-                        }
-                        if (G.fft) {
-                            const cplx<T> Q = qo[i][r];
-                            c = part ? wz * (double)Q.x
-                                     : wz * ((double)Q.x * (double)V.x + (double)Q.y * (double)V.y);
-                        } else {
-                            const double a = (!G.has_A || mask_eval(G.mA, mx, my, mz)) ? n3 : 0.0;
-                            c = part ? wz * a : wz * a * ((double)V.x * (double)V.x + (double)V.y * (double)V.y);
-                        }
+                cplx<T> V = mk<T>(0, 0);
+                if (part == 0) {
+                    V = spec[((int64_t)kz * ns + (mx < 0 ? mx + ns : mx)) * ns + (my < 0 ? my + ns : my)];
+                    if (G.phase_only) {
+                        // z / |z| (utils_fftw.c:78-93) on the cells a shell selects
+                        const T a = hypot(V.x, V.y);
+                        V = mk<T>(V.x / a, V.y / a);
                     }
-                    cv[i][r] = c;
                 }
-            }
-            for (int o = 0; o < G.nsuf; ++o) {
-                const int2 iv = sivl[o];
-                double sd = 0.0, si = 0.0;
-                if (mask_is_radial(smask[o].kind)) {
+                if (G.fft) {
+                    const cplx<T> Q = qo[i][r];
+                    return part ? wz * (double)Q.x : wz * ((double)Q.x * (double)V.x + (double)Q.y * (double)V.y);
+                }
+                const double a = (!G.has_A || mask_eval(G.mA, mx, my, mz)) ? n3 : 0.0;
+                return part ? wz * a : wz * a * ((double)V.x * (double)V.x + (double)V.y * (double)V.y);
+            };
+            if constexpr (RADIAL) {
+                // along a line m^2 = mx^2 + mz^2 + my^2 grows with |my|: shell t
+                // is the range a_t <= |my| <= b_t.  Contributions go to this
+                // wavefront's LDS region (free after the transform) as [LW][N]
+                // doubles; lane (t, part) sums its ranges line by line.
+                double *cb = reinterpret_cast<double *>(wb);
+                wave_sync();
 #pragma unroll
-                    for (int i = 0; i < IT; ++i) {
-                        double s = 0.0;
+                for (int i = 0; i < IT; ++i) {
+                    const int b = lane + 64 * i;
+                    if (WO::TOT % 64 == 0 || b < WO::TOT) {
+                        const int j = b % NB, l = b / NB;
 #pragma unroll
-                        for (int r = 0; r < R; ++r) s += (m2v[i][r] >= iv.x && m2v[i][r] <= iv.y) ? cv[i][r] : 0.0;
-                        if (prt[i]) si += s;
-                        else sd += s;
+                        for (int r = 0; r < R; ++r) {
+                            int m2, mx, my, mz;
+                            cb[l * N + j + r * NB] = contribution(i, r, l, j + r * NB, m2, mx, my, mz);
+                        }
                     }
-                } else {
+                }
+                wave_sync();
+                const int t = lane & 31, lh = lane >> 5;
+                if (t < G.nsuf) {
+                    const int2 iv = sivl[t];
+                    for (int l = 0; l < LW; ++l) {
+                        const int64_t line = wl0 + l;
+                        if (line >= l1 || (int)(line & 1) != lh) continue;
+                        const int64_t pair = line >> 1;
+                        const int kz = (int)(pair / Bq), jx = (int)(pair - (int64_t)kz * Bq);
+                        const int mx = lattice(band_to_k(jx, Kq, Bq, N), N), mz = lattice(kz, N);
+                        const int bse = mx * mx + mz * mz;
+                        if (iv.y < bse) continue;
+                        // largest ub with bse + ub^2 <= hi, smallest ua with bse + ua^2 >= lo
+                        int ub = (int)sqrtf((float)(iv.y - bse));
+                        while (ub * ub > iv.y - bse) --ub;
+                        while ((ub + 1) * (ub + 1) <= iv.y - bse) ++ub;
+                        int ua = 0;
+                        if (iv.x > bse) {
+                            ua = (int)sqrtf((float)(iv.x - bse));
+                            while (ua * ua < iv.x - bse) ++ua;
+                            while (ua > 0 && (ua - 1) * (ua - 1) >= iv.x - bse) --ua;
+                        }
+                        if (ub > Kq) ub = Kq;
+                        const double *row = cb + l * N;
+                        double s = 0.0;
+                        for (int u = ua; u <= ub; ++u) {
+                            s += row[u];
+                            if (u > 0 && 2 * u != N) s += row[N - u];  // my = -u
+                        }
+                        ks.add(s);
+                    }
+                }
+            } else {
+                // cube-corner masks: every cell tested for every suffix
+                double cv[IT][R];
+                int crd[IT][R], prt[IT];
+#pragma unroll
+                for (int i = 0; i < IT; ++i) {
+                    const int b = lane + 64 * i;
+                    const int j = b % NB, l = b / NB;
+                    prt[i] = (int)((wl0 + l) & 1);
+#pragma unroll
+                    for (int r = 0; r < R; ++r) {
+                        int m2 = -1, mx = 0, my = 0, mz = 0;
+                        cv[i][r] = (WO::TOT % 64 == 0 || b < WO::TOT)
+                                       ? contribution(i, r, l, j + r * NB, m2, mx, my, mz) : 0.0;
+                        crd[i][r] = (m2 < 0) ? -1 : (((mx + 512) << 20) | ((my + 512) << 10) | (mz + 512));
+                    }
+                }
+                for (int o = 0; o < G.nsuf; ++o) {
                     const MaskParams mp = smask[o];
+                    double sd = 0.0, si = 0.0;
 #pragma unroll
                     for (int i = 0; i < IT; ++i) {
                         double s = 0.0;
 #pragma unroll
                         for (int r = 0; r < R; ++r) {
                             const int cc = crd[i][r];
-                            const bool in = m2v[i][r] >= 0 &&
-                                            mask_eval(mp, ((cc >> 20) & 1023) - 512, ((cc >> 10) & 1023) - 512,
-                                                      (cc & 1023) - 512);
+                            const bool in = cc >= 0 && mask_eval(mp, ((cc >> 20) & 1023) - 512,
+                                                                 ((cc >> 10) & 1023) - 512, (cc & 1023) - 512);
                             s += in ? cv[i][r] : 0.0;
                         }
                         if (prt[i]) si += s;
                         else sd += s;
                     }
+                    sd = wave_sum(sd);
+                    si = wave_sum(si);
+                    if (lane == 0) {
+                        KSum a;
+                        a.s = acc[w][o][0];
+                        a.c = acc[w][o][1];
+                        a.add(sd);
+                        KSum c;
+                        c.s = acc[w][o][2];
+                        c.c = acc[w][o][3];
+                        c.add(si);
+                        acc[w][o][0] = a.s;
+                        acc[w][o][1] = a.c;
+                        acc[w][o][2] = c.s;
+                        acc[w][o][3] = c.c;
+                    }
                 }
-                sd = wave_sum(sd);
-                si = wave_sum(si);
-                if (lane == 0) {
-                    KSum a;
-                    a.s = acc[w][o][0];
-                    a.c = acc[w][o][1];
-                    a.add(sd);
-                    KSum c;
-                    c.s = acc[w][o][2];
-                    c.c = acc[w][o][3];
-                    c.add(si);
-                    acc[w][o][0] = a.s;
-                    acc[w][o][1] = a.c;
-                    acc[w][o][2] = c.s;
-                    acc[w][o][3] = c.c;
-                }
+            }
+        }
+        if constexpr (RADIAL) {
+            const int t = lane & 31, lh = lane >> 5;
+            if (t < G.nsuf) {
+                acc[w][t][2 * lh] = ks.s;
+                acc[w][t][2 * lh + 1] = ks.c;
             }
         }
         __syncthreads();
@@ -507,14 +563,19 @@ static int launch_spec_fx_n(const SpecGroupDev *g, int ng, int64_t t0, int64_t t
 
 template <typename T, int N>
 static int launch_spec_dot_n(const SpecGroupDev *g, int ng, const MaskParams *masks, const int2 *ivl, int64_t i0,
-                             int64_t i1, double2 *partials, int max_chunks, int grid_cap, const void *tw,
+                             int64_t i1, double2 *partials, int max_chunks, int grid_cap, int kinds, const void *tw,
                              hipStream_t st)
 {
     using C = DotCfg<T, N>;
     if (i1 <= i0) return 0;
     const unsigned grid = (unsigned)((i1 - i0 < grid_cap) ? i1 - i0 : grid_cap);
-    hipLaunchKernelGGL((k_spec_dot<T, N>), dim3(grid), dim3(C::BT), 0, st, g, ng, masks, ivl, i0, i1, partials,
-                       max_chunks, reinterpret_cast<const cplx<T> *>(tw));
+    const cplx<T> *twp = reinterpret_cast<const cplx<T> *>(tw);
+    if (kinds & 1)
+        hipLaunchKernelGGL((k_spec_dot<T, N, true>), dim3(grid), dim3(C::BT), 0, st, g, ng, masks, ivl, i0, i1,
+                           partials, max_chunks, twp);
+    if (kinds & 2)
+        hipLaunchKernelGGL((k_spec_dot<T, N, false>), dim3(grid), dim3(C::BT), 0, st, g, ng, masks, ivl, i0, i1,
+                           partials, max_chunks, twp);
     return (int)hipGetLastError();
 }
 
@@ -569,16 +630,16 @@ int run_spec_fx(int fp64, int N, const SpecGroupDev *groups, int ngroups, int64_
 }
 
 int run_spec_dot(int fp64, int N, const SpecGroupDev *groups, int ngroups, const MaskParams *masks, const int2 *ivl,
-                 int64_t i0, int64_t i1, double2 *partials, int max_chunks, int grid_cap, const void *tw,
+                 int64_t i0, int64_t i1, double2 *partials, int max_chunks, int grid_cap, int kinds, const void *tw,
                  hipStream_t st)
 {
     switch (N) {
 #define CASE(n)                                                                                                   \
     case n:                                                                                                       \
         return fp64 ? launch_spec_dot_n<double, n>(groups, ngroups, masks, ivl, i0, i1, partials, max_chunks,    \
-                                                   grid_cap, tw, st)                                              \
+                                                   grid_cap, kinds, tw, st)                                       \
                     : launch_spec_dot_n<float, n>(groups, ngroups, masks, ivl, i0, i1, partials, max_chunks,     \
-                                                  grid_cap, tw, st);
+                                                  grid_cap, kinds, tw, st);
         PS_FOR_EACH_N(CASE)
 #undef CASE
     }
