@@ -63,6 +63,7 @@ struct ps_ctx {
     hipStream_t st2 = nullptr;           // second stream: fused passes overlap the transforms
     void *tw = nullptr;                  // forward twiddles exp(-2 pi i t/N), N entries
     std::map<int, void *> twm;           // twiddle tables of smaller transform grids
+    std::vector<hipStream_t> br;         // branch streams: independent waves of a batch side by side
     std::vector<void *> spec;            // half spectra per slot
     std::vector<int> loaded;
     void *staging = nullptr;             // field upload buffer (N^3 doubles)
@@ -127,6 +128,10 @@ extern "C" int ps_ctx_destroy(ps_ctx *c)
     void *bufs[] = {c->tw, c->staging, c->gd, c->gi, c->accD, c->accI, c->partials, c->dsum, c->dleg, c->dcorner};
     for (void *p : bufs) if (p) (void)hipFree(p);
     if (c->st2) (void)hipStreamSynchronize(c->st2);
+    for (hipStream_t b : c->br) {
+        (void)hipStreamSynchronize(b);
+        (void)hipStreamDestroy(b);
+    }
     if (c->st) (void)hipStreamDestroy(c->st);
     if (c->st2) (void)hipStreamDestroy(c->st2);
     delete c;
@@ -491,6 +496,9 @@ struct ps_batch {
     MaskParams *dmasks = nullptr;        // suffix masks in suffix order (Fourier method)
     int2 *divl = nullptr;                // their m^2 intervals and counts
     void *arena = nullptr;
+    int concurrent = 0;                  // waves have disjoint buffers and run on branch streams
+    hipEvent_t fork = nullptr;
+    std::vector<hipEvent_t> wend;        // per branch stream: its waves are done
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -522,6 +530,8 @@ extern "C" int ps_batch_destroy(ps_batch *b)
     if (b->ev1) (void)hipEventDestroy(b->ev1);
     for (hipEvent_t e : b->gev) (void)hipEventDestroy(e);
     if (b->zev) (void)hipEventDestroy(b->zev);
+    for (hipEvent_t e : b->wend) (void)hipEventDestroy(e);
+    if (b->fork) (void)hipEventDestroy(b->fork);
     for (void *p : b->allocs) (void)hipFree(p);
     for (SpecWave &w : b->waves) {
         if (w.dlegs) (void)hipFree(w.dlegs);
@@ -552,17 +562,31 @@ static int batch_record_on(ps_batch *b, hipStream_t s, hipStream_t s2)
     ps_ctx *c = b->c;
     const int N = c->N;
     if (b->method == PS_METHOD_FOURIER) {
-        for (SpecWave &w : b->waves) {
+        auto wave = [&](SpecWave &w, hipStream_t ws) -> int {
             const int nl = (int)w.legs.size(), ng = (int)w.groups.size(), M = w.M;
             if (nl > 0) {
-                if (b->dscale) PSCK(run_leg_scale(c->fp64, M, w.dlegs, nl, s));
+                if (b->dscale) PSCK(run_leg_scale(c->fp64, M, w.dlegs, nl, ws));
                 PSCK(run_inverse_g(c->fp64, M, w.dlegs, nl, 0, w.g1_tiles, 0, w.g2_tiles, b->grid_cap, b->g2_div,
-                                   w.tw, s));
+                                   w.tw, ws));
             }
-            PSCK(run_spec_zf(c->fp64, M, w.dgroups, ng, w.dlegs, 0, w.zf_tiles, b->grid_cap, w.kz_cap, w.tw, s));
-            PSCK(run_spec_fx(c->fp64, M, w.dgroups, ng, 0, w.fx_tiles, b->grid_cap, b->g2_div, w.tw, s));
+            PSCK(run_spec_zf(c->fp64, M, w.dgroups, ng, w.dlegs, 0, w.zf_tiles, b->grid_cap, w.kz_cap, w.tw, ws));
+            PSCK(run_spec_fx(c->fp64, M, w.dgroups, ng, 0, w.fx_tiles, b->grid_cap, b->g2_div, w.tw, ws));
             PSCK(run_spec_dot(c->fp64, M, w.dgroups, ng, b->dmasks, b->divl, 0, w.dot_items, b->dpartials,
-                              b->max_chunks, b->grid_cap, w.kinds, w.tw, s));
+                              b->max_chunks, b->grid_cap, w.kinds, w.tw, ws));
+            return PS_OK;
+        };
+        const int nw = (int)b->waves.size(), nb = (int)b->wend.size();
+        if (b->concurrent && nb > 1) {
+            // fork: wave i on branch stream i mod nb; join before the reduction
+            HIPCK(hipEventRecord(b->fork, s));
+            for (int k = 0; k < nb; ++k) HIPCK(hipStreamWaitEvent(c->br[k], b->fork, 0));
+            for (int i = 0; i < nw; ++i) PSCK(wave(b->waves[i], c->br[i % nb]));
+            for (int k = 0; k < nb; ++k) {
+                HIPCK(hipEventRecord(b->wend[k], c->br[k]));
+                HIPCK(hipStreamWaitEvent(s, b->wend[k], 0));
+            }
+        } else {
+            for (SpecWave &w : b->waves) PSCK(wave(w, s));
         }
         PSCK(run_reduce_partials(b->dpartials, b->max_chunks, b->dcounts, 0, b->nout, b->dsum, b->dsum + b->nout, s));
         return PS_OK;
@@ -879,7 +903,7 @@ static int plan_fourier(ps_batch *b, std::vector<int> &counts)
         }
     }
     // tiles, work items and model bytes
-    size_t arena = 0, nscale = 0;
+    size_t arena = 0, nscale = 0, total_bytes = 0;
     for (SpecWave &w : W) {
         const int M = w.M;
         const int tl = g_tile_lines(M, fp64);
@@ -932,8 +956,12 @@ static int plan_fourier(ps_batch *b, std::vector<int> &counts)
             b->max_chunks = std::max(b->max_chunks, items);
         }
         arena = std::max(arena, w.bytes);
+        total_bytes += w.bytes;
         nscale += w.legs.size();
     }
+    // waves side by side (disjoint buffers, branch streams) when they all fit
+    b->concurrent = (W.size() > 1 && total_bytes <= budget) ? 1 : 0;
+    if (b->concurrent) arena = total_bytes;
     b->nstages = 6;
     b->stage_bytes[5] = 32.0 * (double)b->nout * b->max_chunks;
     b->stage_bytes[4] += b->stage_bytes[5];  // the dot pass writes the partial sums
@@ -946,6 +974,7 @@ static int plan_fourier(ps_batch *b, std::vector<int> &counts)
     char *base = (char *)b->arena;
     for (SpecWave &w : W) {
         const int M = w.M;
+        if (b->concurrent && &w != &W.front()) base += (&w - 1)->bytes;  // disjoint regions
         for (size_t i = 0; i < w.legs.size(); ++i) {
             LegDev &L = w.legs[i];
             const size_t n1 = (size_t)(L.Kz + 1) * L.Bx * M, n2 = (size_t)(L.Kz + 1) * M * M;
@@ -975,6 +1004,17 @@ static int plan_fourier(ps_batch *b, std::vector<int> &counts)
         }
         PCK(hipMalloc(&w.dgroups, sizeof(SpecGroupDev) * w.groups.size()));
         PCK(hipMemcpy(w.dgroups, w.groups.data(), sizeof(SpecGroupDev) * w.groups.size(), hipMemcpyHostToDevice));
+    }
+    if (b->concurrent) {
+        const int nb = std::min((int)W.size(), 4);
+        while ((int)c->br.size() < nb) {
+            hipStream_t t = nullptr;
+            PCK(hipStreamCreateWithFlags(&t, hipStreamNonBlocking));  // default priority
+            c->br.push_back(t);
+        }
+        PCK(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+        b->wend.assign(nb, nullptr);
+        for (auto &e : b->wend) PCK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     }
     return PS_OK;
 }
