@@ -50,13 +50,17 @@ extern "C" {
 #define PS_BINNING_GIVEN 2
 #define PS_BINNING_WATKINSON 3
 
-/* Evaluation of a batch (ps_launch_params::method).  Both give the
- * reference's sum over the grid of the product of the filtered fields:
+/* Evaluation of a batch (ps_set_method, ps_launch_params::method).  Both
+ * give the reference's sum over the grid of the product of the filtered
+ * fields, each group of polygons on the smallest grid on which that product
+ * does not alias (the sum over the spectrum's grid is (N/M)^3 times the sum
+ * over such a grid M):
+ *   DIRECT (default): every leg inverse transformed, the pointwise product
+ *            of the filtered fields and its sum, as the reference does.
  *   FOURIER: per group of polygons sharing their first legs, the product of
  *            those legs is forward transformed once and each last leg is a
  *            sum over its shell in Fourier space (Parseval); power spectra
- *            need no transform.
- *   DIRECT:  every leg inverse transformed, products and sums in real space. */
+ *            need no transform.  Same sums in another order. */
 #define PS_METHOD_FOURIER 0
 #define PS_METHOD_DIRECT 1
 #define PS_MAX_STAGES 6 /* passes ps_batch_profile reports at most */
@@ -107,6 +111,12 @@ int ps_load_field(ps_ctx *ctx, int slot, const void *field, int field_is_double,
  * normalisation is applied to the cells a shell selects, when they are read. */
 int ps_set_phase_only(ps_ctx *ctx, int phase_only);
 
+/* Method (PS_METHOD_DIRECT by default) of the batches created afterwards
+ * without explicit launch parameters, and of ps_batch_autotune.
+ * Replaces: nothing in the reference (it evaluates every polygon the DIRECT
+ * way, polyspectrum.c:51-161); the choice changes the order of the sums only. */
+int ps_set_method(ps_ctx *ctx, int method);
+
 /* One configuration, shaped like the reference: per leg a mask, an inverse
  * FFT of the masked spectrum and of the mask, products into N^3
  * accumulators, then the sums.  result = polyspectrum value, sums (optional,
@@ -136,10 +146,10 @@ int ps_batch_create(ps_ctx *ctx, ps_polygon *polys, int npoly, double kbinwidth,
 /* Launch shape of a batch: persistent grid size of the pass kernels, Zc tiles
  * (DIRECT) or dot-pass steps (FOURIER) per work item (sets how many
  * compensated partial sums each polygon has), the divisor of the G2/Fx tile
- * width (1: 16 FP32 / 8 FP64 columns, 2: half), the DIRECT schedule (overlap
- * 0: each pass over all legs in turn on one stream; 1: per polygon group, the
- * product pass of group g on a second stream beside the inverse transforms of
- * the legs group g+1 introduces) and the method (PS_METHOD_*). */
+ * width (1: 16 FP32 / 8 FP64 columns, 2: half), the schedule of the waves
+ * (groups sharing one transform grid; overlap 1: waves side by side on
+ * branch streams when their buffers fit together, 0: one after another,
+ * reusing one buffer region) and the method (PS_METHOD_*). */
 typedef struct ps_launch_params {
     int grid_cap;
     int chunk_tiles;

@@ -48,7 +48,7 @@ class LaunchParams(ctypes.Structure):
 
 
 METHOD_FOURIER = 0  # prefix product forward transformed once, shell sums per last leg
-METHOD_DIRECT = 1   # every leg inverse transformed, real-space products
+METHOD_DIRECT = 1   # every leg inverse transformed, real-space products (default)
 MAX_STAGES = 6
 
 
@@ -111,6 +111,7 @@ def load_library():
     L.ps_ctx_destroy.argtypes = [_vp]
     L.ps_load_field.argtypes = [_vp, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int]
     L.ps_set_phase_only.argtypes = [_vp, ctypes.c_int]
+    L.ps_set_method.argtypes = [_vp, ctypes.c_int]
     L.ps_polyspectrum.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, ctypes.c_double,
                                   ctypes.c_int, _dp, _dp]
     L.ps_num_polygons.argtypes = [_vp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_double, ctypes.c_int, _dp]
@@ -242,6 +243,16 @@ class Context:
     def set_phase_only(self, flag):
         _check(load_library().ps_set_phase_only(self._h, int(flag)))
 
+    def set_method(self, method):
+        """METHOD_DIRECT (default) or METHOD_FOURIER for the batches created
+        afterwards without explicit launch parameters, and for autotune()."""
+        _check(load_library().ps_set_method(self._h, int(method)))
+        self._method = int(method)
+
+    @property
+    def method(self):
+        return getattr(self, "_method", METHOD_DIRECT)
+
     def polyspectrum(self, n, k, kmin=None, kmax=None, kbinwidth=1.0, kbinning_case=0, slot=0):
         """polyspectrum.c:51-161 on the GPU.  Returns (value, kmin, kmax,
         (sum prod D, sum prod I)); kmin/kmax as updated for DERIVED binning."""
@@ -271,7 +282,7 @@ class Context:
 
     def autotune(self, polygons, kbinwidth=1.0, kbinning_case=0, reps=2):
         """Times candidate launch shapes; later batches of this shape use the best.
-        Returns ({grid_cap, chunk_tiles, g2_div, overlap}, ms)."""
+        Returns ({grid_cap, chunk_tiles, g2_div, overlap, method}, ms)."""
         arr = _polygon_array(polygons)
         lp = LaunchParams()
         ms = ctypes.c_double()
@@ -294,7 +305,7 @@ class Batch:
         lp = None
         if params is not None:
             lp = LaunchParams(int(params["grid_cap"]), int(params["chunk_tiles"]), int(params.get("g2_div", 1)),
-                              int(params.get("overlap", 0)), int(params.get("method", METHOD_FOURIER)))
+                              int(params.get("overlap", 1)), int(params.get("method", ctx.method)))
         _check(load_library().ps_batch_create_ex(ctx._h, arr, len(polygons), float(kbinwidth), int(kbinning_case),
                                                  ctypes.byref(lp) if lp is not None else None, ctypes.byref(h)))
         self._h = h

@@ -60,7 +60,6 @@ struct ps_ctx {
     int device = 0;
     int cus = 256;
     hipStream_t st = nullptr;
-    hipStream_t st2 = nullptr;           // second stream: fused passes overlap the transforms
     void *tw = nullptr;                  // forward twiddles exp(-2 pi i t/N), N entries
     std::map<int, void *> twm;           // twiddle tables of smaller transform grids
     std::vector<hipStream_t> br;         // branch streams: independent waves of a batch side by side
@@ -69,6 +68,7 @@ struct ps_ctx {
     void *staging = nullptr;             // field upload buffer (N^3 doubles)
     int staging_is_double = 0;
     int phase_only = 0;
+    int method = PS_METHOD_DIRECT;       // evaluation of batches created without explicit parameters
     // reference-shaped path buffers (lazily allocated)
     void *gd = nullptr, *gi = nullptr;   // half-spectrum sized work buffers
     double *accD = nullptr, *accI = nullptr;
@@ -127,13 +127,11 @@ extern "C" int ps_ctx_destroy(ps_ctx *c)
     for (auto &kv : c->twm) (void)hipFree(kv.second);
     void *bufs[] = {c->tw, c->staging, c->gd, c->gi, c->accD, c->accI, c->partials, c->dsum, c->dleg, c->dcorner};
     for (void *p : bufs) if (p) (void)hipFree(p);
-    if (c->st2) (void)hipStreamSynchronize(c->st2);
     for (hipStream_t b : c->br) {
         (void)hipStreamSynchronize(b);
         (void)hipStreamDestroy(b);
     }
     if (c->st) (void)hipStreamDestroy(c->st);
-    if (c->st2) (void)hipStreamDestroy(c->st2);
     delete c;
     return PS_OK;
 }
@@ -155,8 +153,7 @@ extern "C" int ps_ctx_create(int nbins, double boxsize, int precision, int nslot
     int st = PS_OK;
     hipDeviceProp_t prop;
     if (hipGetDevice(&c->device) != hipSuccess || hipGetDeviceProperties(&prop, c->device) != hipSuccess ||
-        hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess ||
-        hipStreamCreateWithFlags(&c->st2, hipStreamNonBlocking) != hipSuccess) {
+        hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
         ps_ctx_destroy(c);
         return PS_ERR_HIP;
     }
@@ -215,6 +212,13 @@ extern "C" int ps_get_spectrum(ps_ctx *c, int slot, void *out)
     HIPCK(hipSetDevice(c->device));
     HIPCK(hipMemcpyAsync(out, c->spec[slot], c->csize() * (size_t)c->half_elems(), hipMemcpyDeviceToHost, c->st));
     HIPCK(hipStreamSynchronize(c->st));
+    return PS_OK;
+}
+
+extern "C" int ps_set_method(ps_ctx *c, int method)
+{
+    if (!c || (method != PS_METHOD_DIRECT && method != PS_METHOD_FOURIER)) return PS_ERR_INVALID;
+    c->method = method;
     return PS_OK;
 }
 
@@ -442,57 +446,54 @@ extern "C" int ps_num_polygons(ps_ctx *c, int n, const double *k, double *kmin, 
 }
 
 // ---------------------------------------------------------------- batched path
-struct GroupRange {
-    int64_t g1a = 0, g1b = 0, g2a = 0, g2b = 0, i0 = 0, i1 = 0;
-};
-
-// Fourier-space method: the groups are evaluated in waves; the prefix legs a
-// wave transforms and the planes of its groups live in one buffer arena that
-// every wave reuses in turn.
-struct SpecWave {
-    int M = 0;                           // transform grid of the wave (<= the spectrum's N)
-    const void *tw = nullptr;            // its twiddle table
-    std::vector<LegDev> legs;            // prefix legs transformed in this wave
-    std::vector<SpecGroupDev> groups;
-    std::vector<size_t> leg_off;         // arena offsets of the legs' G1/G2 buffers
-    std::vector<size_t> grp_off;         // arena offsets of the groups' Zf/Fx buffers
+// The groups of a batch run in waves: one transform grid M per wave (the
+// smallest alias-free grid of its groups), its legs transformed at M, its
+// buffers in the batch's arena (disjoint per wave when they all fit, so that
+// the waves run side by side on branch streams; else one region reused in turn).
+struct Wave {
+    int M = 0;
+    const void *tw = nullptr;            // twiddle table of M
+    std::vector<LegDev> legs;            // legs transformed in this wave (Fourier: prefix legs only)
+    std::vector<size_t> leg_off;         // arena offsets of their G1/G2 buffers
     LegDev *dlegs = nullptr;
-    SpecGroupDev *dgroups = nullptr;
-    int64_t g1_tiles = 0, g2_tiles = 0, zf_tiles = 0, fx_tiles = 0, dot_items = 0;
+    int64_t g1_tiles = 0, g2_tiles = 0;
     int kz_cap = 1;                      // 1 + largest Kz of the wave's legs
-    int kinds = 0;                       // dot pass instantiations needed: 1 radial, 2 cube-corner masks
     size_t bytes = 0;
+    // direct method
+    std::vector<GroupDev> groups;        // leg indices local to the wave
+    std::vector<int> suffix;
+    GroupDev *dgroups = nullptr;
+    int *dsuffix = nullptr;
+    int64_t items = 0;                   // Zc work items
+    // Fourier method
+    std::vector<SpecGroupDev> sgroups;
+    std::vector<size_t> grp_off;         // arena offsets of the groups' Zf/Fx buffers
+    SpecGroupDev *dsgroups = nullptr;
+    int64_t zf_tiles = 0, fx_tiles = 0, dot_items = 0;
+    int kinds = 0;                       // dot pass instantiations needed: 1 radial, 2 cube-corner masks
 };
 
 struct ps_batch {
     ps_ctx *c = nullptr;
     int npoly = 0;
-    int method = PS_METHOD_FOURIER;
+    int method = PS_METHOD_DIRECT;
     std::vector<int> poly_n;
     std::vector<int> poly_out;           // output slot of each polygon
     int nout = 0;
-    std::vector<LegDev> legs;
-    std::vector<GroupDev> groups;
+    std::vector<LegDev> legs;            // host table of the de-duplicated legs (masks, bands)
+    std::vector<GroupDev> groups;        // host table of the groups (global leg ids)
     std::vector<int> suffix;
-    int64_t g1_tiles = 0, g2_tiles = 0, items = 0;
     int max_chunks = 1;
     int grid_cap = 2048;
-    int kz_cap = 1;                      // 1 + largest Kz of the legs
     int g2_div = 1;                      // G2 (and Fx) tile width divisor (1 or 2)
     int chunk_tiles = 64;                // Zc tiles (direct) or dot steps (Fourier) per work item
+    int overlap = 1;                     // independent waves may run side by side
     std::vector<void *> allocs;
-    int overlap = 0;                     // schedule, as ps_launch_params::overlap
-    std::vector<GroupRange> ranges;      // per group: tiles of the legs it introduces, its work items
-    std::vector<hipEvent_t> gev;         // per group: its legs are transformed (graph edges)
-    hipEvent_t zev = nullptr;            // every group's products are summed
-    LegDev *dlegs = nullptr;
-    GroupDev *dgroups = nullptr;
-    int *dsuffix = nullptr;
     int *dcounts = nullptr;
-    double *dscale = nullptr;            // per-leg amplitude scales (mixed precision)
+    double *dscale = nullptr;            // per wave-leg amplitude scales
     double2 *dpartials = nullptr;
     double *dsum = nullptr;              // 2*nout
-    std::vector<SpecWave> waves;         // Fourier method
+    std::vector<Wave> waves;
     MaskParams *dmasks = nullptr;        // suffix masks in suffix order (Fourier method)
     int2 *divl = nullptr;                // their m^2 intervals and counts
     void *arena = nullptr;
@@ -528,17 +529,14 @@ extern "C" int ps_batch_destroy(ps_batch *b)
     if (b->graph) (void)hipGraphDestroy(b->graph);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
-    for (hipEvent_t e : b->gev) (void)hipEventDestroy(e);
-    if (b->zev) (void)hipEventDestroy(b->zev);
     for (hipEvent_t e : b->wend) (void)hipEventDestroy(e);
     if (b->fork) (void)hipEventDestroy(b->fork);
     for (void *p : b->allocs) (void)hipFree(p);
-    for (SpecWave &w : b->waves) {
-        if (w.dlegs) (void)hipFree(w.dlegs);
-        if (w.dgroups) (void)hipFree(w.dgroups);
+    for (Wave &w : b->waves) {
+        void *wb[] = {w.dlegs, w.dgroups, w.dsuffix, w.dsgroups};
+        for (void *p : wb) if (p) (void)hipFree(p);
     }
-    void *bufs[] = {b->dlegs, b->dgroups, b->dsuffix, b->dcounts, b->dpartials, b->dsum,
-                    b->dscale, b->dmasks, b->divl, b->arena};
+    void *bufs[] = {b->dcounts, b->dpartials, b->dsum, b->dscale, b->dmasks, b->divl, b->arena};
     for (void *p : bufs) if (p) (void)hipFree(p);
     delete b;
     return PS_OK;
@@ -549,72 +547,51 @@ static bool same_mask(const MaskParams &a, const MaskParams &b)
     return a.kind == b.kind && a.kkf == b.kkf && a.mmin == b.mmin && a.mmax == b.mmax && a.bw == b.bw;
 }
 
-// The launch sequence of one batch (captured into graphs).
-// Fourier method: per wave, the inverse transforms of the prefix legs, then
-// the Zf, Fx and dot passes of the wave's groups.
-// Direct method without overlap: each pass over every leg / group in turn on
-// stream s.  With overlap the HBM-bound G1/G2 passes of group g+1 run on
-// stream s while the compute-bound fused pass of group g runs on stream s2;
-// events order each group's fused pass after the transforms of the legs it
-// reads.
-static int batch_record_on(ps_batch *b, hipStream_t s, hipStream_t s2)
+// The passes of one wave on stream ws: the inverse transforms of its legs,
+// then (direct) the fused product-reduce, or (Fourier) Zf, Fx and dot.
+static int record_wave(ps_batch *b, Wave &w, hipStream_t ws)
 {
     ps_ctx *c = b->c;
-    const int N = c->N;
+    const int M = w.M, nl = (int)w.legs.size();
+    if (nl > 0) {
+        if (b->dscale) PSCK(run_leg_scale(c->fp64, M, w.dlegs, nl, ws));
+        PSCK(run_inverse_g(c->fp64, M, w.dlegs, nl, 0, w.g1_tiles, 0, w.g2_tiles, b->grid_cap, b->g2_div, w.tw, ws));
+    }
     if (b->method == PS_METHOD_FOURIER) {
-        auto wave = [&](SpecWave &w, hipStream_t ws) -> int {
-            const int nl = (int)w.legs.size(), ng = (int)w.groups.size(), M = w.M;
-            if (nl > 0) {
-                if (b->dscale) PSCK(run_leg_scale(c->fp64, M, w.dlegs, nl, ws));
-                PSCK(run_inverse_g(c->fp64, M, w.dlegs, nl, 0, w.g1_tiles, 0, w.g2_tiles, b->grid_cap, b->g2_div,
-                                   w.tw, ws));
-            }
-            PSCK(run_spec_zf(c->fp64, M, w.dgroups, ng, w.dlegs, 0, w.zf_tiles, b->grid_cap, w.kz_cap, w.tw, ws));
-            PSCK(run_spec_fx(c->fp64, M, w.dgroups, ng, 0, w.fx_tiles, b->grid_cap, b->g2_div, w.tw, ws));
-            PSCK(run_spec_dot(c->fp64, M, w.dgroups, ng, b->dmasks, b->divl, 0, w.dot_items, b->dpartials,
-                              b->max_chunks, b->grid_cap, w.kinds, w.tw, ws));
-            return PS_OK;
-        };
-        const int nw = (int)b->waves.size(), nb = (int)b->wend.size();
-        if (b->concurrent && nb > 1) {
-            // fork: wave i on branch stream i mod nb; join before the reduction
-            HIPCK(hipEventRecord(b->fork, s));
-            for (int k = 0; k < nb; ++k) HIPCK(hipStreamWaitEvent(c->br[k], b->fork, 0));
-            for (int i = 0; i < nw; ++i) PSCK(wave(b->waves[i], c->br[i % nb]));
-            for (int k = 0; k < nb; ++k) {
-                HIPCK(hipEventRecord(b->wend[k], c->br[k]));
-                HIPCK(hipStreamWaitEvent(s, b->wend[k], 0));
-            }
-        } else {
-            for (SpecWave &w : b->waves) PSCK(wave(w, s));
+        const int ng = (int)w.sgroups.size();
+        PSCK(run_spec_zf(c->fp64, M, w.dsgroups, ng, w.dlegs, 0, w.zf_tiles, b->grid_cap, w.kz_cap, w.tw, ws));
+        PSCK(run_spec_fx(c->fp64, M, w.dsgroups, ng, 0, w.fx_tiles, b->grid_cap, b->g2_div, w.tw, ws));
+        PSCK(run_spec_dot(c->fp64, M, w.dsgroups, ng, b->dmasks, b->divl, 0, w.dot_items, b->dpartials,
+                          b->max_chunks, b->grid_cap, w.kinds, w.tw, ws));
+    } else {
+        PSCK(run_zc_fused(c->fp64, M, w.dgroups, (int)w.groups.size(), w.dlegs, w.dsuffix, 0, w.items,
+                          b->dpartials, b->max_chunks, b->grid_cap, w.kz_cap, w.tw, ws));
+    }
+    return PS_OK;
+}
+
+// The launch sequence of one batch (captured into graphs): the waves one
+// after another on s, or forked onto the branch streams (wave i on branch
+// i mod nb) and joined, then the final reduction on s.
+static int batch_record_on(ps_batch *b, hipStream_t s)
+{
+    ps_ctx *c = b->c;
+    const int nw = (int)b->waves.size(), nb = (int)b->wend.size();
+    if (b->concurrent && nb > 1) {
+        HIPCK(hipEventRecord(b->fork, s));
+        for (int k = 0; k < nb; ++k) HIPCK(hipStreamWaitEvent(c->br[k], b->fork, 0));
+        for (int i = 0; i < nw; ++i) PSCK(record_wave(b, b->waves[i], c->br[i % nb]));
+        for (int k = 0; k < nb; ++k) {
+            HIPCK(hipEventRecord(b->wend[k], c->br[k]));
+            HIPCK(hipStreamWaitEvent(s, b->wend[k], 0));
         }
-        PSCK(run_reduce_partials(b->dpartials, b->max_chunks, b->dcounts, 0, b->nout, b->dsum, b->dsum + b->nout, s));
-        return PS_OK;
+    } else {
+        for (Wave &w : b->waves) PSCK(record_wave(b, w, s));
     }
-    const int nl = (int)b->legs.size(), ng = (int)b->groups.size();
-    if (b->dscale) PSCK(run_leg_scale(c->fp64, N, b->dlegs, nl, s));
-    if (!b->overlap) {
-        PSCK(run_inverse_g(c->fp64, N, b->dlegs, nl, 0, b->g1_tiles, 0, b->g2_tiles, b->grid_cap, b->g2_div,
-                           c->tw, s));
-        PSCK(run_zc_fused(c->fp64, N, b->dgroups, ng, b->dlegs, b->dsuffix, 0, b->items, b->dpartials,
-                          b->max_chunks, b->grid_cap, b->kz_cap, c->tw, s));
-        PSCK(run_reduce_partials(b->dpartials, b->max_chunks, b->dcounts, 0, b->nout, b->dsum, b->dsum + b->nout, s));
-        return PS_OK;
-    }
-    for (int g = 0; g < ng; ++g) {
-        const GroupRange &R = b->ranges[g];
-        PSCK(run_inverse_g(c->fp64, N, b->dlegs, nl, R.g1a, R.g1b, R.g2a, R.g2b, b->grid_cap, b->g2_div, c->tw, s));
-        HIPCK(hipEventRecord(b->gev[g], s));
-        HIPCK(hipStreamWaitEvent(s2, b->gev[g], 0));
-        PSCK(run_zc_fused(c->fp64, N, b->dgroups, ng, b->dlegs, b->dsuffix, R.i0, R.i1, b->dpartials,
-                          b->max_chunks, b->grid_cap, b->kz_cap, c->tw, s2));
-    }
-    HIPCK(hipEventRecord(b->zev, s2));
-    HIPCK(hipStreamWaitEvent(s, b->zev, 0));
     PSCK(run_reduce_partials(b->dpartials, b->max_chunks, b->dcounts, 0, b->nout, b->dsum, b->dsum + b->nout, s));
     return PS_OK;
 }
-static int batch_record(ps_batch *b) { return batch_record_on(b, b->c->st, b->c->st2); }
+static int batch_record(ps_batch *b) { return batch_record_on(b, b->c->st); }
 
 // ---------------------------------------------------------------- launch-shape tuning
 // Best launch parameters found by ps_batch_autotune, per batch shape; kept for
@@ -631,8 +608,8 @@ static std::string tune_key(const ps_ctx *c, const ps_polygon *polys, int npoly,
         slots = std::max(slots, polys[p].slot + 1);
     }
     char buf[256];
-    snprintf(buf, sizeof(buf), "N%d_fp%d_cu%d_np%d_kc%d_sl%d_bw%.6g_ks%.9g", c->N, c->fp64, c->cus, npoly, kcase, slots,
-             bw, ks);
+    snprintf(buf, sizeof(buf), "N%d_fp%d_m%d_cu%d_np%d_kc%d_sl%d_bw%.6g_ks%.9g", c->N, c->fp64, c->method, c->cus,
+             npoly, kcase, slots, bw, ks);
     return buf;
 }
 
@@ -655,11 +632,11 @@ static void tune_load()
 static ps_launch_params default_params(const ps_ctx *c)
 {
     ps_launch_params lp;
-    lp.grid_cap = 8 * c->cus;
-    lp.chunk_tiles = 1;  // dot steps per work item (the direct method's Zc runs best near 64 tiles)
+    lp.grid_cap = 16 * c->cus;
+    lp.method = c->method;
+    lp.chunk_tiles = 1;  // dot steps (Fourier) or Zc tiles (direct) per work item
     lp.g2_div = 1;
-    lp.overlap = 0;
-    lp.method = PS_METHOD_FOURIER;
+    lp.overlap = 1;
     if (const char *e = getenv("PS_CHUNK_TILES")) lp.chunk_tiles = std::max(1, atoi(e));
     if (const char *e = getenv("PS_GRID_CAP")) lp.grid_cap = std::max(1, atoi(e));
     return lp;
@@ -676,115 +653,49 @@ extern "C" int ps_batch_create(ps_ctx *c, ps_polygon *polys, int npoly, double b
         if (e_ != hipSuccess) return (e_ == hipErrorOutOfMemory) ? PS_ERR_NOMEM : PS_ERR_HIP; \
     } while (0)
 
-// Direct method: every leg inverse transformed (G1, G2), the products and
-// sums in real space in the fused Zc pass.
-static int plan_direct(ps_batch *b, std::vector<int> &counts)
+// smallest supported transform grid m >= need, at most the spectrum's N
+static int alias_free_grid(int need, int N)
 {
-    ps_ctx *c = b->c;
-    const int N = c->N;
-    const size_t cs = c->csize();
-    const int tl = g_tile_lines(N, c->fp64);
-    for (LegDev &L : b->legs) {
-        const int K = L.Kz;
-        b->kz_cap = std::max(b->kz_cap, K + 1);
-        L.g1_begin = b->g1_tiles;
-        L.g2_begin = b->g2_tiles;
-        b->g1_tiles += (int64_t)(K + 1) * ((L.Bx + tl - 1) / tl);
-        b->g2_tiles += 2 * (int64_t)(K + 1) * (N / (tile_rows(N, c->fp64) / b->g2_div));
-        const size_t n1 = (size_t)(K + 1) * L.Bx * N, n2 = (size_t)(K + 1) * N * N;
-        void *p1 = nullptr, *p2 = nullptr;
-        PCK(hipMalloc(&p1, cs * n1 * 2));
-        b->allocs.push_back(p1);
-        PCK(hipMalloc(&p2, cs * n2 * 2));
-        b->allocs.push_back(p2);
-        L.g1d = p1;
-        L.g1i = (char *)p1 + cs * n1;
-        L.g2d = p2;
-        L.g2i = (char *)p2 + cs * n2;
-        // model bytes: G1 reads the spectrum cells of the band box and writes
-        // the D,I band lines; G2 reads them and writes the D,I planes
-        b->stage_bytes[0] += (double)cs * ((double)(K + 1) * L.Bx * std::min(2 * K + 1, N) + 2.0 * n1);
-        b->stage_bytes[1] += (double)cs * (2.0 * n1 + 2.0 * n2);
-    }
-    const int rows = fused_tile_rows(N, c->fp64);
-    const int64_t tiles_per_group = (int64_t)N * (N / rows);
-    const int nchunks = (int)((tiles_per_group + b->chunk_tiles - 1) / b->chunk_tiles);
-    b->max_chunks = nchunks;
-    int legs_seen = 0;
-    for (GroupDev &G : b->groups) {
-        G.item_begin = b->items;
-        b->items += nchunks;
-        for (int o = 0; o < G.nsuf; ++o) counts[G.out_begin + o] = nchunks;
-        // the legs this group introduces: [legs_seen, hi)
-        int hi = legs_seen;
-        if (G.npre >= 1) hi = std::max(hi, G.preA + 1);
-        if (G.npre == 2) hi = std::max(hi, G.preB + 1);
-        for (int o = 0; o < G.nsuf; ++o) hi = std::max(hi, b->suffix[G.suf_begin + o] + 1);
-        GroupRange R;
-        const int nl = (int)b->legs.size();
-        R.g1a = legs_seen < nl ? b->legs[legs_seen].g1_begin : b->g1_tiles;
-        R.g1b = hi < nl ? b->legs[hi].g1_begin : b->g1_tiles;
-        R.g2a = legs_seen < nl ? b->legs[legs_seen].g2_begin : b->g2_tiles;
-        R.g2b = hi < nl ? b->legs[hi].g2_begin : b->g2_tiles;
-        if (hi <= legs_seen) R.g1b = R.g1a, R.g2b = R.g2a;
-        R.i0 = G.item_begin;
-        R.i1 = G.item_begin + nchunks;
-        b->ranges.push_back(R);
-        legs_seen = std::max(legs_seen, hi);
-        // model bytes of the fused pass: every leg's D,I planes read once per group
-        int64_t planes = 0;
-        if (G.npre >= 1) planes += b->legs[G.preA].Kz + 1;
-        if (G.npre == 2) planes += b->legs[G.preB].Kz + 1;
-        for (int o = 0; o < G.nsuf; ++o) planes += b->legs[b->suffix[G.suf_begin + o]].Kz + 1;
-        b->stage_bytes[2] += 2.0 * cs * (double)planes * N * N;
-    }
-    b->nstages = 4;
-    b->stage_bytes[3] = 32.0 * (double)b->nout * b->max_chunks;
-    b->stage_bytes[2] += b->stage_bytes[3];  // the fused pass writes the partial sums
-    if (!c->fp64) {
-        PCK(hipMalloc(&b->dscale, sizeof(double) * b->legs.size()));
-        for (size_t i = 0; i < b->legs.size(); ++i) b->legs[i].scale = b->dscale + i;
-    }
-    PCK(hipMalloc(&b->dlegs, sizeof(LegDev) * b->legs.size()));
-    PCK(hipMalloc(&b->dgroups, sizeof(GroupDev) * b->groups.size()));
-    PCK(hipMalloc(&b->dsuffix, sizeof(int) * b->suffix.size()));
-    PCK(hipMemcpy(b->dlegs, b->legs.data(), sizeof(LegDev) * b->legs.size(), hipMemcpyHostToDevice));
-    PCK(hipMemcpy(b->dgroups, b->groups.data(), sizeof(GroupDev) * b->groups.size(), hipMemcpyHostToDevice));
-    PCK(hipMemcpy(b->dsuffix, b->suffix.data(), sizeof(int) * b->suffix.size(), hipMemcpyHostToDevice));
-    return PS_OK;
+    static const int kGrids[] = {16, 24, 32, 48, 64, 96, 128, 192, 256, 384, 512};
+    for (int m : kGrids)
+        if (m >= need && m <= N) return m;
+    return N;
 }
 
-// Fourier method: only the prefix legs of triangle groups are inverse
-// transformed; each group's prefix product is forward transformed once and
-// every suffix leg is a sum over its shell (power spectra: no transform).
-static int plan_fourier(ps_batch *b, std::vector<int> &counts)
+// Plans the waves of a batch (either method): per group its transform grid,
+// the waves (one grid each, within the arena budget), per wave the legs it
+// transforms, the tiles and work items of every pass, the model bytes, the
+// buffers and the device tables.
+static int plan_waves(ps_batch *b, std::vector<int> &counts)
 {
     ps_ctx *c = b->c;
     const int N = c->N, fp64 = c->fp64;
     const size_t cs = c->csize();
-    // suffix masks and their m^2 intervals
-    const size_t ns = b->suffix.size();
-    std::vector<MaskParams> sm(ns);
-    std::vector<int> skmax(ns);
-    for (size_t s = 0; s < ns; ++s) {
-        sm[s] = b->legs[b->suffix[s]].mask;
-        skmax[s] = b->legs[b->suffix[s]].Kz;
-    }
-    std::vector<char> radial_ok(ns, 0);
-    PCK(hipMalloc(&b->dmasks, sizeof(MaskParams) * ns));
-    PCK(hipMalloc(&b->divl, sizeof(int2) * 2 * ns));
-    {
+    const bool fourier = (b->method == PS_METHOD_FOURIER);
+    const int ng = (int)b->groups.size();
+    // ---- Fourier: suffix masks and their m^2 intervals
+    const size_t nsf = b->suffix.size();
+    std::vector<char> radial_ok(nsf, 0);
+    if (fourier) {
+        std::vector<MaskParams> sm(nsf);
+        std::vector<int> skmax(nsf);
+        for (size_t s = 0; s < nsf; ++s) {
+            sm[s] = b->legs[b->suffix[s]].mask;
+            skmax[s] = b->legs[b->suffix[s]].Kz;
+        }
+        PCK(hipMalloc(&b->dmasks, sizeof(MaskParams) * nsf));
+        PCK(hipMalloc(&b->divl, sizeof(int2) * 2 * nsf));
         int *dk = nullptr;
-        PCK(hipMalloc(&dk, sizeof(int) * ns));
+        PCK(hipMalloc(&dk, sizeof(int) * nsf));
         b->allocs.push_back(dk);
-        PCK(hipMemcpy(b->dmasks, sm.data(), sizeof(MaskParams) * ns, hipMemcpyHostToDevice));
-        PCK(hipMemcpy(dk, skmax.data(), sizeof(int) * ns, hipMemcpyHostToDevice));
-        PSCK(run_shell_intervals(b->dmasks, dk, (int)ns, b->divl, c->st));
-        std::vector<int2> hv(2 * ns);
-        PCK(hipMemcpyAsync(hv.data(), b->divl, sizeof(int2) * 2 * ns, hipMemcpyDeviceToHost, c->st));
+        PCK(hipMemcpy(b->dmasks, sm.data(), sizeof(MaskParams) * nsf, hipMemcpyHostToDevice));
+        PCK(hipMemcpy(dk, skmax.data(), sizeof(int) * nsf, hipMemcpyHostToDevice));
+        PSCK(run_shell_intervals(b->dmasks, dk, (int)nsf, b->divl, c->st));
+        std::vector<int2> hv(2 * nsf);
+        PCK(hipMemcpyAsync(hv.data(), b->divl, sizeof(int2) * 2 * nsf, hipMemcpyDeviceToHost, c->st));
         PCK(hipStreamSynchronize(c->st));
         bool patched = false;
-        for (size_t s = 0; s < ns; ++s) {
+        for (size_t s = 0; s < nsf; ++s) {
             // a radial shell is one m^2 interval; anything else is tested per cell
             const int2 iv = hv[2 * s];
             const int cnt = hv[2 * s + 1].x;
@@ -793,37 +704,38 @@ static int plan_fourier(ps_batch *b, std::vector<int> &counts)
                 hv[2 * s] = make_int2(-1, -1);
                 patched = true;
             }
+            radial_ok[s] = mask_is_radial(sm[s].kind) && hv[2 * s].x >= 0;
         }
-        if (patched) PCK(hipMemcpy(b->divl, hv.data(), sizeof(int2) * 2 * ns, hipMemcpyHostToDevice));
-        for (size_t s = 0; s < ns; ++s) radial_ok[s] = mask_is_radial(sm[s].kind) && hv[2 * s].x >= 0;
+        if (patched) PCK(hipMemcpy(b->divl, hv.data(), sizeof(int2) * 2 * nsf, hipMemcpyHostToDevice));
     }
-    // per group: its band and the transform grid M (smallest supported size
-    // above K_A + K_B + Kq, where the product is alias-free; N at most)
-    static const int kGrids[] = {16, 24, 32, 48, 64, 96, 128, 192, 256, 384, 512};
-    auto grid_for = [&](int need) {
-        for (int m : kGrids)
-            if (m >= need && m <= N) return m;
-        return N;
-    };
-    std::vector<SpecGroupDev> sg;
-    std::vector<int> sgM;
-    for (const GroupDev &G : b->groups) {
-        SpecGroupDev S{};
-        S.fft = (G.npre == 2 || G.pre_sq) ? 1 : 0;
+    // ---- per group: grid and (Fourier) descriptor
+    std::vector<int> gM(ng);
+    std::vector<SpecGroupDev> sg(fourier ? ng : 0);
+    for (int gi = 0; gi < ng; ++gi) {
+        const GroupDev &G = b->groups[gi];
         int Kt = 0;
         for (int o = 0; o < G.nsuf; ++o) Kt = std::max(Kt, b->legs[b->suffix[G.suf_begin + o]].Kz);
+        const int KA = (G.npre >= 1) ? b->legs[G.preA].Kz : 0;
+        const int KB = (G.npre == 2) ? b->legs[G.preB].Kz : (G.pre_sq ? KA : 0);
+        if (!fourier) {
+            // the product of all legs must not alias onto m = 0
+            gM[gi] = alias_free_grid((G.npre == 0 ? 2 * Kt : KA + KB + Kt) + 1, N);
+            continue;
+        }
+        SpecGroupDev S{};
+        S.fft = (G.npre == 2 || G.pre_sq) ? 1 : 0;
         int Kq = Kt, need = 0;
         if (S.fft) {
             // the product of the prefix legs vanishes beyond |m_i| = K_A + K_B
-            const int KA = b->legs[G.preA].Kz, KB = b->legs[G.npre == 2 ? G.preB : G.preA].Kz;
             Kq = std::min(Kq, KA + KB);
             need = KA + KB + Kq + 1;
         } else {
-            if (G.npre == 1) Kq = std::min(Kq, b->legs[G.preA].Kz);
+            if (G.npre == 1) Kq = std::min(Kq, KA);
             need = 2 * Kq + 1;
         }
         Kq = std::min(Kq, N / 2);
-        const int M = grid_for(need);
+        const int M = alias_free_grid(need, N);
+        gM[gi] = M;
         S.Kq = Kq;
         S.Kqz = std::min(Kq, M / 2);
         S.Bq = std::min(2 * Kq + 1, M);
@@ -843,18 +755,16 @@ static int plan_fourier(ps_batch *b, std::vector<int> &counts)
         S.fscale = S.fft ? pow((double)N / (double)M, 3) : 1.0;
         S.radial = 1;
         for (int o = 0; o < G.nsuf; ++o) S.radial &= radial_ok[G.suf_begin + o] ? 1 : 0;
-        sg.push_back(S);
-        sgM.push_back(M);
+        sg[gi] = S;
     }
-    // waves: one transform grid each, within the arena budget; grids in
-    // increasing order, groups in their order within a grid
+    // ---- waves: one grid each, within the arena budget; grids in increasing
+    // order, groups in their order within a grid
     size_t budget = (size_t)16 << 30;
     size_t mfree = 0, mtot = 0;
     if (hipMemGetInfo(&mfree, &mtot) == hipSuccess && mfree / 2 < budget) budget = mfree / 2;
     if (const char *e = getenv("PS_ARENA_MB")) budget = (size_t)std::max(1, atoi(e)) << 20;
-    std::vector<SpecWave> &W = b->waves;
-    std::map<int, int> wave_leg;  // leg id -> index in the current wave
-    std::vector<int> grids(sgM.begin(), sgM.end());
+    std::vector<Wave> &W = b->waves;
+    std::vector<int> grids(gM.begin(), gM.end());
     std::sort(grids.begin(), grids.end());
     grids.erase(std::unique(grids.begin(), grids.end()), grids.end());
     for (int M : grids) {
@@ -862,27 +772,34 @@ static int plan_fourier(ps_batch *b, std::vector<int> &counts)
             const int Bx = std::min(2 * L.Kz + 1, M);
             return 2 * cs * ((size_t)(L.Kz + 1) * Bx * M + (size_t)(L.Kz + 1) * M * M);
         };
+        std::map<int, int> wave_leg;  // leg id -> index in the current wave
         bool fresh = true;
-        for (size_t gi = 0; gi < sg.size(); ++gi) {
-            if (sgM[gi] != M) continue;
-            SpecGroupDev S = sg[gi];
-            const size_t gbytes = S.fft ? 2 * cs * ((size_t)(S.Kqz + 1) * M * M + (size_t)(S.Kqz + 1) * S.Bq * M) : 0;
-            std::vector<int> pre;
-            if (S.fft) {
-                pre.push_back(S.preA);
-                if (S.npre == 2) pre.push_back(S.preB);
+        for (int gi = 0; gi < ng; ++gi) {
+            if (gM[gi] != M) continue;
+            const GroupDev &G = b->groups[gi];
+            // legs the group transforms (Fourier: the prefix of triangle groups only)
+            std::vector<int> use;
+            if (!fourier || sg[gi].fft) {
+                if (G.npre >= 1) use.push_back(G.preA);
+                if (G.npre == 2) use.push_back(G.preB);
             }
+            if (!fourier)
+                for (int o = 0; o < G.nsuf; ++o) use.push_back(b->suffix[G.suf_begin + o]);
+            const size_t gbytes =
+                (fourier && sg[gi].fft)
+                    ? 2 * cs * ((size_t)(sg[gi].Kqz + 1) * M * M + (size_t)(sg[gi].Kqz + 1) * sg[gi].Bq * M)
+                    : 0;
             size_t need = gbytes;
-            for (int p : pre)
+            for (int p : use)
                 if (fresh || !wave_leg.count(p)) need += leg_bytes(b->legs[p]);
-            if (fresh || (!W.back().groups.empty() && W.back().bytes + need > budget)) {
+            if (fresh || (W.back().bytes + need > budget)) {
                 W.emplace_back();
                 W.back().M = M;
                 wave_leg.clear();
                 fresh = false;
             }
-            SpecWave &w = W.back();
-            for (int p : pre) {
+            Wave &w = W.back();
+            for (int p : use) {
                 if (wave_leg.count(p)) continue;
                 wave_leg[p] = (int)w.legs.size();
                 LegDev L = b->legs[p];
@@ -891,25 +808,33 @@ static int plan_fourier(ps_batch *b, std::vector<int> &counts)
                 w.bytes += leg_bytes(L);
                 w.legs.push_back(L);
             }
-            if (S.fft) {
-                S.preA = wave_leg[S.preA];
-                S.preB = wave_leg[S.preB];
+            if (fourier) {
+                SpecGroupDev S = sg[gi];
+                if (S.fft) {
+                    S.preA = wave_leg[S.preA];
+                    S.preB = wave_leg[S.preB];
+                } else {
+                    S.preA = S.preB = 0;
+                }
+                w.grp_off.push_back(S.fft ? w.bytes : 0);
+                w.bytes += gbytes;
+                w.sgroups.push_back(S);
             } else {
-                S.preA = S.preB = 0;
+                GroupDev D = G;
+                if (D.npre >= 1) D.preA = wave_leg[G.preA];
+                if (D.npre == 2) D.preB = wave_leg[G.preB];
+                D.suf_begin = (int)w.suffix.size();
+                for (int o = 0; o < G.nsuf; ++o) w.suffix.push_back(wave_leg[b->suffix[G.suf_begin + o]]);
+                D.fscale = pow((double)N / (double)M, 3);
+                w.groups.push_back(D);
             }
-            w.grp_off.push_back(S.fft ? w.bytes : 0);
-            w.bytes += gbytes;
-            w.groups.push_back(S);
         }
     }
-    // tiles, work items and model bytes
+    // ---- tiles, work items and model bytes
     size_t arena = 0, nscale = 0, total_bytes = 0;
-    for (SpecWave &w : W) {
+    for (Wave &w : W) {
         const int M = w.M;
         const int tl = g_tile_lines(M, fp64);
-        const int zrows = spec_zf_rows(M, fp64);
-        const int fxcols = tile_rows(M, fp64) / b->g2_div;
-        const int dot_lines = b->chunk_tiles * spec_dot_lines(M, fp64);
         if (M == N) {
             w.tw = c->tw;
         } else {
@@ -928,66 +853,97 @@ static int plan_fourier(ps_batch *b, std::vector<int> &counts)
             L.g2_begin = w.g2_tiles;
             w.g1_tiles += (int64_t)(K + 1) * ((L.Bx + tl - 1) / tl);
             w.g2_tiles += 2 * (int64_t)(K + 1) * (M / (tile_rows(M, fp64) / b->g2_div));
+            // G1 reads the spectrum cells of the band box and writes the D,I band
+            // lines; G2 reads them and writes the D,I planes
             const double n1 = (double)(K + 1) * L.Bx * M, n2 = (double)(K + 1) * M * M;
             b->stage_bytes[0] += (double)cs * ((double)(K + 1) * L.Bx * L.Bx + 2.0 * n1);
             b->stage_bytes[1] += (double)cs * (2.0 * n1 + 2.0 * n2);
         }
-        for (SpecGroupDev &S : w.groups) {
-            w.kinds |= S.radial ? 1 : 2;
-            S.dot_lines = dot_lines;
-            S.zf_begin = w.zf_tiles;
-            S.fx_begin = w.fx_tiles;
-            S.dot_begin = w.dot_items;
-            const double planes = (double)(S.Kqz + 1) * M * M, band = (double)(S.Kqz + 1) * S.Bq * M;
-            if (S.fft) {
-                w.zf_tiles += (int64_t)M * (M / zrows);
-                w.fx_tiles += 2 * (int64_t)(S.Kqz + 1) * (M / fxcols);
-                double pre = (double)(w.legs[S.preA].Kz + 1);
-                if (S.npre == 2) pre += (double)(w.legs[S.preB].Kz + 1);
-                b->stage_bytes[2] += 2.0 * cs * (pre * M * M + planes);
-                b->stage_bytes[3] += 2.0 * cs * (planes + band);
-                b->stage_bytes[4] += 2.0 * cs * band;
+        if (fourier) {
+            const int zrows = spec_zf_rows(M, fp64);
+            const int fxcols = tile_rows(M, fp64) / b->g2_div;
+            const int dot_lines = b->chunk_tiles * spec_dot_lines(M, fp64);
+            for (SpecGroupDev &S : w.sgroups) {
+                w.kinds |= S.radial ? 1 : 2;
+                S.dot_lines = dot_lines;
+                S.zf_begin = w.zf_tiles;
+                S.fx_begin = w.fx_tiles;
+                S.dot_begin = w.dot_items;
+                const double planes = (double)(S.Kqz + 1) * M * M, band = (double)(S.Kqz + 1) * S.Bq * M;
+                if (S.fft) {
+                    w.zf_tiles += (int64_t)M * (M / zrows);
+                    w.fx_tiles += 2 * (int64_t)(S.Kqz + 1) * (M / fxcols);
+                    double pre = (double)(w.legs[S.preA].Kz + 1);
+                    if (S.npre == 2) pre += (double)(w.legs[S.preB].Kz + 1);
+                    b->stage_bytes[2] += 2.0 * cs * (pre * M * M + planes);
+                    b->stage_bytes[3] += 2.0 * cs * (planes + band);
+                    b->stage_bytes[4] += 2.0 * cs * band;
+                }
+                b->stage_bytes[4] += (double)cs * (S.Kqz + 1) * (double)S.Bq * S.Bq;  // spectrum cells of the band
+                const int64_t nlines = 2 * (int64_t)(S.Kqz + 1) * S.Bq;
+                const int items = (int)((nlines + dot_lines - 1) / dot_lines);
+                w.dot_items += items;
+                for (int o = 0; o < S.nsuf; ++o) counts[S.out_begin + o] = items;
+                b->max_chunks = std::max(b->max_chunks, items);
             }
-            b->stage_bytes[4] += (double)cs * (S.Kqz + 1) * (double)S.Bq * S.Bq;  // spectrum cells of the band
-            const int64_t nlines = 2 * (int64_t)(S.Kqz + 1) * S.Bq;
-            const int items = (int)((nlines + dot_lines - 1) / dot_lines);
-            w.dot_items += items;
-            for (int o = 0; o < S.nsuf; ++o) counts[S.out_begin + o] = items;
-            b->max_chunks = std::max(b->max_chunks, items);
+        } else {
+            const int rows = fused_tile_rows(M, fp64);
+            const int64_t tiles_per_group = (int64_t)M * (M / rows);
+            const int nchunks = (int)((tiles_per_group + b->chunk_tiles - 1) / b->chunk_tiles);
+            for (GroupDev &G : w.groups) {
+                G.chunk_tiles = b->chunk_tiles;
+                G.item_begin = w.items;
+                w.items += nchunks;
+                for (int o = 0; o < G.nsuf; ++o) counts[G.out_begin + o] = nchunks;
+                b->max_chunks = std::max(b->max_chunks, nchunks);
+                // model bytes of the fused pass: every leg's D,I planes read once per group
+                int64_t planes = 0;
+                if (G.npre >= 1) planes += w.legs[G.preA].Kz + 1;
+                if (G.npre == 2) planes += w.legs[G.preB].Kz + 1;
+                for (int o = 0; o < G.nsuf; ++o) planes += w.legs[w.suffix[G.suf_begin + o]].Kz + 1;
+                b->stage_bytes[2] += 2.0 * cs * (double)planes * M * M;
+            }
         }
         arena = std::max(arena, w.bytes);
         total_bytes += w.bytes;
         nscale += w.legs.size();
     }
-    // waves side by side (disjoint buffers, branch streams) when they all fit
-    b->concurrent = (W.size() > 1 && total_bytes <= budget) ? 1 : 0;
+    b->nstages = fourier ? 6 : 4;
+    b->stage_bytes[b->nstages - 1] = 32.0 * (double)b->nout * b->max_chunks;
+    b->stage_bytes[b->nstages - 2] += b->stage_bytes[b->nstages - 1];  // the last pass writes the partial sums
+    // waves side by side (disjoint buffers, branch streams) when allowed and they all fit
+    b->concurrent = (b->overlap && W.size() > 1 && total_bytes <= budget) ? 1 : 0;
     if (b->concurrent) arena = total_bytes;
-    b->nstages = 6;
-    b->stage_bytes[5] = 32.0 * (double)b->nout * b->max_chunks;
-    b->stage_bytes[4] += b->stage_bytes[5];  // the dot pass writes the partial sums
-    // buffers and device tables
+    // ---- buffers and device tables
     if (arena > 0) PCK(hipMalloc(&b->arena, arena));
-    // amplitude scales in both precisions: the packed forward transform of
-    // P_D + i P_I needs parts of like size (P_D = D_A D_B is ~|H|^2 larger)
-    if (nscale > 0) PCK(hipMalloc(&b->dscale, sizeof(double) * nscale));
+    // amplitude scales: the direct method in mixed precision (its last pass
+    // packs D + i I of one leg); the Fourier method in both precisions (its
+    // forward transform packs parts of the prefix product, P_D ~ |H|^2 P_I)
+    if (nscale > 0 && (fourier || !fp64)) PCK(hipMalloc(&b->dscale, sizeof(double) * nscale));
     size_t si = 0;
     char *base = (char *)b->arena;
-    for (SpecWave &w : W) {
+    for (size_t wi = 0; wi < W.size(); ++wi) {
+        Wave &w = W[wi];
         const int M = w.M;
-        if (b->concurrent && &w != &W.front()) base += (&w - 1)->bytes;  // disjoint regions
+        if (b->concurrent && wi > 0) base += W[wi - 1].bytes;  // disjoint regions
         for (size_t i = 0; i < w.legs.size(); ++i) {
             LegDev &L = w.legs[i];
             const size_t n1 = (size_t)(L.Kz + 1) * L.Bx * M, n2 = (size_t)(L.Kz + 1) * M * M;
             char *p = base + w.leg_off[i];
-            L.g1d = p;
+            L.g1d = L.spec ? p : nullptr;
             L.g1i = p + cs * n1;
-            L.g2d = p + 2 * cs * n1;
+            L.g2d = L.spec ? p + 2 * cs * n1 : nullptr;
             L.g2i = p + 2 * cs * n1 + cs * n2;
             L.scale = b->dscale ? b->dscale + si++ : nullptr;
         }
-        for (size_t g = 0; g < w.groups.size(); ++g) {
-            SpecGroupDev &S = w.groups[g];
-            if (S.fft) {
+        if (!w.legs.empty()) {
+            PCK(hipMalloc(&w.dlegs, sizeof(LegDev) * w.legs.size()));
+            PCK(hipMemcpy(w.dlegs, w.legs.data(), sizeof(LegDev) * w.legs.size(), hipMemcpyHostToDevice));
+        }
+        if (fourier) {
+            for (size_t g = 0; g < w.sgroups.size(); ++g) {
+                SpecGroupDev &S = w.sgroups[g];
+                if (!S.fft) continue;
                 const size_t np = (size_t)(S.Kqz + 1) * M * M, nb = (size_t)(S.Kqz + 1) * S.Bq * M;
                 char *p = base + w.grp_off[g];
                 S.qd = p;
@@ -997,13 +953,15 @@ static int plan_fourier(ps_batch *b, std::vector<int> &counts)
                 S.scaleA = w.legs[S.preA].scale;
                 S.scaleB = (S.npre == 2) ? w.legs[S.preB].scale : nullptr;
             }
+            PCK(hipMalloc(&w.dsgroups, sizeof(SpecGroupDev) * w.sgroups.size()));
+            PCK(hipMemcpy(w.dsgroups, w.sgroups.data(), sizeof(SpecGroupDev) * w.sgroups.size(),
+                          hipMemcpyHostToDevice));
+        } else {
+            PCK(hipMalloc(&w.dgroups, sizeof(GroupDev) * w.groups.size()));
+            PCK(hipMemcpy(w.dgroups, w.groups.data(), sizeof(GroupDev) * w.groups.size(), hipMemcpyHostToDevice));
+            PCK(hipMalloc(&w.dsuffix, sizeof(int) * w.suffix.size()));
+            PCK(hipMemcpy(w.dsuffix, w.suffix.data(), sizeof(int) * w.suffix.size(), hipMemcpyHostToDevice));
         }
-        if (!w.legs.empty()) {
-            PCK(hipMalloc(&w.dlegs, sizeof(LegDev) * w.legs.size()));
-            PCK(hipMemcpy(w.dlegs, w.legs.data(), sizeof(LegDev) * w.legs.size(), hipMemcpyHostToDevice));
-        }
-        PCK(hipMalloc(&w.dgroups, sizeof(SpecGroupDev) * w.groups.size()));
-        PCK(hipMemcpy(w.dgroups, w.groups.data(), sizeof(SpecGroupDev) * w.groups.size(), hipMemcpyHostToDevice));
     }
     if (b->concurrent) {
         const int nb = std::min((int)W.size(), 4);
@@ -1019,6 +977,7 @@ static int plan_fourier(ps_batch *b, std::vector<int> &counts)
     return PS_OK;
 }
 #undef PCK
+
 
 extern "C" int ps_batch_create_ex(ps_ctx *c, ps_polygon *polys, int npoly, double bw, int kcase,
                                   const ps_launch_params *params, ps_batch **out)
@@ -1049,7 +1008,7 @@ extern "C" int ps_batch_create_ex(ps_ctx *c, ps_polygon *polys, int npoly, doubl
     b->method = lp.method;
     b->grid_cap = lp.grid_cap;
     b->chunk_tiles = lp.chunk_tiles;
-    b->overlap = (lp.method == PS_METHOD_DIRECT) ? lp.overlap : 0;
+    b->overlap = lp.overlap;
     b->g2_div = (tile_rows(c->N, c->fp64) / lp.g2_div >= 4) ? lp.g2_div : 1;
     // 1. legs of every polygon, de-duplicated by (slot, mask)
     std::vector<std::vector<int>> pleg(npoly);
@@ -1159,6 +1118,7 @@ extern "C" int ps_batch_create_ex(ps_ctx *c, ps_polygon *polys, int npoly, doubl
             G.suf_begin = (int)b->suffix.size();
             G.out_begin = b->nout;
             G.chunk_tiles = b->chunk_tiles;
+            G.fscale = 1.0;
             for (size_t q = s0; q < s1; ++q) {
                 const int p = ps_[q];
                 const std::vector<int> &l = pleg[p];
@@ -1172,7 +1132,7 @@ extern "C" int ps_batch_create_ex(ps_ctx *c, ps_polygon *polys, int npoly, doubl
     }
     // 4. the method's buffers, tiles and device tables
     std::vector<int> counts(b->nout, 0);
-    int st = (b->method == PS_METHOD_DIRECT) ? plan_direct(b, counts) : plan_fourier(b, counts);
+    int st = plan_waves(b, counts);
     if (st != PS_OK) {
         ps_batch_destroy(b);
         return st;
@@ -1189,9 +1149,6 @@ extern "C" int ps_batch_create_ex(ps_ctx *c, ps_polygon *polys, int npoly, doubl
     BCK(hipMemcpy(b->dcounts, counts.data(), sizeof(int) * counts.size(), hipMemcpyHostToDevice));
     BCK(hipEventCreate(&b->ev0));
     BCK(hipEventCreate(&b->ev1));
-    b->gev.assign(b->groups.size(), nullptr);
-    for (auto &e : b->gev) BCK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    BCK(hipEventCreateWithFlags(&b->zev, hipEventDisableTiming));
     // 5. capture the launch sequence once
     if (!getenv("PS_NO_GRAPH")) {
         BCK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
@@ -1303,7 +1260,7 @@ extern "C" int ps_multi_create(ps_batch **batches, int nbatch, ps_multi **out)
         st = PS_ERR_HIP;
     if (st == PS_OK && hipStreamBeginCapture(m->st, hipStreamCaptureModeThreadLocal) != hipSuccess) st = PS_ERR_HIP;
     if (st == PS_OK) {
-        for (int i = 0; i < nbatch && st == PS_OK; ++i) st = batch_record_on(batches[i], m->st, batches[i]->c->st2);
+        for (int i = 0; i < nbatch && st == PS_OK; ++i) st = batch_record_on(batches[i], m->st);
         hipGraph_t g = nullptr;
         const hipError_t e = hipStreamEndCapture(m->st, &g);
         if (e != hipSuccess && st == PS_OK) st = PS_ERR_HIP;
@@ -1501,7 +1458,7 @@ extern "C" int ps_batch_autotune(ps_ctx *c, ps_polygon *polys, int npoly, double
     if (!c || !polys || npoly < 1 || reps < 1) return PS_ERR_INVALID;
     const int caps[] = {2 * c->cus, 4 * c->cus, 8 * c->cus, 16 * c->cus};
     // work-item sizes: Zc tiles (direct) or dot steps (Fourier) per item
-    const int chunks_direct[] = {16, 64, 256};
+    const int chunks_direct[] = {1, 4, 16, 64};
     const int chunks_fourier[] = {1, 2, 4};
     const int divs[] = {1, 2};
     std::vector<ps_polygon> work(polys, polys + npoly);
@@ -1524,31 +1481,24 @@ extern "C" int ps_batch_autotune(ps_ctx *c, ps_polygon *polys, int npoly, double
     };
     ps_launch_params bp = default_params(c);
     double bms = 1e300;
-    ps_launch_params bdirect = bp;
-    double bdms = 1e300;
-    for (int method : {PS_METHOD_FOURIER, PS_METHOD_DIRECT})
-        for (int cap : caps)
-            for (int ch : (method == PS_METHOD_FOURIER ? std::vector<int>(std::begin(chunks_fourier),
-                                                                       std::end(chunks_fourier))
-                                                       : std::vector<int>(std::begin(chunks_direct),
-                                                                          std::end(chunks_direct))))
-                for (int dv : divs) {
-                    const ps_launch_params lp{cap, ch, dv, 0, method};
-                    double t = 0.0;
-                    const int s = trial(lp, &t);
-                    if (s != PS_OK) return s;
-                    if (t < bms) {
-                        bms = t;
-                        bp = lp;
-                    }
-                    if (method == PS_METHOD_DIRECT && t < bdms) {
-                        bdms = t;
-                        bdirect = lp;
-                    }
+    const int method = c->method;
+    for (int cap : caps)
+        for (int ch : (method == PS_METHOD_FOURIER
+                           ? std::vector<int>(std::begin(chunks_fourier), std::end(chunks_fourier))
+                           : std::vector<int>(std::begin(chunks_direct), std::end(chunks_direct))))
+            for (int dv : divs) {
+                const ps_launch_params lp{cap, ch, dv, 1, method};
+                double t = 0.0;
+                const int s = trial(lp, &t);
+                if (s != PS_OK) return s;
+                if (t < bms) {
+                    bms = t;
+                    bp = lp;
                 }
-    {  // the overlapped schedule at the best direct shape
-        ps_launch_params lp = bdirect;
-        lp.overlap = 1;
+            }
+    {  // the waves one after another at the best shape
+        ps_launch_params lp = bp;
+        lp.overlap = 0;
         double t = 0.0;
         const int s = trial(lp, &t);
         if (s != PS_OK) return s;
@@ -1588,7 +1538,7 @@ extern "C" int ps_batch_profile(ps_batch *b, int reps, double *stage_ms, double 
     if (!b || reps < 1 || !stage_ms) return PS_ERR_INVALID;
     ps_ctx *c = b->c;
     HIPCK(hipSetDevice(c->device));
-    const int N = c->N, fp = c->fp64;
+    const int fp = c->fp64;
     const hipStream_t s = c->st;
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> marks;
     int st = PS_OK;
@@ -1607,42 +1557,37 @@ extern "C" int ps_batch_profile(ps_batch *b, int reps, double *stage_ms, double 
         (void)hipEventRecord(e1, s);
     };
     for (int r = 0; r < reps && st == PS_OK; ++r) {
-        if (b->method == PS_METHOD_FOURIER) {
-            for (SpecWave &w : b->waves) {
-                const int nl = (int)w.legs.size(), ng = (int)w.groups.size(), M = w.M;
-                if (nl > 0) {
-                    timed(0, [&] {
-                        const int x = b->dscale ? run_leg_scale(fp, M, w.dlegs, nl, s) : PS_OK;
-                        return x != PS_OK ? x
-                                          : run_inverse_g(fp, M, w.dlegs, nl, 0, w.g1_tiles, 0, 0, b->grid_cap,
-                                                          b->g2_div, w.tw, s);
-                    });
-                    timed(1, [&] {
-                        return run_inverse_g(fp, M, w.dlegs, nl, 0, 0, 0, w.g2_tiles, b->grid_cap, b->g2_div, w.tw, s);
-                    });
-                }
-                timed(2, [&] {
-                    return run_spec_zf(fp, M, w.dgroups, ng, w.dlegs, 0, w.zf_tiles, b->grid_cap, w.kz_cap, w.tw, s);
+        for (Wave &w : b->waves) {
+            const int nl = (int)w.legs.size(), M = w.M;
+            if (nl > 0) {
+                timed(0, [&] {
+                    const int x = b->dscale ? run_leg_scale(fp, M, w.dlegs, nl, s) : PS_OK;
+                    return x != PS_OK ? x
+                                      : run_inverse_g(fp, M, w.dlegs, nl, 0, w.g1_tiles, 0, 0, b->grid_cap, b->g2_div,
+                                                      w.tw, s);
                 });
-                timed(3, [&] { return run_spec_fx(fp, M, w.dgroups, ng, 0, w.fx_tiles, b->grid_cap, b->g2_div, w.tw, s); });
-                timed(4, [&] {
-                    return run_spec_dot(fp, M, w.dgroups, ng, b->dmasks, b->divl, 0, w.dot_items, b->dpartials,
-                                        b->max_chunks, b->grid_cap, w.kinds, w.tw, s);
+                timed(1, [&] {
+                    return run_inverse_g(fp, M, w.dlegs, nl, 0, 0, 0, w.g2_tiles, b->grid_cap, b->g2_div, w.tw, s);
                 });
             }
-        } else {
-            const int nl = (int)b->legs.size();
-            timed(0, [&] {
-                const int x = b->dscale ? run_leg_scale(fp, N, b->dlegs, nl, s) : PS_OK;
-                return x != PS_OK ? x
-                                  : run_inverse_g(fp, N, b->dlegs, nl, 0, b->g1_tiles, 0, 0, b->grid_cap, b->g2_div,
-                                                  c->tw, s);
-            });
-            timed(1, [&] { return run_inverse_g(fp, N, b->dlegs, nl, 0, 0, 0, b->g2_tiles, b->grid_cap, b->g2_div, c->tw, s); });
-            timed(2, [&] {
-                return run_zc_fused(fp, N, b->dgroups, (int)b->groups.size(), b->dlegs, b->dsuffix, 0, b->items,
-                                    b->dpartials, b->max_chunks, b->grid_cap, b->kz_cap, c->tw, s);
-            });
+            if (b->method == PS_METHOD_FOURIER) {
+                const int ng = (int)w.sgroups.size();
+                timed(2, [&] {
+                    return run_spec_zf(fp, M, w.dsgroups, ng, w.dlegs, 0, w.zf_tiles, b->grid_cap, w.kz_cap, w.tw, s);
+                });
+                timed(3, [&] {
+                    return run_spec_fx(fp, M, w.dsgroups, ng, 0, w.fx_tiles, b->grid_cap, b->g2_div, w.tw, s);
+                });
+                timed(4, [&] {
+                    return run_spec_dot(fp, M, w.dsgroups, ng, b->dmasks, b->divl, 0, w.dot_items, b->dpartials,
+                                        b->max_chunks, b->grid_cap, w.kinds, w.tw, s);
+                });
+            } else {
+                timed(2, [&] {
+                    return run_zc_fused(fp, M, w.dgroups, (int)w.groups.size(), w.dlegs, w.dsuffix, 0, w.items,
+                                        b->dpartials, b->max_chunks, b->grid_cap, w.kz_cap, w.tw, s);
+                });
+            }
         }
         timed(b->nstages - 1, [&] {
             return run_reduce_partials(b->dpartials, b->max_chunks, b->dcounts, 0, b->nout, b->dsum, b->dsum + b->nout, s);

@@ -35,6 +35,10 @@ struct LegDev {
 
 // A batch of polygons sharing a prefix product: output s of the group is
 // sum_x P(x) * D_{suffix[s]}(x) (and the same for I), P = prod of prefix legs.
+// Every leg is band limited to |m_i| <= K, so the sum over the spectrum's
+// grid equals (N_spectrum / N)^3 times the sum over any grid N > K_A + K_B +
+// K_t (no product term aliases onto m = 0): each group runs on the smallest
+// supported such grid.
 struct GroupDev {
     int npre;            // 0, 1 or 2 prefix legs
     int pre_sq;          // prefix leg preA enters squared (k1 == k2)
@@ -46,6 +50,7 @@ struct GroupDev {
     int chunk_tiles;     // Zc tiles per work item
     int pad_;
     int64_t item_begin;  // first work item of this group
+    double fscale;       // (N_spectrum / N)^3: the group runs on an alias-free grid N <= N_spectrum
 };
 
 // A polygon group evaluated in Fourier space.  Output o of the group is the

@@ -497,9 +497,12 @@ __global__ void __launch_bounds__((ZcCfg<T, N>::BT)) k_inv_zc_fused(const GroupD
             const LegDev &Ls = legs[suffix[G.suf_begin + o]];
             const double st = Ls.scale ? *Ls.scale : 1.0;
             f *= G.suf_sq ? st * st : st;
+            // the grid ratio (N_spectrum / N)^3 of a group run on a reduced grid
+            const double g = G.fscale;
+            f *= g;
             const int64_t po = (int64_t)(G.out_begin + o) * max_chunks + chunk;
             partials[2 * po + 0] = make_double2(a.s * f, a.c * f);
-            partials[2 * po + 1] = make_double2(c.s, c.c);
+            partials[2 * po + 1] = make_double2(c.s * g, c.c * g);
         }
         __syncthreads();
     }

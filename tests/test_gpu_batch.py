@@ -106,7 +106,8 @@ def test_batch_matches_plain_path(gpu_lib):
 
 
 def test_autotune_and_params(gpu_lib):
-    """Tuned launch shapes change only summation order, not results."""
+    """Tuned launch shapes change only summation order, not results; the
+    Fourier method gives the direct method's values."""
     P = gpu_lib
     N, L = 32, 100.0
     f = lognormal_field(N, 77)
@@ -114,49 +115,48 @@ def test_autotune_and_params(gpu_lib):
     polys = _polys_triangles(kf, [(4.0, 4.0)], [0.2, 1.0, 2.0], 0)
     with P.Context(N, L, P.PREC_FP64) as ctx:
         ctx.load_field(f)
+        assert ctx.method == P.METHOD_DIRECT
         b0 = ctx.batch(polys, 1.0, 0, params={"grid_cap": 64, "chunk_tiles": 1, "g2_div": 2})
         v0, _, _ = b0.execute()
-        assert b0.params == {"grid_cap": 64, "chunk_tiles": 1, "g2_div": 2, "overlap": 0, "method": 0}
+        assert b0.params == {"grid_cap": 64, "chunk_tiles": 1, "g2_div": 2, "overlap": 1,
+                             "method": P.METHOD_DIRECT}
         b0.close()
-        # the direct method with the two-stream schedule sums in the same order as without
-        pd = {"grid_cap": 64, "chunk_tiles": 1, "g2_div": 2, "method": P.METHOD_DIRECT}
-        vd = []
-        for ov in (0, 1):
-            bo = ctx.batch(polys, 1.0, 0, params=dict(pd, overlap=ov))
-            vd.append(bo.execute()[0])
-            assert bo.params["overlap"] == ov and bo.params["method"] == P.METHOD_DIRECT
-            bo.close()
-        np.testing.assert_array_equal(vd[1], vd[0])
-        np.testing.assert_allclose(vd[0], v0, rtol=1e-11)
         best, ms = ctx.autotune(polys, 1.0, 0, reps=1)
-        assert ms > 0 and best["grid_cap"] > 0
+        assert ms > 0 and best["grid_cap"] > 0 and best["method"] == P.METHOD_DIRECT
         b1 = ctx.batch(polys, 1.0, 0)
         assert b1.params == best
         v1, _, _ = b1.execute()
         b1.close()
+        ctx.set_method(P.METHOD_FOURIER)
+        bf = ctx.batch(polys, 1.0, 0)
+        assert bf.params["method"] == P.METHOD_FOURIER
+        vf, _, _ = bf.execute()
+        bf.close()
     np.testing.assert_allclose(v1, v0, rtol=1e-13)
+    np.testing.assert_allclose(vf, v0, rtol=1e-11)
 
 
 @pytest.mark.parametrize("prec_name", ["PREC_FP64", "PREC_MIXED"])
 def test_overlap_schedule_matches(gpu_lib, prec_name):
-    """Several polygon groups (two triangle prefixes, one power-spectrum group),
-    direct method: the two-stream schedule gives the single-stream values bit
-    for bit."""
+    """Several polygon groups on different transform grids (three triangle
+    prefixes, power spectra): the waves run side by side on branch streams or
+    one after another, with the same values bit for bit, in both methods."""
     P = gpu_lib
     N, L = 64, 100.0
     f = lognormal_field(N, 41)
     kf = 2 * math.pi / L
     polys = _polys_triangles(kf, [(7.96, 7.96), (5.0, 9.0), (3.0, 3.0)], [0.4, 1.3, 2.2], 2)
     polys += [{"n": 2, "k": [m * kf] * 2, "kmin": [m * kf] * 2, "kmax": [m * kf] * 2} for m in range(1, 12)]
-    out = {}
-    with P.Context(N, L, getattr(P, prec_name)) as ctx:
-        ctx.load_field(f)
-        for ov in (0, 1):
-            b = ctx.batch(polys, 1.0, 2, params={"grid_cap": 512, "chunk_tiles": 16, "g2_div": 1, "overlap": ov,
-                                                 "method": P.METHOD_DIRECT})
-            out[ov] = b.execute()
-            again = b.execute()
-            b.close()
-            np.testing.assert_array_equal(out[ov][0], again[0])
-    np.testing.assert_array_equal(out[1][0], out[0][0])
-    np.testing.assert_array_equal(out[1][1], out[0][1])
+    for method in (P.METHOD_DIRECT, P.METHOD_FOURIER):
+        out = {}
+        with P.Context(N, L, getattr(P, prec_name)) as ctx:
+            ctx.load_field(f)
+            for ov in (0, 1):
+                b = ctx.batch(polys, 1.0, 2, params={"grid_cap": 512, "chunk_tiles": 16 if method else 2,
+                                                     "g2_div": 1, "overlap": ov, "method": method})
+                out[ov] = b.execute()
+                again = b.execute()
+                b.close()
+                np.testing.assert_array_equal(out[ov][0], again[0])
+        np.testing.assert_array_equal(out[1][0], out[0][0])
+        np.testing.assert_array_equal(out[1][1], out[0][1])

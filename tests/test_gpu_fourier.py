@@ -1,5 +1,5 @@
-"""The Fourier-space evaluation of a batch (the default method) against the
-direct one and the oracle.
+"""The Fourier-space evaluation of a batch (METHOD_FOURIER, opt-in) against
+the direct method and the oracle.
 
 Per group of polygons sharing their first legs, the product of those legs is
 forward transformed once and every last leg becomes a sum over its shell
