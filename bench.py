@@ -8,12 +8,16 @@ Mpc/h; 4 pairs k1 = k2 in {0.1, 0.4, 0.7, 1.0} h/Mpc x 32 theta bins
 cosThetaBinwidth = 0.05 (run.c:68-81), kbinwidth = 1.
 
 A step is one evaluation of all 128 triangles (B and Npolygons) by the batched
-mixed-precision path (FP32 FFTs, FP64 compensated sums; the autotuned method,
-normally the Fourier-space one: per k1 = k2 pair the squared leg is inverse
-transformed, squared and forward transformed once on an alias-free reduced
-grid, and each triangle is a sum over its k3 shell), replayed from its captured
-hipGraph, with the field's spectrum already resident in HBM.  Beside it the
+mixed-precision path (FP32 FFTs, FP64 compensated sums) with the reference's
+method: every leg's filtered field inverse transformed, the pointwise product
+and its sum.  Each k1 = k2 group runs on the smallest grid M on which that
+product does not alias (M > K1 + K2 + K3 in cells; M = 24, 96, 192, 256 here,
+the sums scaled by (512/M)^3, exact).  The step replays the batch's captured
+hipGraph with the field's spectrum already resident in HBM.  Beside it the
 script measures:
+  * the opt-in Fourier-space method (same sums in another order: the product
+    of the k1, k2 legs forward transformed once, each triangle a sum over its
+    k3 shell), reported under "fourier_method";
   * the reference-shaped FP64 path (ps_polyspectrum, one triangle per call)
     on a sample of triangles, and the batched FP64 path on all 128 (the
     accuracy reference for the mixed path);
@@ -178,6 +182,34 @@ def adjoint_timing(P, N):
             "grad_rms": float(np.sqrt(np.mean(g ** 2)))}
 
 
+def fourier_method(P, N, L, field, polys, steps, warmup, vals64, autotune):
+    """The same sweep by the Fourier-space method (Context.set_method)."""
+    print("[bench] Fourier-space method", file=sys.stderr, flush=True)
+    ctx = P.Context(N, L, P.PREC_MIXED)
+    ctx.load_field(field)
+    ctx.set_method(P.METHOD_FOURIER)
+    if autotune:
+        ctx.autotune(polys, 1.0, P.BINNING["GIVEN_BINNING"], reps=2)
+    b = ctx.batch(polys, 1.0, P.BINNING["GIVEN_BINNING"])
+    for _ in range(warmup):
+        b.execute()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        v, npol, _ = b.execute()
+    el = time.perf_counter() - t0
+    ms, by, names = b.profile(reps=3)
+    params = b.params
+    b.close()
+    ctx.close()
+    rel = np.abs(v - vals64) / np.abs(vals64)
+    return {"triangles_per_s": round(len(polys) * steps / el, 1), "ms_per_step": round(1e3 * el / steps, 4),
+            "stages": {names[i]: {"ms": round(float(ms[i]), 4), "GB": round(float(by[i]) / 1e9, 4)}
+                       for i in range(len(names))},
+            "accuracy_vs_fp64_direct": {"rel_err_max": float(rel.max()), "median": float(np.median(rel)),
+                                        "fraction_le_1e-5": float(np.mean(rel <= 1e-5))},
+            "launch_params": params}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -187,6 +219,8 @@ def main():
     ap.add_argument("--plain-samples", type=int, default=4)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--method", choices=("direct", "fourier"), default="direct",
+                    help="method of the measured step (direct: the reference's; fourier: opt-in, same sums)")
     ap.add_argument("--skip-extras", action="store_true",
                     help="skip the realisation-batch and adjoint measurements")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -219,6 +253,8 @@ def main():
     # ---- mixed-precision batched path (the measured value)
     ctx = P.Context(N, L, P.PREC_MIXED)
     ctx.load_field(field)
+    if args.method == "fourier":
+        ctx.set_method(P.METHOD_FOURIER)
     tuned, tuned_ms = None, None
     if not args.no_autotune:
         print("[bench] autotuning the launch shape", file=sys.stderr, flush=True)
@@ -239,7 +275,8 @@ def main():
     stage_ms, stage_bytes, names = batch.profile(reps=3)
     dom = int(np.argmax(stage_ms))
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", "bispectrum512_hbm_traffic.json")
+    tfile = os.path.join(ROOT, "profiles", "bispectrum512_hbm_traffic.json" if args.method == "direct"
+                         else "bispectrum512_fourier_hbm_traffic.json")
     if N == 512 and os.path.exists(tfile):
         # HBM bytes per step of the dominant pass (all its launches) from the
         # committed FETCH_SIZE/WRITE_SIZE summary
@@ -286,7 +323,10 @@ def main():
         "data": "synthetic (seeded log-normal field generated on the host)",
         "config": {"workload": "equilateral-pair bispectrum sweep: 512^3 log-normal DENS field, L=300 Mpc/h, "
                                "k1=k2 in {0.1,0.4,0.7,1.0} h/Mpc x 32 theta, GIVEN_BINNING dcos=0.05, kbinwidth=1",
-                   "grid": N, "boxsize": L, "triangles_per_step": ntri},
+                   "grid": N, "boxsize": L, "triangles_per_step": ntri,
+                   "method": ("direct: every leg inverse transformed, product and sum, per group on its "
+                              "alias-free grid") if args.method == "direct" else
+                             "fourier: prefix product forward transformed, shell sums per last leg"},
         "roofline": {"bound": "hbm", "kernel": names[dom],
                      "achieved": round(float(stage_bytes[dom] / stage_ms[dom] / 1e6), 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -311,6 +351,9 @@ def main():
         "launch_params": {"used": used_params, "autotuned": tuned is not None,
                           "autotune_best_ms": tuned_ms},
     }
+    if not args.skip_extras and args.method == "direct":
+        out["fourier_method"] = fourier_method(P, N, L, field, polys, args.steps, args.warmup, vals64,
+                                               not args.no_autotune)
     if not args.skip_extras:
         out["realisations"] = realisation_batch(P)
         out["adjoint"] = adjoint_timing(P, N)
