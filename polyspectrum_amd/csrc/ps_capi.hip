@@ -385,7 +385,7 @@ static int plain_leg(ps_ctx *c, const MaskParams &mp, const void *spec, int mode
     const int tl = g_tile_lines(N, c->fp64);
     const int64_t g1 = (int64_t)(N / 2 + 1) * ((N + tl - 1) / tl);
     const int64_t g2 = 2 * (int64_t)(N / 2 + 1) * (N / tile_rows(N, c->fp64));
-    PSCK(run_inverse_g(c->fp64, N, c->dleg, 1, 0, g1, 0, g2, 1 << 30, 1, c->tw, c->st));
+    PSCK(run_inverse_g(c->fp64, N, c->dleg, 1, 0, g1, 0, g2, 1 << 30, 1, 0, c->tw, c->st));
     PSCK(run_zc_plain(c->fp64, N, L.g2d, L.g2i, N / 2, spec ? c->accD : nullptr, c->accI, mode, c->partials, c->tw,
                       c->st));
     // the next leg overwrites the host-side descriptor buffer: keep order
@@ -457,6 +457,12 @@ struct Wave {
     std::vector<size_t> leg_off;         // arena offsets of their G1/G2 buffers
     LegDev *dlegs = nullptr;
     int64_t g1_tiles = 0, g2_tiles = 0;
+    // direct method, D parts only (dmode): the legs with D-only tiles, and the
+    // legs without their spectrum (I parts) for the sums taken at planning
+    int dmode = 0;
+    LegDev *dlegs_d = nullptr, *dlegs_i = nullptr;
+    int64_t g1_tiles_d = 0, g2_tiles_d = 0;
+    std::vector<int64_t> g1b_d, g2b_d;   // per leg: first D-only G1 / G2 tile
     int kz_cap = 1;                      // 1 + largest Kz of the wave's legs
     size_t bytes = 0;
     // direct method
@@ -533,7 +539,7 @@ extern "C" int ps_batch_destroy(ps_batch *b)
     if (b->fork) (void)hipEventDestroy(b->fork);
     for (void *p : b->allocs) (void)hipFree(p);
     for (Wave &w : b->waves) {
-        void *wb[] = {w.dlegs, w.dgroups, w.dsuffix, w.dsgroups};
+        void *wb[] = {w.dlegs, w.dgroups, w.dsuffix, w.dsgroups, w.dlegs_d, w.dlegs_i};
         for (void *p : wb) if (p) (void)hipFree(p);
     }
     void *bufs[] = {b->dcounts, b->dpartials, b->dsum, b->dscale, b->dmasks, b->divl, b->arena};
@@ -549,13 +555,22 @@ static bool same_mask(const MaskParams &a, const MaskParams &b)
 
 // The passes of one wave on stream ws: the inverse transforms of its legs,
 // then (direct) the fused product-reduce, or (Fourier) Zf, Fx and dot.
+// Direct waves in dmode transform the D parts only.
 static int record_wave(ps_batch *b, Wave &w, hipStream_t ws)
 {
     ps_ctx *c = b->c;
     const int M = w.M, nl = (int)w.legs.size();
+    if (b->method == PS_METHOD_DIRECT && w.dmode) {
+        PSCK(run_inverse_g(c->fp64, M, w.dlegs_d, nl, 0, w.g1_tiles_d, 0, w.g2_tiles_d, b->grid_cap, b->g2_div, 1,
+                           w.tw, ws));
+        PSCK(run_zc_fused(c->fp64, M, w.dgroups, (int)w.groups.size(), w.dlegs_d, w.dsuffix, 0, w.items,
+                          b->dpartials, b->max_chunks, b->grid_cap, w.kz_cap, 1, w.tw, ws));
+        return PS_OK;
+    }
     if (nl > 0) {
         if (b->dscale) PSCK(run_leg_scale(c->fp64, M, w.dlegs, nl, ws));
-        PSCK(run_inverse_g(c->fp64, M, w.dlegs, nl, 0, w.g1_tiles, 0, w.g2_tiles, b->grid_cap, b->g2_div, w.tw, ws));
+        PSCK(run_inverse_g(c->fp64, M, w.dlegs, nl, 0, w.g1_tiles, 0, w.g2_tiles, b->grid_cap, b->g2_div, 0, w.tw,
+                           ws));
     }
     if (b->method == PS_METHOD_FOURIER) {
         const int ng = (int)w.sgroups.size();
@@ -565,8 +580,26 @@ static int record_wave(ps_batch *b, Wave &w, hipStream_t ws)
                           b->max_chunks, b->grid_cap, w.kinds, w.tw, ws));
     } else {
         PSCK(run_zc_fused(c->fp64, M, w.dgroups, (int)w.groups.size(), w.dlegs, w.dsuffix, 0, w.items,
-                          b->dpartials, b->max_chunks, b->grid_cap, w.kz_cap, w.tw, ws));
+                          b->dpartials, b->max_chunks, b->grid_cap, w.kz_cap, 0, w.tw, ws));
     }
+    return PS_OK;
+}
+
+// Direct method: the I sums (polygon counts, field independent) of every
+// group, from the legs' bare masks, once when the batch is planned.
+static int batch_i_sums(ps_batch *b)
+{
+    ps_ctx *c = b->c;
+    for (Wave &w : b->waves) {
+        const int nl = (int)w.legs.size();
+        PSCK(run_inverse_g(c->fp64, w.M, w.dlegs_i, nl, 0, w.g1_tiles, 0, w.g2_tiles, b->grid_cap, b->g2_div, 0,
+                           w.tw, c->st));
+        PSCK(run_zc_fused(c->fp64, w.M, w.dgroups, (int)w.groups.size(), w.dlegs_i, w.dsuffix, 0, w.items,
+                          b->dpartials, b->max_chunks, b->grid_cap, w.kz_cap, 0, w.tw, c->st));
+    }
+    PSCK(run_reduce_partials(b->dpartials, b->max_chunks, b->dcounts, 0, b->nout, b->dsum, b->dsum + b->nout,
+                             c->st));
+    HIPCK(hipStreamSynchronize(c->st));
     return PS_OK;
 }
 
@@ -588,7 +621,9 @@ static int batch_record_on(ps_batch *b, hipStream_t s)
     } else {
         for (Wave &w : b->waves) PSCK(record_wave(b, w, s));
     }
-    PSCK(run_reduce_partials(b->dpartials, b->max_chunks, b->dcounts, 0, b->nout, b->dsum, b->dsum + b->nout, s));
+    // direct method: the I sums were taken when the batch was planned
+    double *outI = (b->method == PS_METHOD_DIRECT) ? nullptr : b->dsum + b->nout;
+    PSCK(run_reduce_partials(b->dpartials, b->max_chunks, b->dcounts, 0, b->nout, b->dsum, outI, s));
     return PS_OK;
 }
 static int batch_record(ps_batch *b) { return batch_record_on(b, b->c->st); }
@@ -846,18 +881,29 @@ static int plan_waves(ps_batch *b, std::vector<int> &counts)
             }
             w.tw = it->second;
         }
+        // direct waves transform the D parts only where the fused pass can
+        // pair rows (an even number of rows per wavefront)
+        w.dmode = (!fourier && zc_dmode_supported(M, fp64)) ? 1 : 0;
+        const int parts = w.dmode ? 1 : 2;
+        const int64_t g2_per_kz = M / (tile_rows(M, fp64) / b->g2_div);
         for (LegDev &L : w.legs) {
             const int K = L.Kz;
             w.kz_cap = std::max(w.kz_cap, K + 1);
             L.g1_begin = w.g1_tiles;
             L.g2_begin = w.g2_tiles;
             w.g1_tiles += (int64_t)(K + 1) * ((L.Bx + tl - 1) / tl);
-            w.g2_tiles += 2 * (int64_t)(K + 1) * (M / (tile_rows(M, fp64) / b->g2_div));
-            // G1 reads the spectrum cells of the band box and writes the D,I band
-            // lines; G2 reads them and writes the D,I planes
+            w.g2_tiles += 2 * (int64_t)(K + 1) * g2_per_kz;
+            if (w.dmode) {
+                w.g1b_d.push_back(w.g1_tiles_d);
+                w.g2b_d.push_back(w.g2_tiles_d);
+                w.g1_tiles_d += (int64_t)(K + 1) * ((L.Bx + 2 * tl - 1) / (2 * tl));
+                w.g2_tiles_d += (int64_t)(K + 1) * g2_per_kz;
+            }
+            // G1 reads the spectrum cells of the band box and writes the band
+            // lines (D, I or D only); G2 reads them and writes the planes
             const double n1 = (double)(K + 1) * L.Bx * M, n2 = (double)(K + 1) * M * M;
-            b->stage_bytes[0] += (double)cs * ((double)(K + 1) * L.Bx * L.Bx + 2.0 * n1);
-            b->stage_bytes[1] += (double)cs * (2.0 * n1 + 2.0 * n2);
+            b->stage_bytes[0] += (double)cs * ((double)(K + 1) * L.Bx * L.Bx + parts * n1);
+            b->stage_bytes[1] += (double)cs * (parts * n1 + parts * n2);
         }
         if (fourier) {
             const int zrows = spec_zf_rows(M, fp64);
@@ -901,7 +947,7 @@ static int plan_waves(ps_batch *b, std::vector<int> &counts)
                 if (G.npre >= 1) planes += w.legs[G.preA].Kz + 1;
                 if (G.npre == 2) planes += w.legs[G.preB].Kz + 1;
                 for (int o = 0; o < G.nsuf; ++o) planes += w.legs[w.suffix[G.suf_begin + o]].Kz + 1;
-                b->stage_bytes[2] += 2.0 * cs * (double)planes * M * M;
+                b->stage_bytes[2] += parts * cs * (double)planes * M * M;
             }
         }
         arena = std::max(arena, w.bytes);
@@ -939,6 +985,28 @@ static int plan_waves(ps_batch *b, std::vector<int> &counts)
         if (!w.legs.empty()) {
             PCK(hipMalloc(&w.dlegs, sizeof(LegDev) * w.legs.size()));
             PCK(hipMemcpy(w.dlegs, w.legs.data(), sizeof(LegDev) * w.legs.size(), hipMemcpyHostToDevice));
+        }
+        if (!fourier && !w.legs.empty()) {
+            // I parts only (bare masks), for the sums taken at planning
+            std::vector<LegDev> li(w.legs);
+            for (LegDev &L : li) {
+                L.spec = nullptr;
+                L.g1d = L.g2d = nullptr;
+                L.scale = nullptr;
+            }
+            PCK(hipMalloc(&w.dlegs_i, sizeof(LegDev) * li.size()));
+            PCK(hipMemcpy(w.dlegs_i, li.data(), sizeof(LegDev) * li.size(), hipMemcpyHostToDevice));
+            if (w.dmode) {
+                // D parts only, with their own tiles
+                std::vector<LegDev> ld(w.legs);
+                for (size_t i = 0; i < ld.size(); ++i) {
+                    ld[i].g1_begin = w.g1b_d[i];
+                    ld[i].g2_begin = w.g2b_d[i];
+                    ld[i].scale = nullptr;
+                }
+                PCK(hipMalloc(&w.dlegs_d, sizeof(LegDev) * ld.size()));
+                PCK(hipMemcpy(w.dlegs_d, ld.data(), sizeof(LegDev) * ld.size(), hipMemcpyHostToDevice));
+            }
         }
         if (fourier) {
             for (size_t g = 0; g < w.sgroups.size(); ++g) {
@@ -1149,6 +1217,10 @@ extern "C" int ps_batch_create_ex(ps_ctx *c, ps_polygon *polys, int npoly, doubl
     BCK(hipMemcpy(b->dcounts, counts.data(), sizeof(int) * counts.size(), hipMemcpyHostToDevice));
     BCK(hipEventCreate(&b->ev0));
     BCK(hipEventCreate(&b->ev1));
+    if (b->method == PS_METHOD_DIRECT) {
+        st = batch_i_sums(b);
+        if (st != PS_OK) goto fail;
+    }
     // 5. capture the launch sequence once
     if (!getenv("PS_NO_GRAPH")) {
         BCK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
@@ -1559,15 +1631,28 @@ extern "C" int ps_batch_profile(ps_batch *b, int reps, double *stage_ms, double 
     for (int r = 0; r < reps && st == PS_OK; ++r) {
         for (Wave &w : b->waves) {
             const int nl = (int)w.legs.size(), M = w.M;
+            if (b->method == PS_METHOD_DIRECT && w.dmode) {
+                timed(0, [&] {
+                    return run_inverse_g(fp, M, w.dlegs_d, nl, 0, w.g1_tiles_d, 0, 0, b->grid_cap, b->g2_div, 1, w.tw, s);
+                });
+                timed(1, [&] {
+                    return run_inverse_g(fp, M, w.dlegs_d, nl, 0, 0, 0, w.g2_tiles_d, b->grid_cap, b->g2_div, 1, w.tw, s);
+                });
+                timed(2, [&] {
+                    return run_zc_fused(fp, M, w.dgroups, (int)w.groups.size(), w.dlegs_d, w.dsuffix, 0, w.items,
+                                        b->dpartials, b->max_chunks, b->grid_cap, w.kz_cap, 1, w.tw, s);
+                });
+                continue;
+            }
             if (nl > 0) {
                 timed(0, [&] {
                     const int x = b->dscale ? run_leg_scale(fp, M, w.dlegs, nl, s) : PS_OK;
                     return x != PS_OK ? x
                                       : run_inverse_g(fp, M, w.dlegs, nl, 0, w.g1_tiles, 0, 0, b->grid_cap, b->g2_div,
-                                                      w.tw, s);
+                                                      0, w.tw, s);
                 });
                 timed(1, [&] {
-                    return run_inverse_g(fp, M, w.dlegs, nl, 0, 0, 0, w.g2_tiles, b->grid_cap, b->g2_div, w.tw, s);
+                    return run_inverse_g(fp, M, w.dlegs, nl, 0, 0, 0, w.g2_tiles, b->grid_cap, b->g2_div, 0, w.tw, s);
                 });
             }
             if (b->method == PS_METHOD_FOURIER) {
@@ -1585,12 +1670,13 @@ extern "C" int ps_batch_profile(ps_batch *b, int reps, double *stage_ms, double 
             } else {
                 timed(2, [&] {
                     return run_zc_fused(fp, M, w.dgroups, (int)w.groups.size(), w.dlegs, w.dsuffix, 0, w.items,
-                                        b->dpartials, b->max_chunks, b->grid_cap, w.kz_cap, w.tw, s);
+                                        b->dpartials, b->max_chunks, b->grid_cap, w.kz_cap, 0, w.tw, s);
                 });
             }
         }
         timed(b->nstages - 1, [&] {
-            return run_reduce_partials(b->dpartials, b->max_chunks, b->dcounts, 0, b->nout, b->dsum, b->dsum + b->nout, s);
+            double *outI = (b->method == PS_METHOD_DIRECT) ? nullptr : b->dsum + b->nout;
+            return run_reduce_partials(b->dpartials, b->max_chunks, b->dcounts, 0, b->nout, b->dsum, outI, s);
         });
     }
     if (hipStreamSynchronize(s) != hipSuccess && st == PS_OK) st = PS_ERR_HIP;

@@ -95,15 +95,19 @@ int fused_tile_rows(int N, int fp64);
 int g_tile_lines(int N, int fp64);
 
 int run_forward(int fp64, int N, const void *field, int field_is_double, void *H, const void *tw, hipStream_t st);
-// G1 over tiles [g1a, g1b), G2 over tiles [g2a, g2b) of the leg table
+// G1 over tiles [g1a, g1b), G2 over tiles [g2a, g2b) of the leg table;
+// dmode: G1 tiles of 2 TLC D lines and no I lines (the leg table's tile
+// counts must match: D-part G2 tiles only)
 int run_inverse_g(int fp64, int N, const LegDev *legs, int nlegs, int64_t g1a, int64_t g1b, int64_t g2a, int64_t g2b,
-                  int grid_cap, int g2_div, const void *tw, hipStream_t st);
+                  int grid_cap, int g2_div, int dmode, const void *tw, hipStream_t st);
 int run_zc_plain(int fp64, int N, const void *gd, const void *gi, int Kz, double *accD, double *accI, int mode,
                  double2 *partials, const void *tw, hipStream_t st);
-// Zc over work items [i0, i1)
+// Zc over work items [i0, i1); dmode: D parts only, two rows per line (the I
+// sums are taken once per batch), where zc_dmode_supported(N, fp64)
 int run_zc_fused(int fp64, int N, const GroupDev *groups, int ngroups, const LegDev *legs, const int *suffix,
-                 int64_t i0, int64_t i1, double2 *partials, int max_chunks, int grid_cap, int kz_cap, const void *tw,
-                 hipStream_t st);
+                 int64_t i0, int64_t i1, double2 *partials, int max_chunks, int grid_cap, int kz_cap, int dmode,
+                 const void *tw, hipStream_t st);
+int zc_dmode_supported(int N, int fp64);
 int run_reduce_partials(const double2 *partials, int64_t stride, const int *counts, int64_t uniform_count,
                         int64_t nout, double *outD, double *outI, hipStream_t st);
 int run_fill(double *p, int64_t n, double v, hipStream_t st);

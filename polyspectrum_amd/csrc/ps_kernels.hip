@@ -104,27 +104,29 @@ __global__ void __launch_bounds__(NT) k_c2c_strided(cplx<T> *__restrict__ data, 
 
 // ================================================================ inverse
 
-// G1: masked gather + C2C along ky.  Tile = (leg, kz, band tile of TLC kx
-// lines); LDS holds TLC D-lines (mask * spectrum) and TLC I-lines (mask).
+// G1: masked gather + C2C along ky.  Tile = (leg, kz, band tile of kx
+// lines); LDS holds TLC D-lines (mask * spectrum) and TLC I-lines (mask), or
+// (dmode: D parts only) 2 TLC D-lines.
 template <typename T, int N>
 __global__ void __launch_bounds__(NT) k_inv_g1(const LegDev *__restrict__ legs, int nlegs, int64_t t0, int64_t t1,
-                                               const cplx<T> *__restrict__ tw)
+                                               int dmode, const cplx<T> *__restrict__ tw)
 {
     constexpr int TLC = tile_lines<T, N>() / 2;
     constexpr int TL2 = 2 * TLC;
     __shared__ cplx<T> lds[N * TL2];
     const int tid = threadIdx.x;
+    const int lpt = dmode ? TL2 : TLC;  // kx lines per tile
     for (int64_t tile = t0 + blockIdx.x; tile < t1; tile += gridDim.x) {
         const int li = find_owner(legs, nlegs, tile, &LegDev::g1_begin);
         const LegDev &L = legs[li];
-        const int nbt = (L.Bx + TLC - 1) / TLC;
+        const int nbt = (L.Bx + lpt - 1) / lpt;
         const int64_t local = tile - L.g1_begin;
         const int kz = (int)(local / nbt);
-        const int j0 = (int)(local % nbt) * TLC;
+        const int j0 = (int)(local % nbt) * lpt;
         const int mz = lattice(kz, N);
         const cplx<T> *spec = reinterpret_cast<const cplx<T> *>(L.spec);
         const T inv_scale = L.scale ? (T)(1.0 / *L.scale) : (T)1;
-        for (int idx = tid; idx < TLC * N; idx += NT) {
+        for (int idx = tid; idx < lpt * N; idx += NT) {
             const int l = idx / N, ky = idx % N;
             const int j = j0 + l;
             cplx<T> d = mk<T>(0, 0), u = mk<T>(0, 0);
@@ -146,19 +148,19 @@ __global__ void __launch_bounds__(NT) k_inv_g1(const LegDev *__restrict__ legs, 
                 }
             }
             lds[lidx<TL2>(ky, l)] = d;
-            lds[lidx<TL2>(ky, l + TLC)] = u;
+            if (!dmode) lds[lidx<TL2>(ky, l + TLC)] = u;
         }
         __syncthreads();
         fft_lds<T, N, TL2, +1>(lds, tw, tid);
         cplx<T> *gd = reinterpret_cast<cplx<T> *>(L.g1d);
         cplx<T> *gi = reinterpret_cast<cplx<T> *>(L.g1i);
-        for (int idx = tid; idx < TLC * N; idx += NT) {
+        for (int idx = tid; idx < lpt * N; idx += NT) {
             const int l = idx / N, y = idx % N;
             const int j = j0 + l;
             if (j < L.Bx) {
                 const int64_t o = ((int64_t)kz * L.Bx + j) * N + y;
                 if (gd) gd[o] = lds[lidx<TL2>(y, l)];
-                gi[o] = lds[lidx<TL2>(y, l + TLC)];
+                if (!dmode) gi[o] = lds[lidx<TL2>(y, l + TLC)];
             }
         }
         __syncthreads();
@@ -305,29 +307,37 @@ __global__ void __launch_bounds__(NT) k_inv_zc_plain(const cplx<T> *__restrict__
 // a chunk of consecutive tiles of one group; its partial sums go to
 // partials[out][chunk] as (sum, compensation).  Dynamic LDS: two staging
 // buffers of kz_cap planes (kz_cap = 1 + the largest Kz of the batch).
-template <typename T, int N> struct ZcCfg {
-    static constexpr int LW = wave_lines<T, N>();
+//
+// D-only mode (DM): the I parts (the inverse transforms of the bare masks)
+// do not depend on the field, so their sums are taken once when a batch is
+// planned; every execution then transforms the D parts only, two adjacent y
+// rows per complex line (D(a) + i D(b)), so half as many lines per tile.
+template <typename T, int N, bool DM = false> struct ZcCfg {
+    static constexpr int LW0 = wave_lines<T, N>();  // y rows per wavefront
+    static constexpr int LW = DM ? LW0 / 2 : LW0;   // transformed lines per wavefront
     // 4 wavefronts: measured faster at 512^3 than 8 (whose 16-row tiles read
     // whole 128 B lines per plane but synchronise twice as many wavefronts)
     static constexpr int NWV = 4;
     static constexpr int BT = 64 * NWV;  // threads per workgroup
-    static constexpr int ROWS = NWV * LW;
-    static constexpr int SP = ROWS + 1;  // staging pitch
+    static constexpr int ROWS = NWV * LW0;  // y rows of a tile
+    static constexpr int SL = NWV * LW;     // staged lines of a tile
+    static constexpr int SP = SL + 1;       // staging pitch
     static constexpr int NH = N / 2 + 1;
     // staged values per thread when prefetching (legs with more planes are
     // staged without overlap)
-    static constexpr int MPT = ((NH * ROWS + BT - 1) / BT < 4) ? (NH * ROWS + BT - 1) / BT : 4;
+    static constexpr int MPT = ((NH * SL + BT - 1) / BT < 4) ? (NH * SL + BT - 1) / BT : 4;
 };
+template <typename T, int N> constexpr bool zc_dmode_ok() { return wave_lines<T, N>() % 2 == 0; }
 
-template <typename T, int N>
-__global__ void __launch_bounds__((ZcCfg<T, N>::BT)) k_inv_zc_fused(const GroupDev *__restrict__ groups, int ngroups,
-                                                     const LegDev *__restrict__ legs,
-                                                     const int *__restrict__ suffix, int64_t i0, int64_t i1,
-                                                     double2 *__restrict__ partials, int max_chunks,
-                                                     const cplx<T> *__restrict__ twg, int kz_cap, int nbuf)
+template <typename T, int N, bool DM>
+__global__ void __launch_bounds__((ZcCfg<T, N, DM>::BT)) k_inv_zc_fused(const GroupDev *__restrict__ groups,
+                                                         int ngroups, const LegDev *__restrict__ legs,
+                                                         const int *__restrict__ suffix, int64_t i0, int64_t i1,
+                                                         double2 *__restrict__ partials, int max_chunks,
+                                                         const cplx<T> *__restrict__ twg, int kz_cap, int nbuf)
 {
-    using C = ZcCfg<T, N>;
-    constexpr int LW = C::LW, NWV = C::NWV, ROWS = C::ROWS, SP = C::SP, MPT = C::MPT, BT = C::BT;
+    using C = ZcCfg<T, N, DM>;
+    constexpr int LW = C::LW, NWV = C::NWV, ROWS = C::ROWS, SL = C::SL, SP = C::SP, MPT = C::MPT, BT = C::BT;
     constexpr int LP = wave_pitch<N>();
     using WO = WaveOut<T, N, LW>;
     extern __shared__ unsigned char dyn_lds[];
@@ -358,29 +368,37 @@ __global__ void __launch_bounds__((ZcCfg<T, N>::BT)) k_inv_zc_fused(const GroupD
         cplx<T> pfD[MPT], pfI[MPT];
         // step (t, q) = (tile, leg): registers for up to MPT*NT staged values,
         // the rest (legs with many planes) copied directly at store time
+        // staged line l of a tile: X = D and Y = I of row y0 + l, or (DM)
+        // X = D of row y0 + 2l and Y = D of row y0 + 2l + 1
+        auto load_xy = [&](const LegDev &L, int kz, int x, int y0, int l, cplx<T> &X, cplx<T> &Y) {
+            const cplx<T> *gd = reinterpret_cast<const cplx<T> *>(L.g2d);
+            if constexpr (DM) {
+                const int64_t o = ((int64_t)kz * N + x) * N + y0 + 2 * l;
+                X = gd[o];
+                Y = gd[o + 1];
+            } else {
+                const cplx<T> *gi = reinterpret_cast<const cplx<T> *>(L.g2i);
+                const int64_t o = ((int64_t)kz * N + x) * N + y0 + l;
+                X = gd ? gd[o] : mk<T>(0, 0);
+                Y = gi ? gi[o] : mk<T>(0, 0);
+            }
+        };
         auto fetch = [&](int64_t t, int q) {
             const LegDev &L = leg_of(q);
             const int x = (int)(t / tiles_per_x), y0 = (int)(t % tiles_per_x) * ROWS;
-            const int cnt = (L.Kz + 1) * ROWS;
-            const cplx<T> *gd = reinterpret_cast<const cplx<T> *>(L.g2d);
-            const cplx<T> *gi = reinterpret_cast<const cplx<T> *>(L.g2i);
+            const int cnt = (L.Kz + 1) * SL;
 #pragma unroll
             for (int m = 0; m < MPT; ++m) {
                 const int idx = tid + m * BT;
                 pfD[m] = mk<T>(0, 0);
                 pfI[m] = mk<T>(0, 0);
-                if (idx < cnt) {
-                    const int row = idx % ROWS, kz = idx / ROWS;
-                    const int64_t o = ((int64_t)kz * N + x) * N + y0 + row;
-                    if (gd) pfD[m] = gd[o];
-                    if (gi) pfI[m] = gi[o];
-                }
+                if (idx < cnt) load_xy(L, idx / SL, x, y0, idx % SL, pfD[m], pfI[m]);
             }
         };
         auto store = [&](int64_t t, int q, int b) {
             const LegDev &L = leg_of(q);
             const int x = (int)(t / tiles_per_x), y0 = (int)(t % tiles_per_x) * ROWS;
-            const int cnt = (L.Kz + 1) * ROWS;
+            const int cnt = (L.Kz + 1) * SL;
             cplx<T> *sd = stage_buf + (2 * b) * part, *si = stage_buf + (2 * b + 1) * part;
             // the z line of row y is Z[k] = X[k] + i Y[k] for k <= N/2 and
             // Z[N-k] = conj X[k] + i conj Y[k]; only real parts at k = 0, N/2
@@ -395,14 +413,12 @@ __global__ void __launch_bounds__((ZcCfg<T, N>::BT)) k_inv_zc_fused(const GroupD
 #pragma unroll
             for (int m = 0; m < MPT; ++m) {
                 const int idx = tid + m * BT;
-                if (idx < cnt) put(idx / ROWS, idx % ROWS, pfD[m], pfI[m]);
+                if (idx < cnt) put(idx / SL, idx % SL, pfD[m], pfI[m]);
             }
-            const cplx<T> *gd = reinterpret_cast<const cplx<T> *>(L.g2d);
-            const cplx<T> *gi = reinterpret_cast<const cplx<T> *>(L.g2i);
             for (int idx = tid + MPT * BT; idx < cnt; idx += BT) {
-                const int row = idx % ROWS, kz = idx / ROWS;
-                const int64_t o = ((int64_t)kz * N + x) * N + y0 + row;
-                put(kz, row, gd ? gd[o] : mk<T>(0, 0), gi ? gi[o] : mk<T>(0, 0));
+                cplx<T> X, Y;
+                load_xy(L, idx / SL, x, y0, idx % SL, X, Y);
+                put(idx / SL, idx % SL, X, Y);
             }
         };
         fetch(t_begin, 0);
@@ -452,15 +468,18 @@ __global__ void __launch_bounds__((ZcCfg<T, N>::BT)) k_inv_zc_fused(const GroupD
                 if (G.pre_sq) run(sink2);
                 else run(sink1);
             } else {
+                // DM: both parts of a line are D of two rows, one sum
                 double sd = 0.0, si = 0.0;
                 auto sink1 = [&](int i, int r, int, int, cplx<T> v) {
                     sd += (double)pd[i][r] * (double)v.x;
-                    si += (double)pi[i][r] * (double)v.y;
+                    if constexpr (DM) sd += (double)pi[i][r] * (double)v.y;
+                    else si += (double)pi[i][r] * (double)v.y;
                 };
                 auto sink2 = [&](int i, int r, int, int, cplx<T> v) {
                     const double dv = (double)v.x, iv = (double)v.y;
                     sd += (double)pd[i][r] * (dv * dv);
-                    si += (double)pi[i][r] * (iv * iv);
+                    if constexpr (DM) sd += (double)pi[i][r] * (iv * iv);
+                    else si += (double)pi[i][r] * (iv * iv);
                 };
                 if (G.suf_sq) run(sink2);
                 else run(sink1);
@@ -725,12 +744,12 @@ static int launch_forward_n(const void *field, int field_is_double, void *H, con
 
 template <typename T, int N>
 static int launch_inverse_g_n(const LegDev *legs_dev, int nlegs, int64_t g1a, int64_t g1b, int64_t g2a, int64_t g2b,
-                              int grid_cap, int g2_div, const void *tw, hipStream_t st)
+                              int grid_cap, int g2_div, int dmode, const void *tw, hipStream_t st)
 {
     const cplx<T> *twp = reinterpret_cast<const cplx<T> *>(tw);
     if (g1b > g1a) {
         const unsigned g = (unsigned)((g1b - g1a < grid_cap) ? g1b - g1a : grid_cap);
-        hipLaunchKernelGGL((k_inv_g1<T, N>), dim3(g), dim3(NT), 0, st, legs_dev, nlegs, g1a, g1b, twp);
+        hipLaunchKernelGGL((k_inv_g1<T, N>), dim3(g), dim3(NT), 0, st, legs_dev, nlegs, g1a, g1b, dmode, twp);
     }
     if (g2b > g2a) {
         const unsigned g = (unsigned)((g2b - g2a < grid_cap) ? g2b - g2a : grid_cap);
@@ -753,17 +772,17 @@ static int launch_zc_plain_n(const void *gd, const void *gi, int Kz, double *acc
     return (int)hipGetLastError();
 }
 
-template <typename T, int N>
-static int launch_zc_fused_n(const GroupDev *groups, int ngroups, const LegDev *legs, const int *suffix,
+template <typename T, int N, bool DM>
+static int launch_zc_fused_m(const GroupDev *groups, int ngroups, const LegDev *legs, const int *suffix,
                              int64_t i0, int64_t i1, double2 *partials, int max_chunks, int grid_cap, int kz_cap,
                              const void *tw, hipStream_t st)
 {
-    using C = ZcCfg<T, N>;
+    using C = ZcCfg<T, N, DM>;
     if (kz_cap < 1 || kz_cap > C::NH) kz_cap = C::NH;
     // two staging buffers (prefetch overlapped with the transforms) when they
     // fit next to the static LDS, else one
     hipFuncAttributes fa;
-    hipError_t e = hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(&k_inv_zc_fused<T, N>));
+    hipError_t e = hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(&k_inv_zc_fused<T, N, DM>));
     if (e != hipSuccess) return (int)e;
     int dev = 0, lds_max = 0;
     e = hipGetDevice(&dev);
@@ -774,15 +793,29 @@ static int launch_zc_fused_n(const GroupDev *groups, int ngroups, const LegDev *
     if (fa.sharedSizeBytes + 2 * per_buf > (size_t)lds_max) nbuf = 1;
     if (fa.sharedSizeBytes + nbuf * per_buf > (size_t)lds_max) return PS_ERR_UNSUPPORTED_N;
     const size_t dyn = nbuf * per_buf;
-    e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_inv_zc_fused<T, N>),
+    e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_inv_zc_fused<T, N, DM>),
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
     if (e != hipSuccess) return (int)e;
     if (i1 <= i0) return 0;
-    const int64_t cap = (C::NWV == 8) ? (grid_cap + 1) / 2 : grid_cap;  // same thread count in flight
-    const unsigned g = (unsigned)((i1 - i0 < cap) ? i1 - i0 : cap);
-    hipLaunchKernelGGL((k_inv_zc_fused<T, N>), dim3(g), dim3(C::BT), dyn, st, groups, ngroups, legs, suffix, i0, i1,
-                       partials, max_chunks, reinterpret_cast<const cplx<T> *>(tw), kz_cap, nbuf);
+    const unsigned g = (unsigned)((i1 - i0 < grid_cap) ? i1 - i0 : grid_cap);
+    hipLaunchKernelGGL((k_inv_zc_fused<T, N, DM>), dim3(g), dim3(C::BT), dyn, st, groups, ngroups, legs, suffix, i0,
+                       i1, partials, max_chunks, reinterpret_cast<const cplx<T> *>(tw), kz_cap, nbuf);
     return (int)hipGetLastError();
+}
+
+template <typename T, int N>
+static int launch_zc_fused_n(const GroupDev *groups, int ngroups, const LegDev *legs, const int *suffix,
+                             int64_t i0, int64_t i1, double2 *partials, int max_chunks, int grid_cap, int kz_cap,
+                             int dmode, const void *tw, hipStream_t st)
+{
+    if (dmode) {
+        if constexpr (zc_dmode_ok<T, N>())
+            return launch_zc_fused_m<T, N, true>(groups, ngroups, legs, suffix, i0, i1, partials, max_chunks,
+                                                 grid_cap, kz_cap, tw, st);
+        return PS_ERR_INVALID;
+    }
+    return launch_zc_fused_m<T, N, false>(groups, ngroups, legs, suffix, i0, i1, partials, max_chunks, grid_cap,
+                                          kz_cap, tw, st);
 }
 
 #define PS_FOR_EACH_N(X) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512)
@@ -800,10 +833,10 @@ static int dispatch_forward(int N, const void *field, int fd, void *H, const voi
 
 template <typename T>
 static int dispatch_inverse_g(int N, const LegDev *legs, int nlegs, int64_t g1a, int64_t g1b, int64_t g2a, int64_t g2b,
-                              int cap, int g2_div, const void *tw, hipStream_t st)
+                              int cap, int g2_div, int dmode, const void *tw, hipStream_t st)
 {
     switch (N) {
-#define CASE(n) case n: return launch_inverse_g_n<T, n>(legs, nlegs, g1a, g1b, g2a, g2b, cap, g2_div, tw, st);
+#define CASE(n) case n: return launch_inverse_g_n<T, n>(legs, nlegs, g1a, g1b, g2a, g2b, cap, g2_div, dmode, tw, st);
         PS_FOR_EACH_N(CASE)
 #undef CASE
     }
@@ -825,10 +858,10 @@ static int dispatch_zc_plain(int N, const void *gd, const void *gi, int Kz, doub
 template <typename T>
 static int dispatch_zc_fused(int N, const GroupDev *groups, int ngroups, const LegDev *legs, const int *suffix,
                              int64_t i0, int64_t i1, double2 *partials, int max_chunks, int cap, int kz_cap,
-                             const void *tw, hipStream_t st)
+                             int dmode, const void *tw, hipStream_t st)
 {
     switch (N) {
-#define CASE(n) case n: return launch_zc_fused_n<T, n>(groups, ngroups, legs, suffix, i0, i1, partials, max_chunks, cap, kz_cap, tw, st);
+#define CASE(n) case n: return launch_zc_fused_n<T, n>(groups, ngroups, legs, suffix, i0, i1, partials, max_chunks, cap, kz_cap, dmode, tw, st);
         PS_FOR_EACH_N(CASE)
 #undef CASE
     }
@@ -852,7 +885,7 @@ int tile_rows(int N, int fp64) { return tile_lines_for(N, fp64 != 0); }
 int fused_tile_rows(int N, int fp64)
 {
     switch (N) {
-#define CASE(n) case n: return fp64 ? ZcCfg<double, n>::ROWS : ZcCfg<float, n>::ROWS;
+#define CASE(n) case n: return fp64 ? ZcCfg<double, n>::ROWS : ZcCfg<float, n>::ROWS;  // both modes
         PS_FOR_EACH_N(CASE)
 #undef CASE
     }
@@ -867,10 +900,10 @@ int run_forward(int fp64, int N, const void *field, int field_is_double, void *H
 }
 
 int run_inverse_g(int fp64, int N, const LegDev *legs, int nlegs, int64_t g1a, int64_t g1b, int64_t g2a, int64_t g2b,
-                  int grid_cap, int g2_div, const void *tw, hipStream_t st)
+                  int grid_cap, int g2_div, int dmode, const void *tw, hipStream_t st)
 {
-    return fp64 ? dispatch_inverse_g<double>(N, legs, nlegs, g1a, g1b, g2a, g2b, grid_cap, g2_div, tw, st)
-                : dispatch_inverse_g<float>(N, legs, nlegs, g1a, g1b, g2a, g2b, grid_cap, g2_div, tw, st);
+    return fp64 ? dispatch_inverse_g<double>(N, legs, nlegs, g1a, g1b, g2a, g2b, grid_cap, g2_div, dmode, tw, st)
+                : dispatch_inverse_g<float>(N, legs, nlegs, g1a, g1b, g2a, g2b, grid_cap, g2_div, dmode, tw, st);
 }
 
 int run_zc_plain(int fp64, int N, const void *gd, const void *gi, int Kz, double *accD, double *accI, int mode,
@@ -881,11 +914,23 @@ int run_zc_plain(int fp64, int N, const void *gd, const void *gi, int Kz, double
 }
 
 int run_zc_fused(int fp64, int N, const GroupDev *groups, int ngroups, const LegDev *legs, const int *suffix,
-                 int64_t i0, int64_t i1, double2 *partials, int max_chunks, int grid_cap, int kz_cap, const void *tw,
-                 hipStream_t st)
+                 int64_t i0, int64_t i1, double2 *partials, int max_chunks, int grid_cap, int kz_cap, int dmode,
+                 const void *tw, hipStream_t st)
 {
-    return fp64 ? dispatch_zc_fused<double>(N, groups, ngroups, legs, suffix, i0, i1, partials, max_chunks, grid_cap, kz_cap, tw, st)
-                : dispatch_zc_fused<float>(N, groups, ngroups, legs, suffix, i0, i1, partials, max_chunks, grid_cap, kz_cap, tw, st);
+    return fp64 ? dispatch_zc_fused<double>(N, groups, ngroups, legs, suffix, i0, i1, partials, max_chunks, grid_cap,
+                                            kz_cap, dmode, tw, st)
+                : dispatch_zc_fused<float>(N, groups, ngroups, legs, suffix, i0, i1, partials, max_chunks, grid_cap,
+                                           kz_cap, dmode, tw, st);
+}
+
+int zc_dmode_supported(int N, int fp64)
+{
+    switch (N) {
+#define CASE(n) case n: return fp64 ? zc_dmode_ok<double, n>() : zc_dmode_ok<float, n>();
+        PS_FOR_EACH_N(CASE)
+#undef CASE
+    }
+    return 0;
 }
 
 int run_reduce_partials(const double2 *partials, int64_t stride, const int *counts, int64_t uniform_count,
