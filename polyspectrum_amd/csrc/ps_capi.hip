@@ -743,6 +743,39 @@ static int plan_waves(ps_batch *b, std::vector<int> &counts)
         }
         if (patched) PCK(hipMemcpy(b->divl, hv.data(), sizeof(int2) * 2 * nsf, hipMemcpyHostToDevice));
     }
+    // ---- every leg's mask as an m^2 interval where it is radial (the G1
+    // gather then tests an integer range instead of a square root per cell)
+    {
+        const size_t nl = b->legs.size();
+        std::vector<MaskParams> lm(nl);
+        std::vector<int> lk(nl);
+        for (size_t i = 0; i < nl; ++i) {
+            lm[i] = b->legs[i].mask;
+            lk[i] = b->legs[i].Kz;
+        }
+        MaskParams *dm = nullptr;
+        int *dk = nullptr;
+        int2 *dv = nullptr;
+        PCK(hipMalloc(&dm, sizeof(MaskParams) * nl));
+        b->allocs.push_back(dm);
+        PCK(hipMalloc(&dk, sizeof(int) * nl));
+        b->allocs.push_back(dk);
+        PCK(hipMalloc(&dv, sizeof(int2) * 2 * nl));
+        b->allocs.push_back(dv);
+        PCK(hipMemcpy(dm, lm.data(), sizeof(MaskParams) * nl, hipMemcpyHostToDevice));
+        PCK(hipMemcpy(dk, lk.data(), sizeof(int) * nl, hipMemcpyHostToDevice));
+        PSCK(run_shell_intervals(dm, dk, (int)nl, dv, c->st));
+        std::vector<int2> hv(2 * nl);
+        PCK(hipMemcpyAsync(hv.data(), dv, sizeof(int2) * 2 * nl, hipMemcpyDeviceToHost, c->st));
+        PCK(hipStreamSynchronize(c->st));
+        for (size_t i = 0; i < nl; ++i) {
+            const int2 iv = hv[2 * i];
+            const int cnt = hv[2 * i + 1].x;
+            const bool ok = mask_is_radial(lm[i].kind) && (iv.y < iv.x ? cnt == 0 : cnt == iv.y - iv.x + 1);
+            b->legs[i].ivlo = ok ? iv.x : -1;
+            b->legs[i].ivhi = ok ? iv.y : -1;
+        }
+    }
     // ---- per group: grid and (Fourier) descriptor
     std::vector<int> gM(ng);
     std::vector<SpecGroupDev> sg(fourier ? ng : 0);

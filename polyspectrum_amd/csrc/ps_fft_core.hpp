@@ -183,6 +183,37 @@ __device__ __forceinline__ void stage(cplx<T> *lds, const cplx<T> *__restrict__ 
     __syncthreads();
 }
 
+// f(idx) for idx = tid, tid + NT, ... < TOT, fully unrolled so that the
+// independent global loads of all iterations are in flight together
+template <int TOT, class F> __device__ __forceinline__ void tile_loop(int tid, F &&f)
+{
+#pragma unroll
+    for (int k = 0; k < (TOT + NT - 1) / NT; ++k) {
+        const int idx = tid + k * NT;
+        if (TOT % NT == 0 || idx < TOT) f(idx);
+    }
+}
+
+// the same with at most U iterations unrolled (bodies with much per-item work)
+template <int TOT, int U, class F> __device__ __forceinline__ void tile_loop_u(int tid, F &&f)
+{
+#pragma unroll U
+    for (int k = 0; k < (TOT + NT - 1) / NT; ++k) {
+        const int idx = tid + k * NT;
+        if (TOT % NT == 0 || idx < TOT) f(idx);
+    }
+}
+
+// The twiddle table copied into LDS once per workgroup: the stages read
+// their twiddles inside the dependency chain of every butterfly, where a
+// global (cache) read would stall the stage.  The caller's next barrier
+// publishes it.
+template <typename T, int N>
+__device__ __forceinline__ void stage_twiddles(cplx<T> *stw, const cplx<T> *__restrict__ tw, int tid)
+{
+    for (int i = tid; i < N; i += NT) stw[i] = tw[i];
+}
+
 template <typename T, int N, int TLC, int S, int ST = 0>
 __device__ __forceinline__ void fft_lds(cplx<T> *lds, const cplx<T> *__restrict__ tw, int tid)
 {

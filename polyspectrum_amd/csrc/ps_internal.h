@@ -31,6 +31,8 @@ struct LegDev {
     void *g1d, *g1i;   // band buffers [Kz+1][Bx][N] (D part may be null)
     void *g2d, *g2i;   // full buffers [Kz+1][N][N]
     double *scale;     // device: rms |spectrum| over the shell (D part enters G1 divided by it)
+    int ivlo = -1;     // radial mask: the cells with ivlo <= m^2 <= ivhi ((1, 0): none);
+    int ivhi = -1;     // ivlo < 0: test every cell with mask_eval
 };
 
 // A batch of polygons sharing a prefix product: output s of the group is

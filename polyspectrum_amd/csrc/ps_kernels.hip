@@ -44,19 +44,21 @@ __global__ void __launch_bounds__(NT) k_fwd_z(const TI *__restrict__ in, cplx<T>
     constexpr int TL = tile_lines<T, N>();  // real rows per tile
     constexpr int TLC = TL / 2;
     __shared__ cplx<T> lds[N * TLC];
+    __shared__ cplx<T> stw[N];
     const int tid = threadIdx.x;
     const int y0 = blockIdx.x * TL, x = blockIdx.y;
     const int64_t base = ((int64_t)x * N + y0) * N;
-    for (int idx = tid; idx < TL * N; idx += NT) {
+    stage_twiddles<T, N>(stw, tw, tid);
+    tile_loop<TL * N>(tid, [&](int idx) {
         const int row = idx / N, z = idx % N;
         T v = ld_real<T>(in, base + idx);
         T *p = reinterpret_cast<T *>(&lds[lidx<TLC>(z, row >> 1)]);
         p[row & 1] = v;
-    }
+    });
     __syncthreads();
-    fft_lds<T, N, TLC, -1>(lds, tw, tid);
+    fft_lds<T, N, TLC, -1>(lds, stw, tid);
     constexpr int NH = N / 2 + 1;
-    for (int idx = tid; idx < NH * TLC; idx += NT) {
+    tile_loop<NH * TLC>(tid, [&](int idx) {
         const int c = idx % TLC, kz = idx / TLC;
         const cplx<T> z1 = lds[lidx<TLC>(kz, c)];
         const cplx<T> z2 = lds[lidx<TLC>((N - kz) % N, c)];
@@ -65,7 +67,7 @@ __global__ void __launch_bounds__(NT) k_fwd_z(const TI *__restrict__ in, cplx<T>
         const int64_t o = ((int64_t)kz * N + x) * N + y0 + 2 * c;
         H[o] = A;
         H[o + 1] = B;
-    }
+    });
 }
 
 // Y pass: C2C along the contiguous axis for nlines consecutive lines.
@@ -74,12 +76,14 @@ __global__ void __launch_bounds__(NT) k_c2c_contig(cplx<T> *__restrict__ data, c
 {
     constexpr int TLC = tile_lines<T, N>();
     __shared__ cplx<T> lds[N * TLC];
+    __shared__ cplx<T> stw[N];
     const int tid = threadIdx.x;
     cplx<T> *base = data + (int64_t)blockIdx.x * TLC * N;
-    for (int idx = tid; idx < TLC * N; idx += NT) lds[lidx<TLC>(idx % N, idx / N)] = base[idx];
+    stage_twiddles<T, N>(stw, tw, tid);
+    tile_loop<TLC * N>(tid, [&](int idx) { lds[lidx<TLC>(idx % N, idx / N)] = base[idx]; });
     __syncthreads();
-    fft_lds<T, N, TLC, S>(lds, tw, tid);
-    for (int idx = tid; idx < TLC * N; idx += NT) base[idx] = lds[lidx<TLC>(idx % N, idx / N)];
+    fft_lds<T, N, TLC, S>(lds, stw, tid);
+    tile_loop<TLC * N>(tid, [&](int idx) { base[idx] = lds[lidx<TLC>(idx % N, idx / N)]; });
 }
 
 // X pass: C2C along the stride-N axis; block = (ky tile, plane).
@@ -88,18 +92,20 @@ __global__ void __launch_bounds__(NT) k_c2c_strided(cplx<T> *__restrict__ data, 
 {
     constexpr int TLC = tile_lines<T, N>();
     __shared__ cplx<T> lds[N * TLC];
+    __shared__ cplx<T> stw[N];
     const int tid = threadIdx.x;
     cplx<T> *base = data + (int64_t)blockIdx.y * N * N + blockIdx.x * TLC;
-    for (int idx = tid; idx < TLC * N; idx += NT) {
+    stage_twiddles<T, N>(stw, tw, tid);
+    tile_loop<TLC * N>(tid, [&](int idx) {
         const int l = idx % TLC, e = idx / TLC;
         lds[lidx<TLC>(e, l)] = base[(int64_t)e * N + l];
-    }
+    });
     __syncthreads();
-    fft_lds<T, N, TLC, S>(lds, tw, tid);
-    for (int idx = tid; idx < TLC * N; idx += NT) {
+    fft_lds<T, N, TLC, S>(lds, stw, tid);
+    tile_loop<TLC * N>(tid, [&](int idx) {
         const int l = idx % TLC, e = idx / TLC;
         base[(int64_t)e * N + l] = lds[lidx<TLC>(e, l)];
-    }
+    });
 }
 
 // ================================================================ inverse
@@ -114,8 +120,10 @@ __global__ void __launch_bounds__(NT) k_inv_g1(const LegDev *__restrict__ legs, 
     constexpr int TLC = tile_lines<T, N>() / 2;
     constexpr int TL2 = 2 * TLC;
     __shared__ cplx<T> lds[N * TL2];
+    __shared__ cplx<T> stw[N];
     const int tid = threadIdx.x;
     const int lpt = dmode ? TL2 : TLC;  // kx lines per tile
+    stage_twiddles<T, N>(stw, tw, tid);
     for (int64_t tile = t0 + blockIdx.x; tile < t1; tile += gridDim.x) {
         const int li = find_owner(legs, nlegs, tile, &LegDev::g1_begin);
         const LegDev &L = legs[li];
@@ -126,13 +134,14 @@ __global__ void __launch_bounds__(NT) k_inv_g1(const LegDev *__restrict__ legs, 
         const int mz = lattice(kz, N);
         const cplx<T> *spec = reinterpret_cast<const cplx<T> *>(L.spec);
         const T inv_scale = L.scale ? (T)(1.0 / *L.scale) : (T)1;
-        for (int idx = tid; idx < lpt * N; idx += NT) {
+        auto gather = [&](int idx) {
             const int l = idx / N, ky = idx % N;
             const int j = j0 + l;
             cplx<T> d = mk<T>(0, 0), u = mk<T>(0, 0);
             if (j < L.Bx) {
                 const int mx = lattice(band_to_k(j, L.Kx, L.Bx, N), N), my = lattice(ky, N);
-                if (mask_eval(L.mask, mx, my, mz)) {
+                const int m2 = mx * mx + my * my + mz * mz;
+                if (L.ivlo >= 0 ? (m2 >= L.ivlo && m2 <= L.ivhi) : mask_eval(L.mask, mx, my, mz)) {
                     u.x = (T)1;
                     if (spec) {
                         // the cell's index on the spectrum's grid (ns >= N)
@@ -149,12 +158,14 @@ __global__ void __launch_bounds__(NT) k_inv_g1(const LegDev *__restrict__ legs, 
             }
             lds[lidx<TL2>(ky, l)] = d;
             if (!dmode) lds[lidx<TL2>(ky, l + TLC)] = u;
-        }
+        };
+        if (dmode) tile_loop_u<TL2 * N, 4>(tid, gather);
+        else tile_loop_u<TLC * N, 4>(tid, gather);
         __syncthreads();
-        fft_lds<T, N, TL2, +1>(lds, tw, tid);
+        fft_lds<T, N, TL2, +1>(lds, stw, tid);
         cplx<T> *gd = reinterpret_cast<cplx<T> *>(L.g1d);
         cplx<T> *gi = reinterpret_cast<cplx<T> *>(L.g1i);
-        for (int idx = tid; idx < lpt * N; idx += NT) {
+        auto put = [&](int idx) {
             const int l = idx / N, y = idx % N;
             const int j = j0 + l;
             if (j < L.Bx) {
@@ -162,7 +173,9 @@ __global__ void __launch_bounds__(NT) k_inv_g1(const LegDev *__restrict__ legs, 
                 if (gd) gd[o] = lds[lidx<TL2>(y, l)];
                 if (!dmode) gi[o] = lds[lidx<TL2>(y, l + TLC)];
             }
-        }
+        };
+        if (dmode) tile_loop<TL2 * N>(tid, put);
+        else tile_loop<TLC * N>(tid, put);
         __syncthreads();
     }
 }
@@ -175,8 +188,10 @@ __global__ void __launch_bounds__(NT) k_inv_g2(const LegDev *__restrict__ legs, 
 {
     constexpr int TLC = tile_lines<T, N>() / WD;  // WD: tile width divisor picked by the autotuner
     __shared__ cplx<T> lds[N * TLC];
+    __shared__ cplx<T> stw[N];
     const int tid = threadIdx.x;
     constexpr int nyt = N / TLC;
+    stage_twiddles<T, N>(stw, tw, tid);
     for (int64_t tile = t0 + blockIdx.x; tile < t1; tile += gridDim.x) {
         const int li = find_owner(legs, nlegs, tile, &LegDev::g2_begin);
         const LegDev &L = legs[li];
@@ -188,20 +203,20 @@ __global__ void __launch_bounds__(NT) k_inv_g2(const LegDev *__restrict__ legs, 
         const int y0 = (int)(rest % nyt) * TLC;
         const cplx<T> *g = reinterpret_cast<const cplx<T> *>(part ? L.g1i : L.g1d);
         cplx<T> *o = reinterpret_cast<cplx<T> *>(part ? L.g2i : L.g2d);
-        for (int idx = tid; idx < TLC * N; idx += NT) {
+        tile_loop<TLC * N>(tid, [&](int idx) {
             const int l = idx % TLC, kx = idx / TLC;
             const int j = k_to_band(kx, L.Kx, L.Bx, N);
             cplx<T> d = mk<T>(0, 0);
             if (j >= 0 && g) d = g[((int64_t)kz * L.Bx + j) * N + y0 + l];
             lds[lidx<TLC>(kx, l)] = d;
-        }
+        });
         __syncthreads();
-        fft_lds<T, N, TLC, +1>(lds, tw, tid);
+        fft_lds<T, N, TLC, +1>(lds, stw, tid);
         if (o) {
-            for (int idx = tid; idx < TLC * N; idx += NT) {
+            tile_loop<TLC * N>(tid, [&](int idx) {
                 const int l = idx % TLC, x = idx / TLC;
                 o[((int64_t)kz * N + x) * N + y0 + l] = lds[lidx<TLC>(x, l)];
-            }
+            });
         }
         __syncthreads();
     }

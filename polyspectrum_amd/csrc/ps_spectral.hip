@@ -205,8 +205,10 @@ __global__ void __launch_bounds__(NT) k_spec_fx(const SpecGroupDev *__restrict__
 {
     constexpr int TLC = tile_lines<T, N>() / WD;
     __shared__ cplx<T> lds[N * TLC];
+    __shared__ cplx<T> stw[N];
     const int tid = threadIdx.x;
     constexpr int nyt = N / TLC;
+    stage_twiddles<T, N>(stw, tw, tid);
     for (int64_t tile = t0 + blockIdx.x; tile < t1; tile += gridDim.x) {
         const int gi = find_owner(groups, ngroups, tile, &SpecGroupDev::fx_begin);
         const SpecGroupDev &G = groups[gi];
@@ -218,12 +220,12 @@ __global__ void __launch_bounds__(NT) k_spec_fx(const SpecGroupDev *__restrict__
         const int y0 = (int)(rest % nyt) * TLC;
         const cplx<T> *in = reinterpret_cast<const cplx<T> *>(part ? G.qi : G.qd);
         cplx<T> *out = reinterpret_cast<cplx<T> *>(part ? G.bi : G.bd);
-        for (int idx = tid; idx < TLC * N; idx += NT) {
+        tile_loop<TLC * N>(tid, [&](int idx) {
             const int l = idx % TLC, xx = idx / TLC;
             lds[lidx<TLC>(xx, l)] = in[((int64_t)kz * N + xx) * N + y0 + l];
-        }
+        });
         __syncthreads();
-        fft_lds<T, N, TLC, -1>(lds, tw, tid);
+        fft_lds<T, N, TLC, -1>(lds, stw, tid);
         const int Bq = G.Bq;
         for (int idx = tid; idx < TLC * Bq; idx += NT) {
             const int l = idx % TLC, jx = idx / TLC;
