@@ -498,8 +498,8 @@ __global__ void __launch_bounds__((ZcCfg<T, N, DM>::BT)) k_inv_zc_fused(const Gr
                 };
                 if (G.suf_sq) run(sink2);
                 else run(sink1);
-                sd = wave_sum(sd);
-                si = wave_sum(si);
+                sd = wave_sum_dpp(sd);
+                if constexpr (!DM) si = wave_sum_dpp(si);
                 if (lane == 0) {
                     const int o = q - G.npre;
                     KSum a; a.s = acc[w][o][0]; a.c = acc[w][o][1]; a.add(sd);

@@ -201,4 +201,28 @@ __device__ __forceinline__ double wave_sum(double s)
     return s;
 }
 
+// a double moved across lanes by a DPP control (all lanes of the row valid)
+template <int CTRL> __device__ __forceinline__ double dpp_f64(double v)
+{
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)u, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), CTRL, 0xF, 0xF, false);
+    return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
+// FP64 sum over a wavefront (valid in every lane): within each 16-lane row by
+// DPP moves (quad_perm [1,0,3,2], [2,3,0,1], row_ror 4, row_ror 8: VALU
+// latency only), then across the four rows by two lane shuffles.  A fixed
+// order, so the result is reproducible.
+__device__ __forceinline__ double wave_sum_dpp(double s)
+{
+    s += dpp_f64<0xB1>(s);
+    s += dpp_f64<0x4E>(s);
+    s += dpp_f64<0x124>(s);
+    s += dpp_f64<0x128>(s);
+    s += __shfl_xor(s, 16, 64);
+    s += __shfl_xor(s, 32, 64);
+    return s;
+}
+
 }  // namespace ps
