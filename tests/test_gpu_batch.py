@@ -160,3 +160,39 @@ def test_overlap_schedule_matches(gpu_lib, prec_name):
                 np.testing.assert_array_equal(out[ov][0], again[0])
         np.testing.assert_array_equal(out[1][0], out[0][0])
         np.testing.assert_array_equal(out[1][1], out[0][1])
+
+
+@pytest.mark.parametrize("prec_name", ["PREC_FP64", "PREC_MIXED"])
+def test_direct_waves_in_one_arena_region(gpu_lib, prec_name):
+    """Direct method with the arena budget below one wave: every wave (grid)
+    reuses one buffer region in turn, for the planning run of the polygon
+    counts and for every execution; the values equal the side-by-side plan's."""
+    import os
+    P = gpu_lib
+    N, L = 48, 100.0
+    f = lognormal_field(N, 57)
+    kf = 2 * math.pi / L
+    polys = _polys_triangles(kf, [(2.0, 2.0), (4.0, 4.0), (6.5, 6.5)], [0.3, 1.2, 2.6], 2)
+    polys += [{"n": 2, "k": [m * kf] * 2, "kmin": [m * kf] * 2, "kmax": [m * kf] * 2} for m in range(1, 9)]
+    out = []
+    with P.Context(N, L, getattr(P, prec_name)) as ctx:
+        ctx.load_field(f)
+        for budget in (None, "1"):
+            if budget:
+                os.environ["PS_ARENA_MB"] = budget
+            try:
+                b = ctx.batch(polys, 1.0, 2, params={"grid_cap": 1024, "chunk_tiles": 1, "method": P.METHOD_DIRECT})
+                out.append(b.execute())
+                again = b.execute()
+                np.testing.assert_array_equal(out[-1][0], again[0])
+                b.close()
+            finally:
+                os.environ.pop("PS_ARENA_MB", None)
+    np.testing.assert_array_equal(out[1][0], out[0][0])
+    np.testing.assert_array_equal(out[1][1], out[0][1])
+    if prec_name == "PREC_FP64":
+        spec = O.forward_spectrum(f)
+        for i in (0, 4, 8, 10, 14):
+            p = polys[i]
+            ref = O.polyspectrum(spec, p["n"], p["k"], p["kmin"], p["kmax"], 1.0, 2, L)[0]
+            assert abs(out[0][0][i] - ref) <= 1e-10 * abs(ref), (i, out[0][0][i], ref)
