@@ -1080,6 +1080,57 @@ static int plan_waves(ps_batch *b, std::vector<int> &counts)
 #undef PCK
 
 
+// Direct method: the polygons of one prefix key split into classes by the
+// transform grid their product needs.  grid[q] is the smallest alias-free
+// grid of polygon q; a class runs on the largest grid among its members and
+// transforms the npre prefix legs once more.  The classes minimise
+// sum M^3 * (npre + members) (every leg costs about M^3 on grid M) over every
+// subset of the grids that occur (at most the 11 supported ones); members
+// keep their input order.
+static std::vector<std::vector<int>> split_by_grid(const std::vector<int> &ps_, const std::vector<int> &grid, int npre)
+{
+    std::vector<int> g(grid.begin(), grid.end());
+    std::sort(g.begin(), g.end());
+    g.erase(std::unique(g.begin(), g.end()), g.end());
+    const int k = (int)g.size();
+    std::vector<double> cnt(k, 0.0);
+    for (int m : grid) cnt[std::lower_bound(g.begin(), g.end(), m) - g.begin()] += 1.0;
+    unsigned best = 0;
+    double best_cost = -1.0;
+    for (unsigned sub = 0; sub < (1u << (k - 1)); ++sub) {
+        const unsigned chosen = sub | (1u << (k - 1));  // the largest grid is always used
+        double cost = 0.0, members = 0.0;
+        for (int i = 0; i < k; ++i) {
+            members += cnt[i];
+            if (chosen & (1u << i)) {
+                cost += std::pow((double)g[i], 3) * (npre + members);
+                members = 0.0;
+            }
+        }
+        if (best_cost < 0.0 || cost < best_cost) {
+            best_cost = cost;
+            best = chosen;
+        }
+    }
+    std::vector<std::vector<int>> out;
+    std::vector<int> cls(k, -1);
+    for (int i = k - 1, cur = -1; i >= 0; --i) {
+        if (best & (1u << i)) cur = i;
+        cls[i] = cur;
+    }
+    std::vector<int> slot(k, -1);
+    for (int i = 0; i < k; ++i)
+        if (best & (1u << i)) {
+            slot[i] = (int)out.size();
+            out.emplace_back();
+        }
+    for (size_t q = 0; q < ps_.size(); ++q) {
+        const int i = (int)(std::lower_bound(g.begin(), g.end(), grid[q]) - g.begin());
+        out[slot[cls[i]]].push_back(ps_[q]);
+    }
+    return out;
+}
+
 extern "C" int ps_batch_create_ex(ps_ctx *c, ps_polygon *polys, int npoly, double bw, int kcase,
                                   const ps_launch_params *params, ps_batch **out)
 {
@@ -1199,9 +1250,24 @@ extern "C" int ps_batch_create_ex(ps_ctx *c, ps_polygon *polys, int npoly, doubl
     b->poly_n.assign(npoly, 0);
     for (auto &kv : glist) {
         const int kind = std::get<0>(kv.first);
-        const std::vector<int> &ps_ = kv.second;
-        for (size_t s0 = 0; s0 < ps_.size(); s0 += MAX_SUFFIX) {
-            const size_t s1 = std::min(ps_.size(), s0 + MAX_SUFFIX);
+        std::vector<std::vector<int>> classes{kv.second};
+        if (b->method == PS_METHOD_DIRECT) {
+            const int a = std::get<1>(kv.first), a2 = std::get<2>(kv.first);
+            const int KA = (a >= 0) ? b->legs[a].Kz : 0;
+            const int KB = (a2 >= 0) ? b->legs[a2].Kz : 0;
+            std::vector<int> grid;
+            for (int p : kv.second) {
+                const int Kt = b->legs[kind == 1 ? pleg[p][0] : pleg[p][polys[p].n - 1]].Kz;
+                grid.push_back(alias_free_grid((kind == 1 ? 2 * Kt : KA + KB + Kt) + 1, N));
+            }
+            const int npre = (kind == 3) ? (a == a2 ? 1 : 2) : (kind == 2 ? 1 : 0);
+            classes = split_by_grid(kv.second, grid, npre);
+        }
+        std::vector<std::vector<int>> chunks;  // at most MAX_SUFFIX polygons each
+        for (const std::vector<int> &cl : classes)
+            for (size_t s0 = 0; s0 < cl.size(); s0 += MAX_SUFFIX)
+                chunks.emplace_back(cl.begin() + s0, cl.begin() + std::min(cl.size(), s0 + MAX_SUFFIX));
+        for (const std::vector<int> &ps_ : chunks) {
             GroupDev G{};
             if (kind == 3) {
                 G.preA = std::get<1>(kv.first);
@@ -1215,13 +1281,12 @@ extern "C" int ps_batch_create_ex(ps_ctx *c, ps_polygon *polys, int npoly, doubl
                 G.npre = 0;
                 G.suf_sq = 1;
             }
-            G.nsuf = (int)(s1 - s0);
+            G.nsuf = (int)ps_.size();
             G.suf_begin = (int)b->suffix.size();
             G.out_begin = b->nout;
             G.chunk_tiles = b->chunk_tiles;
             G.fscale = 1.0;
-            for (size_t q = s0; q < s1; ++q) {
-                const int p = ps_[q];
+            for (const int p : ps_) {
                 const std::vector<int> &l = pleg[p];
                 b->suffix.push_back(kind == 1 ? l[0] : l[polys[p].n - 1]);
                 b->poly_out[p] = b->nout;
