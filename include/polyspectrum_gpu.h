@@ -201,6 +201,13 @@ int ps_batch_profile(ps_batch *b, int reps, double *stage_ms, double *stage_byte
 int ps_batch_stage_count(const ps_batch *b);
 const char *ps_batch_stage_name(const ps_batch *b, int stage);
 
+/* The batch's plan (no counterpart in the reference; for tuning): its waves,
+ * each on one transform grid.  ps_batch_wave_info gives wave w's grid size,
+ * the legs it transforms, its polygon groups and the arena bytes it uses
+ * (any output pointer may be NULL). */
+int ps_batch_wave_count(const ps_batch *b);
+int ps_batch_wave_info(const ps_batch *b, int w, int *grid, int *nlegs, int *ngroups, double *bytes);
+
 /* Mean time (ms) of reps forward transforms of a slot's last upload and the
  * algorithmic bytes of one (3 passes, each reading and writing its array). */
 int ps_forward_profile(ps_ctx *ctx, int slot, int reps, double *ms, double *bytes);

@@ -83,6 +83,7 @@ def build():
 
 
 _dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
 _vp = ctypes.c_void_p
 
 
@@ -129,6 +130,10 @@ def load_library():
     L.ps_batch_stage_count.argtypes = [_vp]
     L.ps_batch_stage_name.restype = ctypes.c_char_p
     L.ps_batch_stage_name.argtypes = [_vp, ctypes.c_int]
+    L.ps_batch_wave_count.restype = ctypes.c_int
+    L.ps_batch_wave_count.argtypes = [_vp]
+    L.ps_batch_wave_info.restype = ctypes.c_int
+    L.ps_batch_wave_info.argtypes = [_vp, ctypes.c_int, _ip, _ip, _ip, _dp]
     L.ps_batch_adjoint.argtypes = [_vp, ctypes.c_int, _dp, _vp, ctypes.c_int, ctypes.c_int]
     L.ps_forward_profile.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _dp, _dp]
     L.ps_batch_results.argtypes = [_vp, _dp, _dp, _dp]
@@ -364,6 +369,18 @@ class Batch:
         n = L.ps_batch_stage_count(self._h)
         names = [L.ps_batch_stage_name(self._h, i).decode() for i in range(n)]
         return ms[:n], by[:n], names
+
+    def waves(self):
+        """The plan: per wave (transform grid, legs, groups, arena bytes)."""
+        L = load_library()
+        out = []
+        for w in range(L.ps_batch_wave_count(self._h)):
+            g, nl, ng = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+            by = ctypes.c_double()
+            _check(L.ps_batch_wave_info(self._h, w, ctypes.byref(g), ctypes.byref(nl), ctypes.byref(ng),
+                                        ctypes.byref(by)))
+            out.append((g.value, nl.value, ng.value, by.value))
+        return out
 
     @property
     def last_ms(self):

@@ -1801,6 +1801,19 @@ extern "C" const char *ps_batch_stage_name(const ps_batch *b, int stage)
     return b->method == PS_METHOD_FOURIER ? kStageFourier[stage] : kStageDirect[stage];
 }
 
+extern "C" int ps_batch_wave_count(const ps_batch *b) { return b ? (int)b->waves.size() : 0; }
+
+extern "C" int ps_batch_wave_info(const ps_batch *b, int w, int *grid, int *nlegs, int *ngroups, double *bytes)
+{
+    if (!b || w < 0 || w >= (int)b->waves.size()) return PS_ERR_INVALID;
+    const Wave &v = b->waves[w];
+    if (grid) *grid = v.M;
+    if (nlegs) *nlegs = (int)v.legs.size();
+    if (ngroups) *ngroups = (int)(b->method == PS_METHOD_FOURIER ? v.sgroups.size() : v.groups.size());
+    if (bytes) *bytes = (double)v.bytes;
+    return PS_OK;
+}
+
 extern "C" int ps_forward_profile(ps_ctx *c, int slot, int reps, double *ms, double *bytes)
 {
     if (!c || reps < 1 || !ms || slot < 0 || slot >= c->nslots) return PS_ERR_INVALID;

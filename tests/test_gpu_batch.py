@@ -196,3 +196,36 @@ def test_direct_waves_in_one_arena_region(gpu_lib, prec_name):
             p = polys[i]
             ref = O.polyspectrum(spec, p["n"], p["k"], p["kmin"], p["kmax"], 1.0, 2, L)[0]
             assert abs(out[0][0][i] - ref) <= 1e-10 * abs(ref), (i, out[0][0][i], ref)
+
+
+def test_direct_groups_split_by_grid(gpu_lib):
+    """Triangles sharing k1 = k2 whose third legs need different alias-free
+    grids (M > K1 + K2 + K3) run as classes on their own grids, one wave per
+    grid; each value equals the triangle evaluated alone and the oracle."""
+    P = gpu_lib
+    N, L = 64, 100.0
+    f = lognormal_field(N, 91)
+    kf = 2 * math.pi / L
+    k = 8.0 * kf
+    polys = [{"n": 3, "k": [k, k, m * kf], "kmin": [k, k, m * kf], "kmax": [k, k, m * kf]}
+             for m in (1.0, 10.0, 15.0, 16.0)]
+    params = {"grid_cap": 1024, "chunk_tiles": 1, "method": P.METHOD_DIRECT}
+    with P.Context(N, L, P.PREC_FP64) as ctx:
+        ctx.load_field(f)
+        b = ctx.batch(polys, 1.0, 0, params=params)
+        grids = [w[0] for w in b.waves()]
+        assert len(set(grids)) >= 2 and grids == sorted(grids) and max(grids) <= N, grids
+        assert all(w[1] >= 1 and w[2] >= 1 and w[3] > 0 for w in b.waves())
+        v, npg, _ = b.execute()
+        b.close()
+        for i, p in enumerate(polys):
+            b1 = ctx.batch([p], 1.0, 0, params=params)
+            assert len(b1.waves()) == 1
+            v1, npg1, _ = b1.execute()
+            b1.close()
+            assert abs(v[i] - v1[0]) <= 1e-12 * abs(v1[0]), (i, v[i], v1[0])
+            assert npg[i] == npg1[0]
+    spec = O.forward_spectrum(f)
+    for i, p in enumerate(polys):
+        ref = O.polyspectrum(spec, 3, p["k"], p["kmin"], p["kmax"], 1.0, 0, L)[0]
+        assert abs(v[i] - ref) <= 1e-10 * abs(ref), (i, v[i], ref)
