@@ -348,9 +348,22 @@ template <typename T, int N, bool DM = false> struct ZcCfg {
     static constexpr int MPT = ((NH * SL + BT - 1) / BT < 4) ? (NH * SL + BT - 1) / BT : 4;
 };
 template <typename T, int N> constexpr bool zc_dmode_ok() { return wave_lines<T, N>() % 2 == 0; }
+// Compact form of the fused pass: twiddles read from an LDS table and one
+// staging buffer (the loads of the next step still overlap the transforms, in
+// registers), which leaves registers and LDS for four workgroups per CU where
+// the transforms fit in 128 VGPRs (FP32 D-only at power-of-two lengths 64 ..
+// 256: measured 128: 74 -> 56 us, 256: 141 -> 121 us on the 512^3 sweep) and
+// helps FP64 (1.68 -> 1.47 ms); FP32 at 96 and 192 keep register twiddles and
+// two staging buffers (spills otherwise).
+template <typename T, int N> constexpr bool zc_pow2_mid() { return N >= 64 && N <= 256 && (N & (N - 1)) == 0; }
+template <typename T, int N> constexpr bool zc_compact() { return sizeof(T) == 8 || zc_pow2_mid<T, N>(); }
+template <typename T, int N, bool DM> constexpr int zc_waves()
+{
+    return (DM && sizeof(T) == 4 && zc_pow2_mid<T, N>()) ? 4 : 1;
+}
 
 template <typename T, int N, bool DM>
-__global__ void __launch_bounds__((ZcCfg<T, N, DM>::BT)) k_inv_zc_fused(const GroupDev *__restrict__ groups,
+__global__ void __launch_bounds__((ZcCfg<T, N, DM>::BT)) __attribute__((amdgpu_waves_per_eu(zc_waves<T, N, DM>()))) k_inv_zc_fused(const GroupDev *__restrict__ groups,
                                                          int ngroups, const LegDev *__restrict__ legs,
                                                          const int *__restrict__ suffix, int64_t i0, int64_t i1,
                                                          double2 *__restrict__ partials, int max_chunks,
@@ -366,9 +379,11 @@ __global__ void __launch_bounds__((ZcCfg<T, N, DM>::BT)) k_inv_zc_fused(const Gr
     cplx<T> *stage_buf = reinterpret_cast<cplx<T> *>(dyn_lds);  // [nbuf buffers][2 parts][kz_cap * SP]
     const int part = kz_cap * SP;
     const int tid = threadIdx.x, w = tid / 64, lane = tid & 63;
+    constexpr bool CMP = zc_compact<T, N>();
+    __shared__ cplx<T> stw[CMP ? N : 1];
     WaveTw<T, N, LW> twr;
-    twr.template load<+1>(twg, lane);
-    const RegTw<T, N, LW> tw{twr};
+    if constexpr (CMP) stage_twiddles<T, N>(stw, twg, tid);
+    else twr.template load<+1>(twg, lane);
     cplx<T> *wb = wbuf + w * LW * LP;
     constexpr int tiles_per_x = N / ROWS;
     const int64_t ntile = (int64_t)N * tiles_per_x;
@@ -473,8 +488,15 @@ __global__ void __launch_bounds__((ZcCfg<T, N, DM>::BT)) k_inv_zc_fused(const Gr
             // instantiation, so no per-element selects enter the transforms
             const bool edge = Kz < N / stage_radix(N, 0);
             auto run = [&](auto &sink) {
-                if (edge) wave_fft<T, N, LW, +1, 0, true, RegTw<T, N, LW>>(wb, tw, lane, src, sink);
-                else wave_fft<T, N, LW, +1, 0, false, RegTw<T, N, LW>>(wb, tw, lane, src, sink);
+                if constexpr (CMP) {
+                    const LdsTw<T, N, LW> tw{stw};
+                    if (edge) wave_fft<T, N, LW, +1, 0, true, LdsTw<T, N, LW>>(wb, tw, lane, src, sink);
+                    else wave_fft<T, N, LW, +1, 0, false, LdsTw<T, N, LW>>(wb, tw, lane, src, sink);
+                } else {
+                    const RegTw<T, N, LW> tw{twr};
+                    if (edge) wave_fft<T, N, LW, +1, 0, true, RegTw<T, N, LW>>(wb, tw, lane, src, sink);
+                    else wave_fft<T, N, LW, +1, 0, false, RegTw<T, N, LW>>(wb, tw, lane, src, sink);
+                }
             };
             if (q < G.npre) {
                 auto sink1 = [&](int i, int r, int, int, cplx<T> v) {
@@ -799,8 +821,8 @@ static int launch_zc_fused_m(const GroupDev *groups, int ngroups, const LegDev *
 {
     using C = ZcCfg<T, N, DM>;
     if (kz_cap < 1 || kz_cap > C::NH) kz_cap = C::NH;
-    // two staging buffers (prefetch overlapped with the transforms) when they
-    // fit next to the static LDS, else one
+    // two staging buffers (the next step's LDS stores need no extra barrier)
+    // when they fit next to the static LDS, except in the compact form
     hipFuncAttributes fa;
     hipError_t e = hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(&k_inv_zc_fused<T, N, DM>));
     if (e != hipSuccess) return (int)e;
@@ -809,7 +831,7 @@ static int launch_zc_fused_m(const GroupDev *groups, int ngroups, const LegDev *
     if (e == hipSuccess) e = hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
     if (e != hipSuccess) return (int)e;
     const size_t per_buf = sizeof(cplx<T>) * 2 * (size_t)kz_cap * C::SP;
-    int nbuf = 2;
+    int nbuf = zc_compact<T, N>() ? 1 : 2;
     if (fa.sharedSizeBytes + 2 * per_buf > (size_t)lds_max) nbuf = 1;
     if (fa.sharedSizeBytes + nbuf * per_buf > (size_t)lds_max) return PS_ERR_UNSUPPORTED_N;
     const size_t dyn = nbuf * per_buf;
