@@ -691,7 +691,7 @@ extern "C" int ps_batch_create(ps_ctx *c, ps_polygon *polys, int npoly, double b
 // smallest supported transform grid m >= need, at most the spectrum's N
 static int alias_free_grid(int need, int N)
 {
-    static const int kGrids[] = {16, 24, 32, 48, 64, 96, 128, 192, 256, 384, 512};
+    static const int kGrids[] = {16, 24, 32, 48, 64, 96, 128, 144, 192, 256, 384, 512};
     for (int m : kGrids)
         if (m >= need && m <= N) return m;
     return N;
@@ -871,6 +871,11 @@ static int plan_waves(ps_batch *b, std::vector<int> &counts)
                 if (wave_leg.count(p)) continue;
                 wave_leg[p] = (int)w.legs.size();
                 LegDev L = b->legs[p];
+                // On grid M > K_A + K_B + K_t the modes of a leg that meet the
+                // others in a closed product have |m| <= min(K_i, K_j + K_l)
+                // < M / 2, so a shell wider than the grid (a polygon that
+                // cannot close) keeps only its planes and rows below M / 2.
+                L.Kz = L.Kx = std::min(L.Kz, M / 2);
                 L.Bx = std::min(2 * L.Kz + 1, M);
                 w.leg_off.push_back(w.bytes);
                 w.bytes += leg_bytes(L);

@@ -229,3 +229,64 @@ def test_direct_groups_split_by_grid(gpu_lib):
     for i, p in enumerate(polys):
         ref = O.polyspectrum(spec, 3, p["k"], p["kmin"], p["kmax"], 1.0, 0, L)[0]
         assert abs(v[i] - ref) <= 1e-10 * abs(ref), (i, v[i], ref)
+
+
+def test_reduced_grid_144(gpu_lib):
+    """Triangles whose product needs 128 < M <= 144 run on the 144 grid
+    (radix 3, 3, 8, 2); FP64 equals the oracle on the 192 grid and the mixed
+    path equals FP64 to the mixed bar."""
+    P = gpu_lib
+    N, L = 192, 200.0
+    f = lognormal_field(N, 144)
+    kf = 2 * math.pi / L
+    polys = [{"n": 3, "k": [40 * kf, 40 * kf, m * kf], "kmin": [0.0] * 3, "kmax": [0.0] * 3} for m in (50.0, 55.0, 70.0)]
+    res = {}
+    for prec in (P.PREC_FP64, P.PREC_MIXED):
+        with P.Context(N, L, prec) as ctx:
+            ctx.load_field(f)
+            b = ctx.batch(polys, 1.0, 0, params={"grid_cap": 1024, "chunk_tiles": 1, "method": P.METHOD_DIRECT})
+            grids = [w[0] for w in b.waves()]
+            res[prec] = b.execute()
+            b.close()
+        assert 144 in grids, grids
+    spec = O.forward_spectrum(f)
+    for i in (0, 2):
+        p = polys[i]
+        ref = O.polyspectrum(spec, 3, p["k"], p["kmin"], p["kmax"], 1.0, 0, L)[0]
+        assert abs(res[P.PREC_FP64][0][i] - ref) <= 1e-10 * abs(ref), (i, res[P.PREC_FP64][0][i], ref)
+    rel = np.abs(res[P.PREC_MIXED][0] - res[P.PREC_FP64][0]) / np.abs(res[P.PREC_FP64][0])
+    assert np.all(rel < 1e-5), rel
+
+
+@pytest.mark.parametrize("method", ["METHOD_DIRECT", "METHOD_FOURIER"])
+def test_open_polygons_on_reduced_grids(gpu_lib, method):
+    """Polygons that cannot close (k3 > k1 + k2; a cross spectrum of two
+    separate shells) run on grids smaller than their widest shell (K = 3, 3,
+    14 on M = 24): their sums of D and I products are zero, as the oracle's,
+    and closed polygons in the same batch keep the oracle's values."""
+    P = gpu_lib
+    N, L = 64, 100.0
+    f = lognormal_field(N, 23)
+    kf = 2 * math.pi / L
+    z3, z2 = [0.0] * 3, [0.0] * 2
+    polys = [{"n": 3, "k": [3 * kf, 3 * kf, 14 * kf], "kmin": z3, "kmax": z3},
+             {"n": 2, "k": [3 * kf, 14 * kf], "kmin": z2, "kmax": z2},
+             {"n": 3, "k": [3 * kf, 3 * kf, 5 * kf], "kmin": z3, "kmax": z3},
+             {"n": 2, "k": [3 * kf, 3 * kf], "kmin": z2, "kmax": z2}]
+    with P.Context(N, L, P.PREC_FP64) as ctx:
+        ctx.load_field(f)
+        ctx.set_method(getattr(P, method))
+        b = ctx.batch(polys, 1.0, 0, params={"grid_cap": 512, "chunk_tiles": 1, "method": getattr(P, method)})
+        grids = [w[0] for w in b.waves()]
+        v, npg, sums = b.execute()
+        b.close()
+    assert min(grids) < 2 * 14, grids
+    spec = O.forward_spectrum(f)
+    ref = [O.polyspectrum(spec, p["n"], p["k"], p["kmin"], p["kmax"], 1.0, 0, L) for p in polys]
+    for closed in (2, 3):
+        assert abs(v[closed] - ref[closed][0]) <= 1e-10 * abs(ref[closed][0])
+    for open_, closed in ((0, 2), (1, 3)):
+        sd, si = abs(sums[closed][0]), abs(sums[closed][1])
+        assert abs(ref[open_][3][0]) <= 1e-12 * sd and abs(ref[open_][3][1]) <= 1e-12 * si
+        assert abs(sums[open_][0]) <= 1e-10 * sd, (open_, sums[open_], sums[closed])
+        assert abs(sums[open_][1]) <= 1e-10 * si, (open_, sums[open_], sums[closed])
