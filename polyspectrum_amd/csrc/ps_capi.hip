@@ -384,7 +384,7 @@ static int plain_leg(ps_ctx *c, const MaskParams &mp, const void *spec, int mode
     HIPCK(hipMemcpyAsync(c->dleg, &L, sizeof(LegDev), hipMemcpyHostToDevice, c->st));
     const int tl = g_tile_lines(N, c->fp64);
     const int64_t g1 = (int64_t)(N / 2 + 1) * ((N + tl - 1) / tl);
-    const int64_t g2 = 2 * (int64_t)(N / 2 + 1) * (N / tile_rows(N, c->fp64));
+    const int64_t g2 = 2 * (int64_t)(N / 2 + 1) * (N / g2_tile_cols(N, c->fp64, 1));
     PSCK(run_inverse_g(c->fp64, N, c->dleg, 1, 0, g1, 0, g2, 1 << 30, 1, 0, c->tw, c->st));
     PSCK(run_zc_plain(c->fp64, N, L.g2d, L.g2i, N / 2, spec ? c->accD : nullptr, c->accI, mode, c->partials, c->tw,
                       c->st));
@@ -923,7 +923,7 @@ static int plan_waves(ps_batch *b, std::vector<int> &counts)
         // pair rows (an even number of rows per wavefront)
         w.dmode = (!fourier && zc_dmode_supported(M, fp64)) ? 1 : 0;
         const int parts = w.dmode ? 1 : 2;
-        const int64_t g2_per_kz = M / (tile_rows(M, fp64) / b->g2_div);
+        const int64_t g2_per_kz = M / g2_tile_cols(M, fp64, b->g2_div);
         for (LegDev &L : w.legs) {
             const int K = L.Kz;
             w.kz_cap = std::max(w.kz_cap, K + 1);

@@ -93,6 +93,7 @@ struct SpecGroupDev {
 
 int supported_grid(int N);
 int tile_rows(int N, int fp64);
+int g2_tile_cols(int N, int fp64, int g2_div);  // columns of a G2 tile
 int fused_tile_rows(int N, int fp64);
 int g_tile_lines(int N, int fp64);
 

@@ -192,38 +192,93 @@ template <typename T, int N, int WD>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(band_pass_waves<T, N>()))) k_inv_g2(const LegDev *__restrict__ legs, int nlegs, int64_t t0, int64_t t1, const cplx<T> *__restrict__ tw)
 {
     constexpr int TLC = tile_lines<T, N>() / WD;  // WD: tile width divisor picked by the autotuner
+    constexpr int TOT = TLC * N;
+    constexpr int NL = (TOT + NT - 1) / NT;  // band values per thread of a tile
+    // where they fit, the next tile's band values are loaded into registers
+    // while the current tile is transformed and stored
+    constexpr bool PF = NL <= 8;
     __shared__ cplx<T> lds[N * TLC];
     __shared__ cplx<T> stw[N];
     const int tid = threadIdx.x;
     constexpr int nyt = N / TLC;
     stage_twiddles<T, N>(stw, tw, tid);
-    for (int64_t tile = t0 + blockIdx.x; tile < t1; tile += gridDim.x) {
+    struct Tile {
+        const cplx<T> *g;
+        cplx<T> *o;
+        int kz, y0, Kx, Bx;
+    };
+    auto decode = [&](int64_t tile) -> Tile {
         const int li = find_owner(legs, nlegs, tile, &LegDev::g2_begin);
         const LegDev &L = legs[li];
         const int64_t local = tile - L.g2_begin;
         const int64_t per_part = (int64_t)(L.Kz + 1) * nyt;
         const int part = (int)(local / per_part);
         const int64_t rest = local - part * per_part;
-        const int kz = (int)(rest / nyt);
-        const int y0 = (int)(rest % nyt) * TLC;
-        const cplx<T> *g = reinterpret_cast<const cplx<T> *>(part ? L.g1i : L.g1d);
-        cplx<T> *o = reinterpret_cast<cplx<T> *>(part ? L.g2i : L.g2d);
-        tile_loop_u<TLC * N, 8>(tid, [&](int idx) {
-            const int l = idx % TLC, kx = idx / TLC;
-            const int j = k_to_band(kx, L.Kx, L.Bx, N);
-            cplx<T> d = mk<T>(0, 0);
-            if (j >= 0 && g) d = g[((int64_t)kz * L.Bx + j) * N + y0 + l];
-            lds[lidx<TLC>(kx, l)] = d;
-        });
-        __syncthreads();
-        fft_lds<T, N, TLC, +1>(lds, stw, tid);
-        if (o) {
-            tile_loop_u<TLC * N, 8>(tid, [&](int idx) {
+        Tile t;
+        t.g = reinterpret_cast<const cplx<T> *>(part ? L.g1i : L.g1d);
+        t.o = reinterpret_cast<cplx<T> *>(part ? L.g2i : L.g2d);
+        t.kz = (int)(rest / nyt);
+        t.y0 = (int)(rest % nyt) * TLC;
+        t.Kx = L.Kx;
+        t.Bx = L.Bx;
+        return t;
+    };
+    auto band = [&](const Tile &t, int idx) -> cplx<T> {
+        const int l = idx % TLC, kx = idx / TLC;
+        const int j = k_to_band(kx, t.Kx, t.Bx, N);
+        return (j >= 0 && t.g) ? t.g[((int64_t)t.kz * t.Bx + j) * N + t.y0 + l] : mk<T>(0, 0);
+    };
+    auto put = [&](const Tile &t) {
+        if (t.o) {
+            tile_loop_u<TOT, 8>(tid, [&](int idx) {
                 const int l = idx % TLC, x = idx / TLC;
-                o[((int64_t)kz * N + x) * N + y0 + l] = lds[lidx<TLC>(x, l)];
+                t.o[((int64_t)t.kz * N + x) * N + t.y0 + l] = lds[lidx<TLC>(x, l)];
             });
         }
-        __syncthreads();
+    };
+    if constexpr (PF) {
+        int64_t tile = t0 + blockIdx.x;
+        if (tile >= t1) return;
+        Tile cur = decode(tile);
+        cplx<T> r[NL];
+#pragma unroll
+        for (int m = 0; m < NL; ++m) {
+            const int idx = tid + m * NT;
+            r[m] = (TOT % NT == 0 || idx < TOT) ? band(cur, idx) : mk<T>(0, 0);
+        }
+        while (true) {
+#pragma unroll
+            for (int m = 0; m < NL; ++m) {
+                const int idx = tid + m * NT;
+                if (TOT % NT == 0 || idx < TOT) lds[lidx<TLC>(idx / TLC, idx % TLC)] = r[m];
+            }
+            __syncthreads();
+            const int64_t nt = tile + gridDim.x;
+            Tile nx = cur;
+            if (nt < t1) {
+                nx = decode(nt);
+#pragma unroll
+                for (int m = 0; m < NL; ++m) {
+                    const int idx = tid + m * NT;
+                    r[m] = (TOT % NT == 0 || idx < TOT) ? band(nx, idx) : mk<T>(0, 0);
+                }
+            }
+            fft_lds<T, N, TLC, +1>(lds, stw, tid);
+            put(cur);
+            __syncthreads();
+            if (nt >= t1) break;
+            tile = nt;
+            cur = nx;
+        }
+    } else {
+        for (int64_t tile = t0 + blockIdx.x; tile < t1; tile += gridDim.x) {
+            const Tile cur = decode(tile);
+            tile_loop_u<TOT, 8>(tid, [&](int idx) { lds[lidx<TLC>(idx / TLC, idx % TLC)] = band(cur, idx); });
+            __syncthreads();
+            fft_lds<T, N, TLC, +1>(lds, stw, tid);
+            put(cur);
+            __syncthreads();
+        }
     }
 }
 
@@ -795,7 +850,7 @@ static int launch_inverse_g_n(const LegDev *legs_dev, int nlegs, int64_t g1a, in
     }
     if (g2b > g2a) {
         const unsigned g = (unsigned)((g2b - g2a < grid_cap) ? g2b - g2a : grid_cap);
-        if (g2_div == 2)
+        if (g2_tile_cols(N, sizeof(T) == 8, g2_div) < tile_lines<T, N>())
             hipLaunchKernelGGL((k_inv_g2<T, N, 2>), dim3(g), dim3(NT), 0, st, legs_dev, nlegs, g2a, g2b, twp);
         else
             hipLaunchKernelGGL((k_inv_g2<T, N, 1>), dim3(g), dim3(NT), 0, st, legs_dev, nlegs, g2a, g2b, twp);
@@ -923,6 +978,17 @@ int supported_grid(int N)
 }
 
 int tile_rows(int N, int fp64) { return tile_lines_for(N, fp64 != 0); }
+
+// G2 tile width: half the lines when asked (g2_div = 2), else the full width,
+// or half of it where only the half-width tile's band values fit the
+// registers that prefetch the next tile (at most 8 per thread)
+int g2_tile_cols(int N, int fp64, int g2_div)
+{
+    const int full = tile_lines_for(N, fp64 != 0);
+    if (g2_div == 2) return full / 2;
+    const int per_thread = full * N / NT;
+    return (per_thread > 8 && per_thread / 2 <= 8) ? full / 2 : full;
+}
 
 int fused_tile_rows(int N, int fp64)
 {
