@@ -934,7 +934,8 @@ static int plan_waves(ps_batch *b, std::vector<int> &counts)
             if (w.dmode) {
                 w.g1b_d.push_back(w.g1_tiles_d);
                 w.g2b_d.push_back(w.g2_tiles_d);
-                w.g1_tiles_d += (int64_t)(K + 1) * ((L.Bx + 2 * tl - 1) / (2 * tl));
+                const int ld = g1_dmode_lines(M, fp64);
+                w.g1_tiles_d += (int64_t)(K + 1) * ((L.Bx + ld - 1) / ld);
                 w.g2_tiles_d += (int64_t)(K + 1) * g2_per_kz;
             }
             // G1 reads the spectrum cells of the band box and writes the band

@@ -94,6 +94,7 @@ struct SpecGroupDev {
 int supported_grid(int N);
 int tile_rows(int N, int fp64);
 int g2_tile_cols(int N, int fp64, int g2_div);  // columns of a G2 tile
+int g1_dmode_lines(int N, int fp64);            // kx lines of a D-only G1 tile
 int fused_tile_rows(int N, int fp64);
 int g_tile_lines(int N, int fp64);
 

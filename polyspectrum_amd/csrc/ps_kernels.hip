@@ -186,6 +186,120 @@ k_inv_g1(const LegDev *__restrict__ legs, int nlegs, int64_t t0, int64_t t1, int
     }
 }
 
+// G1 for D parts only (D-only waves): tile = (leg, kz, LPD kx lines).  The
+// next tile's masked cells are gathered into registers while the current
+// tile is transformed and stored, where they fit in 8 values per thread (the
+// tile is halved where only the half tile fits).
+template <typename T, int N> constexpr int g1d_lines_t()
+{
+    constexpr int full = 2 * (tile_lines<T, N>() / 2);
+    constexpr int per = full * N / NT;
+    return (per > 8 && per / 2 <= 8) ? full / 2 : full;
+}
+
+template <typename T, int N>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(band_pass_waves<T, N>())))
+k_inv_g1d(const LegDev *__restrict__ legs, int nlegs, int64_t t0, int64_t t1, const cplx<T> *__restrict__ tw)
+{
+    constexpr int LPD = g1d_lines_t<T, N>();
+    constexpr int TOT = LPD * N;
+    constexpr int NL = (TOT + NT - 1) / NT;
+    constexpr bool PF = NL <= 8;
+    __shared__ cplx<T> lds[N * LPD];
+    __shared__ cplx<T> stw[N];
+    const int tid = threadIdx.x;
+    stage_twiddles<T, N>(stw, tw, tid);
+    struct Tile {
+        int li, kz, j0;
+    };
+    auto decode = [&](int64_t tile) -> Tile {
+        const int li = find_owner(legs, nlegs, tile, &LegDev::g1_begin);
+        const LegDev &L = legs[li];
+        const int nbt = (L.Bx + LPD - 1) / LPD;
+        const int64_t local = tile - L.g1_begin;
+        return Tile{li, (int)(local / nbt), (int)(local % nbt) * LPD};
+    };
+    // masked spectrum cell idx = (line l, ky) of a tile, as in k_inv_g1
+    auto gather = [&](const Tile &t, int idx) -> cplx<T> {
+        const LegDev &L = legs[t.li];
+        const int l = idx / N, ky = idx % N;
+        const int j = t.j0 + l;
+        cplx<T> d = mk<T>(0, 0);
+        if (j < L.Bx) {
+            const int mx = lattice(band_to_k(j, L.Kx, L.Bx, N), N), my = lattice(ky, N), mz = lattice(t.kz, N);
+            const int m2 = mx * mx + my * my + mz * mz;
+            if (L.ivlo >= 0 ? (m2 >= L.ivlo && m2 <= L.ivhi) : mask_eval(L.mask, mx, my, mz)) {
+                const cplx<T> *spec = reinterpret_cast<const cplx<T> *>(L.spec);
+                const int ns = L.ns;
+                d = spec[((int64_t)t.kz * ns + (mx < 0 ? mx + ns : mx)) * ns + (my < 0 ? my + ns : my)];
+                if (L.phase_only) {
+                    // z / |z| (utils_fftw.c:78-93), applied to the cells a shell selects
+                    const T a = hypot(d.x, d.y);
+                    d = mk<T>(d.x / a, d.y / a);
+                }
+                if (L.scale) {
+                    const T inv = (T)(1.0 / *L.scale);
+                    d = mk<T>(d.x * inv, d.y * inv);
+                }
+            }
+        }
+        return d;
+    };
+    auto put = [&](const Tile &t) {
+        const LegDev &L = legs[t.li];
+        cplx<T> *gd = reinterpret_cast<cplx<T> *>(L.g1d);
+        tile_loop_u<TOT, 4>(tid, [&](int idx) {
+            const int l = idx / N, y = idx % N;
+            const int j = t.j0 + l;
+            if (j < L.Bx) gd[((int64_t)t.kz * L.Bx + j) * N + y] = lds[lidx<LPD>(y, l)];
+        });
+    };
+    if constexpr (PF) {
+        int64_t tile = t0 + blockIdx.x;
+        if (tile >= t1) return;
+        Tile cur = decode(tile);
+        cplx<T> r[NL];
+#pragma unroll
+        for (int m = 0; m < NL; ++m) {
+            const int idx = tid + m * NT;
+            r[m] = (TOT % NT == 0 || idx < TOT) ? gather(cur, idx) : mk<T>(0, 0);
+        }
+        while (true) {
+#pragma unroll
+            for (int m = 0; m < NL; ++m) {
+                const int idx = tid + m * NT;
+                if (TOT % NT == 0 || idx < TOT) lds[lidx<LPD>(idx % N, idx / N)] = r[m];
+            }
+            __syncthreads();
+            const int64_t nt = tile + gridDim.x;
+            Tile nx = cur;
+            if (nt < t1) {
+                nx = decode(nt);
+#pragma unroll
+                for (int m = 0; m < NL; ++m) {
+                    const int idx = tid + m * NT;
+                    r[m] = (TOT % NT == 0 || idx < TOT) ? gather(nx, idx) : mk<T>(0, 0);
+                }
+            }
+            fft_lds<T, N, LPD, +1>(lds, stw, tid);
+            put(cur);
+            __syncthreads();
+            if (nt >= t1) break;
+            tile = nt;
+            cur = nx;
+        }
+    } else {
+        for (int64_t tile = t0 + blockIdx.x; tile < t1; tile += gridDim.x) {
+            const Tile cur = decode(tile);
+            tile_loop_u<TOT, 4>(tid, [&](int idx) { lds[lidx<LPD>(idx % N, idx / N)] = gather(cur, idx); });
+            __syncthreads();
+            fft_lds<T, N, LPD, +1>(lds, stw, tid);
+            put(cur);
+            __syncthreads();
+        }
+    }
+}
+
 // G2: C2C along kx from the band to every x.  Tile = (leg, part, kz, y tile
 // of TLC adjacent columns), part 0 = D, 1 = I; stores are TLC-value segments.
 template <typename T, int N, int WD>
@@ -846,7 +960,10 @@ static int launch_inverse_g_n(const LegDev *legs_dev, int nlegs, int64_t g1a, in
     const cplx<T> *twp = reinterpret_cast<const cplx<T> *>(tw);
     if (g1b > g1a) {
         const unsigned g = (unsigned)((g1b - g1a < grid_cap) ? g1b - g1a : grid_cap);
-        hipLaunchKernelGGL((k_inv_g1<T, N>), dim3(g), dim3(NT), 0, st, legs_dev, nlegs, g1a, g1b, dmode, twp);
+        if (dmode)
+            hipLaunchKernelGGL((k_inv_g1d<T, N>), dim3(g), dim3(NT), 0, st, legs_dev, nlegs, g1a, g1b, twp);
+        else
+            hipLaunchKernelGGL((k_inv_g1<T, N>), dim3(g), dim3(NT), 0, st, legs_dev, nlegs, g1a, g1b, dmode, twp);
     }
     if (g2b > g2a) {
         const unsigned g = (unsigned)((g2b - g2a < grid_cap) ? g2b - g2a : grid_cap);
@@ -978,6 +1095,16 @@ int supported_grid(int N)
 }
 
 int tile_rows(int N, int fp64) { return tile_lines_for(N, fp64 != 0); }
+
+int g1_dmode_lines(int N, int fp64)
+{
+    switch (N) {
+#define CASE(n) case n: return fp64 ? g1d_lines_t<double, n>() : g1d_lines_t<float, n>();
+        PS_FOR_EACH_N(CASE)
+#undef CASE
+    }
+    return 0;
+}
 
 // G2 tile width: half the lines when asked (g2_div = 2), else the full width,
 // or half of it where only the half-width tile's band values fit the
