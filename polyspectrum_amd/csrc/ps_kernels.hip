@@ -321,7 +321,14 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(band_pa
         cplx<T> *o;
         int kz, y0, Kx, Bx;
     };
+    // Tiles narrower than a 128-byte line share each line with the next y
+    // tile.  Workgroups are dispatched to the 8 XCDs (8 separate L2 caches)
+    // in turn, so workgroups b and b + 8 share an L2: within every complete
+    // block of 16 tiles, tile 2r + h is processed as tile r + 8h, and both
+    // halves of a line are read through one L2 at about the same time.
+    constexpr bool PAIR = TLC * (int)sizeof(cplx<T>) < 128;
     auto decode = [&](int64_t tile) -> Tile {
+        if (PAIR && (tile & ~(int64_t)15) >= t0 && (tile | 15) < t1) tile = (tile & ~(int64_t)15) | ((tile & 7) << 1) | ((tile >> 3) & 1);
         const int li = find_owner(legs, nlegs, tile, &LegDev::g2_begin);
         const LegDev &L = legs[li];
         const int64_t local = tile - L.g2_begin;
