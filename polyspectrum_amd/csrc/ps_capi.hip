@@ -477,6 +477,8 @@ struct Wave {
     SpecGroupDev *dsgroups = nullptr;
     int64_t zf_tiles = 0, fx_tiles = 0, dot_items = 0;
     int kinds = 0;                       // dot pass instantiations needed: 1 radial, 2 cube-corner masks
+    double cost = 0.0;                   // model bytes of one execution (balances the branch streams)
+    int branch = 0;                      // branch stream when waves run side by side
 };
 
 struct ps_batch {
@@ -506,6 +508,7 @@ struct ps_batch {
     int concurrent = 0;                  // waves have disjoint buffers and run on branch streams
     hipEvent_t fork = nullptr;
     std::vector<hipEvent_t> wend;        // per branch stream: its waves are done
+    std::vector<int> rec_order;          // waves in capture order when side by side
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -604,16 +607,17 @@ static int batch_i_sums(ps_batch *b)
 }
 
 // The launch sequence of one batch (captured into graphs): the waves one
-// after another on s, or forked onto the branch streams (wave i on branch
-// i mod nb) and joined, then the final reduction on s.
+// after another on s, or forked onto the branch streams (each wave on the
+// branch the planner gave it, in the planner's capture order) and joined,
+// then the final reduction on s.
 static int batch_record_on(ps_batch *b, hipStream_t s)
 {
     ps_ctx *c = b->c;
-    const int nw = (int)b->waves.size(), nb = (int)b->wend.size();
+    const int nb = (int)b->wend.size();
     if (b->concurrent && nb > 1) {
         HIPCK(hipEventRecord(b->fork, s));
         for (int k = 0; k < nb; ++k) HIPCK(hipStreamWaitEvent(c->br[k], b->fork, 0));
-        for (int i = 0; i < nw; ++i) PSCK(record_wave(b, b->waves[i], c->br[i % nb]));
+        for (int i : b->rec_order) PSCK(record_wave(b, b->waves[i], c->br[b->waves[i].branch]));
         for (int k = 0; k < nb; ++k) {
             HIPCK(hipEventRecord(b->wend[k], c->br[k]));
             HIPCK(hipStreamWaitEvent(s, b->wend[k], 0));
@@ -908,6 +912,8 @@ static int plan_waves(ps_batch *b, std::vector<int> &counts)
     for (Wave &w : W) {
         const int M = w.M;
         const int tl = g_tile_lines(M, fp64);
+        double wave_bytes0 = 0.0;
+        for (int i = 0; i < PS_MAX_STAGES; ++i) wave_bytes0 += b->stage_bytes[i];
         if (M == N) {
             w.tw = c->tw;
         } else {
@@ -989,6 +995,8 @@ static int plan_waves(ps_batch *b, std::vector<int> &counts)
                 b->stage_bytes[2] += parts * cs * (double)planes * M * M;
             }
         }
+        w.cost = -wave_bytes0;
+        for (int i = 0; i < PS_MAX_STAGES; ++i) w.cost += b->stage_bytes[i];
         arena = std::max(arena, w.bytes);
         total_bytes += w.bytes;
         nscale += w.legs.size();
@@ -1072,6 +1080,26 @@ static int plan_waves(ps_batch *b, std::vector<int> &counts)
     }
     if (b->concurrent) {
         const int nb = std::min((int)W.size(), 4);
+        // waves to branches, largest first onto the least loaded branch; each
+        // branch runs its waves smallest first (the small grids are latency
+        // bound and overlap the large ones), and the capture takes the
+        // branches' first waves first, so every branch starts at once
+        std::vector<int> idx(W.size());
+        for (size_t i = 0; i < W.size(); ++i) idx[i] = (int)i;
+        std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return W[x].cost > W[y].cost; });
+        std::vector<double> load(nb, 0.0);
+        std::vector<std::vector<int>> per(nb);
+        for (int i : idx) {
+            const int k = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+            W[i].branch = k;
+            load[k] += W[i].cost;
+            per[k].push_back(i);
+        }
+        for (auto &v : per) std::reverse(v.begin(), v.end());
+        b->rec_order.clear();
+        for (size_t pos = 0; b->rec_order.size() < W.size(); ++pos)
+            for (int k = 0; k < nb; ++k)
+                if (pos < per[k].size()) b->rec_order.push_back(per[k][pos]);
         while ((int)c->br.size() < nb) {
             hipStream_t t = nullptr;
             PCK(hipStreamCreateWithFlags(&t, hipStreamNonBlocking));  // default priority
