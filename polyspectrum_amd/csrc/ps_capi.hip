@@ -693,6 +693,8 @@ extern "C" int ps_batch_create(ps_ctx *c, ps_polygon *polys, int npoly, double b
     } while (0)
 
 // smallest supported transform grid m >= need, at most the spectrum's N
+// (216 is supported but not used as a reduced grid: its fused pass holds one
+// line per wavefront and is slower than 256 on the 512^3 sweep)
 static int alias_free_grid(int need, int N)
 {
     static const int kGrids[] = {16, 24, 32, 48, 64, 96, 128, 144, 192, 256, 384, 512};

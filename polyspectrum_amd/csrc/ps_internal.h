@@ -13,7 +13,7 @@ constexpr int MAX_SUFFIX = 32;  // suffix legs per fused group (per-wave accumul
 // lines (or rows) per LDS tile: 16 FP32 / 8 FP64 lines of N <= 512 fit 64 KB
 __host__ __device__ constexpr int tile_lines_for(int N, bool fp64)
 {
-    return (N <= 64) ? ((N % 16 == 0) ? 16 : 8) : (fp64 ? 8 : 16);
+    return (N % 16 != 0) ? 8 : ((N <= 64) ? 16 : (fp64 ? 8 : 16));
 }
 
 // One filtered field ("leg"): a shell mask applied to a spectrum (D part) and

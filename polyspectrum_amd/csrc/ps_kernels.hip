@@ -1039,7 +1039,7 @@ static int launch_zc_fused_n(const GroupDev *groups, int ngroups, const LegDev *
                                           kz_cap, tw, st);
 }
 
-#define PS_FOR_EACH_N(X) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(144) X(192) X(256) X(384) X(512)
+#define PS_FOR_EACH_N(X) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(144) X(192) X(216) X(256) X(384) X(512)
 
 template <typename T>
 static int dispatch_forward(int N, const void *field, int fd, void *H, const void *tw, hipStream_t st)

@@ -581,7 +581,7 @@ static int launch_spec_dot_n(const SpecGroupDev *g, int ng, const MaskParams *ma
     return (int)hipGetLastError();
 }
 
-#define PS_FOR_EACH_N(X) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(144) X(192) X(256) X(384) X(512)
+#define PS_FOR_EACH_N(X) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(144) X(192) X(216) X(256) X(384) X(512)
 
 int spec_zf_rows(int N, int fp64)
 {

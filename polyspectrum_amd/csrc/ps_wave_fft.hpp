@@ -26,12 +26,21 @@ constexpr int largest_divisor_leq(int n, int cap)
     return 1;
 }
 
+constexpr int largest_even_divisor_leq(int n, int cap)
+{
+    for (int d = cap; d >= 2; --d)
+        if (n % d == 0 && d % 2 == 0) return d;
+    return 0;
+}
+
 // lines per wavefront: about 1024 complex values in flight, dividing N/4 so
-// that 4 wavefronts cover a whole number of rows of a tile
+// that 4 wavefronts cover a whole number of rows of a tile; an even count
+// where one divides (the D-only fused pass pairs rows)
 template <typename T, int N> constexpr int wave_lines()
 {
     constexpr int cap = (1024 / N < 1) ? 1 : 1024 / N;
-    constexpr int d = largest_divisor_leq(N / 4, cap);
+    constexpr int de = largest_even_divisor_leq(N / 4, cap);
+    constexpr int d = de > 0 ? de : largest_divisor_leq(N / 4, cap);
     return (sizeof(T) == 8 && d > 1) ? d / 2 : d;
 }
 

@@ -40,7 +40,7 @@ def test_python_binding_loads():
     L = P.load_library()
     assert L.ps_strerror(0) == b"ok"
     assert b"unsupported" in L.ps_strerror(-2)
-    for n in (16, 24, 32, 48, 64, 96, 128, 144, 192, 256, 384, 512):
+    for n in (16, 24, 32, 48, 64, 96, 128, 144, 192, 216, 256, 384, 512):
         assert L.ps_grid_supported(n) == 1
     for n in (0, 10, 20, 1024, 640):
         assert L.ps_grid_supported(n) == 0

@@ -231,15 +231,18 @@ def test_direct_groups_split_by_grid(gpu_lib):
         assert abs(v[i] - ref) <= 1e-10 * abs(ref), (i, v[i], ref)
 
 
-def test_reduced_grid_144(gpu_lib):
-    """Triangles whose product needs 128 < M <= 144 run on the 144 grid
-    (radix 3, 3, 8, 2); FP64 equals the oracle on the 192 grid and the mixed
-    path equals FP64 to the mixed bar."""
+@pytest.mark.parametrize("N,L,m,grid", [(192, 200.0, (40.0, 40.0, (50.0, 55.0, 70.0)), 144),
+                                         (216, 250.0, (60.0, 60.0, (80.0, 85.0, 30.0)), 216)])
+def test_reduced_grid_3x3(gpu_lib, N, L, m, grid):
+    """Grids 2^a 3^b with b >= 2: triangles whose product needs 128 < M <= 144
+    run on the 144 grid (radix 3, 3, 8, 2) inside a 192 context; on a 216
+    context (radix 3, 3, 3, 8; wavefront lines and tile columns not multiples
+    of 16) triangles run on the full grid or a reduced one.  FP64 equals the
+    oracle and the mixed path equals FP64 to the mixed bar."""
     P = gpu_lib
-    N, L = 192, 200.0
-    f = lognormal_field(N, 144)
+    f = lognormal_field(N, grid)
     kf = 2 * math.pi / L
-    polys = [{"n": 3, "k": [40 * kf, 40 * kf, m * kf], "kmin": [0.0] * 3, "kmax": [0.0] * 3} for m in (50.0, 55.0, 70.0)]
+    polys = [{"n": 3, "k": [m[0] * kf, m[1] * kf, m3 * kf], "kmin": [0.0] * 3, "kmax": [0.0] * 3} for m3 in m[2]]
     res = {}
     for prec in (P.PREC_FP64, P.PREC_MIXED):
         with P.Context(N, L, prec) as ctx:
@@ -248,7 +251,7 @@ def test_reduced_grid_144(gpu_lib):
             grids = [w[0] for w in b.waves()]
             res[prec] = b.execute()
             b.close()
-        assert 144 in grids, grids
+        assert grid in grids, grids
     spec = O.forward_spectrum(f)
     for i in (0, 2):
         p = polys[i]

@@ -79,7 +79,7 @@ def test_plain_fp64_matches_oracle(gpu_lib, N, L, n, m1, m2, theta, case, bw, ph
     assert abs(sums[1] - rsums[1].real) <= 1e-9 * abs(rsums[1].real)
 
 
-@pytest.mark.parametrize("N", [16, 24, 32, 48, 64, 96, 128, 144])
+@pytest.mark.parametrize("N", [16, 24, 32, 48, 64, 96, 128, 144, 216])
 @pytest.mark.parametrize("prec", [1, 0])
 def test_forward_spectrum(gpu_lib, N, prec):
     """R2C of the GPU (layout [kz][kx][ky]) against numpy's rfftn."""
