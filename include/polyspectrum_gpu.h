@@ -89,7 +89,8 @@ const char *ps_strerror(int status);
 /* Number of GPUs visible to HIP (0 when none). */
 int ps_device_count(void);
 
-/* Whether an N^3 grid has a compiled FFT plan (N = 2^a 3^b, 16..512). */
+/* Whether an N^3 grid has a compiled FFT plan (N = 16, 24, 32, 48, 64, 96,
+ * 128, 144, 192, 216, 256, 384 or 512). */
 int ps_grid_supported(int nbins);
 
 /* Create a context for N^3 grids of side boxsize (h^-1 Mpc) on the current
