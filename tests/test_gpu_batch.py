@@ -293,3 +293,29 @@ def test_open_polygons_on_reduced_grids(gpu_lib, method):
         assert abs(ref[open_][3][0]) <= 1e-12 * sd and abs(ref[open_][3][1]) <= 1e-12 * si
         assert abs(sums[open_][0]) <= 1e-10 * sd, (open_, sums[open_], sums[closed])
         assert abs(sums[open_][1]) <= 1e-10 * si, (open_, sums[open_], sums[closed])
+
+
+@pytest.mark.parametrize("prec_name", ["PREC_FP64", "PREC_MIXED"])
+def test_persistent_grid_sizes_agree(gpu_lib, prec_name):
+    """A persistent grid of 3 workgroups runs every pass through its
+    prefetching tile loop many times (next tile loaded while the current one
+    is transformed); the values equal those of a grid with one workgroup per
+    tile, bit for bit, with either G2 tile width."""
+    P = gpu_lib
+    N, L = 64, 100.0
+    f = lognormal_field(N, 3)
+    kf = 2 * math.pi / L
+    polys = _polys_triangles(kf, [(7.0, 7.0), (4.0, 9.0)], [0.5, 1.4, 2.5], 2)
+    out = {}
+    with P.Context(N, L, getattr(P, prec_name)) as ctx:
+        ctx.load_field(f)
+        for cap in (3, 4096):
+            for div in (1, 2):
+                b = ctx.batch(polys, 1.0, 2, params={"grid_cap": cap, "chunk_tiles": 1, "g2_div": div,
+                                                     "method": P.METHOD_DIRECT})
+                out[(cap, div)] = b.execute()
+                b.close()
+    for div in (1, 2):
+        np.testing.assert_array_equal(out[(3, div)][0], out[(4096, div)][0])
+        np.testing.assert_array_equal(out[(3, div)][1], out[(4096, div)][1])
+    np.testing.assert_allclose(out[(3, 1)][0], out[(3, 2)][0], rtol=1e-12 if prec_name == "PREC_FP64" else 1e-5)
