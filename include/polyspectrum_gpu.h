@@ -220,8 +220,11 @@ int ps_forward_profile(ps_ctx *ctx, int slot, int reps, double *ms, double *byte
  * it evaluates L_A[D_B E] + L_B[D_A E] + L_W[D_A D_B] with L_X = IFFT * mask_X * FFT,
  * D_X = L_X delta, W = sum_t a_t mask_t and E = L_W delta: every filtered field
  * is recomputed, none is stored per polygon.  grad: N^3 values (float64 if
- * grad_is_double, else float32), host memory unless on_device.  Not available
- * with useOnlyPhase (PS_ERR_INVALID).  No counterpart in the reference. */
+ * grad_is_double, else float32), host memory unless on_device.  The first call
+ * allocates its work buffers (two half spectra and five float64 N^3 fields,
+ * 6.4 GB at 512^3 in mixed precision) and the batch keeps them until
+ * ps_batch_destroy.  Not available with useOnlyPhase (PS_ERR_INVALID).  No
+ * counterpart in the reference. */
 int ps_batch_adjoint(ps_batch *b, int slot, const double *weights, void *grad, int grad_is_double, int on_device);
 
 /* Re-run the forward transform of a slot from the last host upload (timing

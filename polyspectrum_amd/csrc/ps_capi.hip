@@ -509,6 +509,7 @@ struct ps_batch {
     hipEvent_t fork = nullptr;
     std::vector<hipEvent_t> wend;        // per branch stream: its waves are done
     std::vector<int> rec_order;          // waves in capture order when side by side
+    void *adj[9] = {};                   // adjoint work buffers, kept after the first ps_batch_adjoint
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -547,6 +548,7 @@ extern "C" int ps_batch_destroy(ps_batch *b)
     }
     void *bufs[] = {b->dcounts, b->dpartials, b->dsum, b->dscale, b->dmasks, b->divl, b->arena};
     for (void *p : bufs) if (p) (void)hipFree(p);
+    for (void *p : b->adj) if (p) (void)hipFree(p);
     delete b;
     return PS_OK;
 }
@@ -1527,11 +1529,10 @@ extern "C" int ps_batch_adjoint(ps_batch *b, int slot, const double *weights, vo
         const double cn = pow(volume, n - 1) * pow(c->N, -n * 3);
         coef[o] += weights[p] * cn / b->last_sumI[o];
     }
+    int st = PS_OK;
     void *H1 = nullptr, *H2 = nullptr;
     double *DA = nullptr, *DB = nullptr, *E = nullptr, *Q = nullptr, *G = nullptr, *dw = nullptr;
     MaskParams *dm = nullptr;
-    int st = PS_OK;
-    const int maxm = MAX_SUFFIX + 2;
 #define ACK(call)                                                   \
     do {                                                            \
         hipError_t e_ = (call);                                     \
@@ -1542,19 +1543,36 @@ extern "C" int ps_batch_adjoint(ps_batch *b, int slot, const double *weights, vo
         int r_ = (call);                                            \
         if (r_ != PS_OK) { st = (r_ > 0) ? PS_ERR_HIP : r_; goto done; } \
     } while (0)
-    ACK(hipMalloc(&H1, hb));
-    ACK(hipMalloc(&H2, hb));
-    ACK(hipMalloc(&DA, fb));
-    ACK(hipMalloc(&DB, fb));
-    ACK(hipMalloc(&E, fb));
-    ACK(hipMalloc(&Q, fb));
-    ACK(hipMalloc(&G, fb));
-    ACK(hipMalloc(&dm, sizeof(MaskParams) * maxm * 4));
-    ACK(hipMalloc(&dw, sizeof(double) * maxm * 4));
+    {
+        // two half spectra, five FP64 fields and the mask lists, allocated by
+        // the first call and kept with the batch
+        const size_t nml = b->suffix.size() + 2;  // leg A, leg B, every suffix leg of the batch
+        const size_t sz[9] = {hb, hb, fb, fb, fb, fb, fb, sizeof(MaskParams) * nml, sizeof(double) * nml};
+        for (int i = 0; i < 9; ++i)
+            if (!b->adj[i]) ACK(hipMalloc(&b->adj[i], sz[i]));
+    }
+    H1 = b->adj[0];
+    H2 = b->adj[1];
+    DA = (double *)b->adj[2];
+    DB = (double *)b->adj[3];
+    E = (double *)b->adj[4];
+    Q = (double *)b->adj[5];
+    G = (double *)b->adj[6];
+    dm = (MaskParams *)b->adj[7];
+    dw = (double *)b->adj[8];
     RCK(run_fill(G, cells, 0.0, c->st));
-    for (const GroupDev &g : b->groups) {
-        const int first_leg = b->suffix[g.suf_begin];
-        if (b->leg_slot[first_leg] != slot) continue;
+    {
+    // groups with the same prefix legs (split by transform grid or by the
+    // suffix limit when the batch was planned) form one weighted mask W
+    std::map<std::tuple<int, int, int, int>, std::vector<int>> prefix_groups;
+    for (int gi = 0; gi < (int)b->groups.size(); ++gi) {
+        const GroupDev &g = b->groups[gi];
+        if (b->leg_slot[b->suffix[g.suf_begin]] != slot) continue;
+        prefix_groups[std::make_tuple(g.npre, g.pre_sq, g.npre >= 1 ? g.preA : -1, g.npre == 2 ? g.preB : -1)]
+            .push_back(gi);
+    }
+    for (const auto &kv : prefix_groups) {
+        const GroupDev &g = b->groups[kv.second[0]];
         // host mask lists: [0] leg A, [1] leg B, [2..] the weighted suffix masks W
         std::vector<MaskParams> hm;
         std::vector<double> hw;
@@ -1565,19 +1583,23 @@ extern "C" int ps_batch_adjoint(ps_batch *b, int slot, const double *weights, vo
         hw.push_back(1.0);
         hm.push_back(g.npre == 2 ? b->legs[g.preB].mask : MaskParams{});
         hw.push_back(1.0);
-        for (int q = 0; q < g.nsuf; ++q) {
-            hm.push_back(b->legs[b->suffix[g.suf_begin + q]].mask);
-            hw.push_back(two * coef[g.out_begin + q]);
+        for (int gi : kv.second) {
+            const GroupDev &h = b->groups[gi];
+            for (int q = 0; q < h.nsuf; ++q) {
+                hm.push_back(b->legs[b->suffix[h.suf_begin + q]].mask);
+                hw.push_back(two * coef[h.out_begin + q]);
+            }
         }
-        // slot 0 of the device lists: A (1 mask), slot 1: B, slot 2: W (nsuf masks)
-        MaskParams *mA = dm, *mB = dm + maxm, *mW = dm + 2 * maxm;
-        double *wA = dw, *wB = dw + maxm, *wW = dw + 2 * maxm;
+        const int nsw = (int)hm.size() - 2;
+        // device lists: [0] A, [1] B, [2..] W (nsw masks)
+        MaskParams *mA = dm, *mB = dm + 1, *mW = dm + 2;
+        double *wA = dw, *wB = dw + 1, *wW = dw + 2;
         ACK(hipMemcpyAsync(mA, &hm[0], sizeof(MaskParams), hipMemcpyHostToDevice, c->st));
         ACK(hipMemcpyAsync(wA, &hw[0], sizeof(double), hipMemcpyHostToDevice, c->st));
         ACK(hipMemcpyAsync(mB, &hm[1], sizeof(MaskParams), hipMemcpyHostToDevice, c->st));
         ACK(hipMemcpyAsync(wB, &hw[1], sizeof(double), hipMemcpyHostToDevice, c->st));
-        ACK(hipMemcpyAsync(mW, &hm[2], sizeof(MaskParams) * g.nsuf, hipMemcpyHostToDevice, c->st));
-        ACK(hipMemcpyAsync(wW, &hw[2], sizeof(double) * g.nsuf, hipMemcpyHostToDevice, c->st));
+        ACK(hipMemcpyAsync(mW, &hm[2], sizeof(MaskParams) * nsw, hipMemcpyHostToDevice, c->st));
+        ACK(hipMemcpyAsync(wW, &hw[2], sizeof(double) * nsw, hipMemcpyHostToDevice, c->st));
         const void *spec = c->spec[slot];
         auto filtered = [&](const MaskParams *m, const double *w, int nm, double *out) -> int {
             // out = IFFT(mask * spectrum)
@@ -1597,40 +1619,41 @@ extern "C" int ps_batch_adjoint(ps_batch *b, int slot, const double *weights, vo
             // S_t = sum D_t^2: dS_t = 2 L_t delta (masks are idempotent), the
             // factor 2 is in the weights
             ACK(hipMemcpyAsync(H1, spec, hb, hipMemcpyDeviceToDevice, c->st));
-            RCK(run_apply_masks(c->fp64, N, H1, mW, wW, g.nsuf, c->st));
+            RCK(run_apply_masks(c->fp64, N, H1, mW, wW, nsw, c->st));
             RCK(run_inverse_real(c->fp64, N, H1, G, 1, c->tw, c->st));
         } else if (g.npre == 1 && !g.pre_sq) {
             // S_t = sum D_A D_t: dS = L_A[E] + L_W[D_A] = 2 IFFT(F_A W delta)
             ACK(hipMemcpyAsync(H1, spec, hb, hipMemcpyDeviceToDevice, c->st));
             RCK(run_apply_masks(c->fp64, N, H1, mA, wA, 1, c->st));
-            RCK(run_apply_masks(c->fp64, N, H1, mW, wW, g.nsuf, c->st));
+            RCK(run_apply_masks(c->fp64, N, H1, mW, wW, nsw, c->st));
             RCK(run_inverse_real(c->fp64, N, H1, G, 1, c->tw, c->st));
         } else if (g.pre_sq) {
             // S_t = sum D_A^2 D_t: grad = L_A[2 D_A E] + L_W[D_A^2]
             RCK(filtered(mA, wA, 1, DA));
-            RCK(filtered(mW, wW, g.nsuf, E));
+            RCK(filtered(mW, wW, nsw, E));
             RCK(run_mul_fields(Q, DA, E, 2.0, cells, c->st));
             RCK(project(Q, H2, mA, wA, 1));
             RCK(run_mul_fields(Q, DA, nullptr, 1.0, cells, c->st));
-            RCK(project(Q, H1, mW, wW, g.nsuf));
+            RCK(project(Q, H1, mW, wW, nsw));
             RCK(run_add_spec(c->fp64, H1, H2, half, c->st));
             RCK(run_inverse_real(c->fp64, N, H1, G, 1, c->tw, c->st));
         } else {
             // S_t = sum D_A D_B D_t: grad = L_A[D_B E] + L_B[D_A E] + L_W[D_A D_B]
             RCK(filtered(mA, wA, 1, DA));
             RCK(filtered(mB, wB, 1, DB));
-            RCK(filtered(mW, wW, g.nsuf, E));
+            RCK(filtered(mW, wW, nsw, E));
             RCK(run_mul_fields(Q, DB, E, 1.0, cells, c->st));
             RCK(project(Q, H2, mA, wA, 1));
             RCK(run_mul_fields(Q, DA, E, 1.0, cells, c->st));
             RCK(project(Q, H1, mB, wB, 1));
             RCK(run_add_spec(c->fp64, H2, H1, half, c->st));
             RCK(run_mul_fields(Q, DA, DB, 1.0, cells, c->st));
-            RCK(project(Q, H1, mW, wW, g.nsuf));
+            RCK(project(Q, H1, mW, wW, nsw));
             RCK(run_add_spec(c->fp64, H1, H2, half, c->st));
             RCK(run_inverse_real(c->fp64, N, H1, G, 1, c->tw, c->st));
         }
         ACK(hipStreamSynchronize(c->st));  // the host mask lists are reused
+    }
     }
     if (on_device) {
         if (grad_is_double) {
@@ -1651,8 +1674,6 @@ done:
 #undef ACK
 #undef RCK
     (void)hipStreamSynchronize(c->st);
-    void *bufs[] = {H1, H2, DA, DB, E, Q, G, dm, dw};
-    for (void *p : bufs) if (p) (void)hipFree(p);
     return st;
 }
 
