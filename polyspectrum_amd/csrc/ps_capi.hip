@@ -1289,22 +1289,40 @@ extern "C" int ps_batch_create_ex(ps_ctx *c, ps_polygon *polys, int npoly, doubl
     for (auto &kv : glist) {
         const int kind = std::get<0>(kv.first);
         std::vector<std::vector<int>> classes{kv.second};
+        std::vector<int> class_chunk{MAX_SUFFIX};
         if (b->method == PS_METHOD_DIRECT) {
             const int a = std::get<1>(kv.first), a2 = std::get<2>(kv.first);
             const int KA = (a >= 0) ? b->legs[a].Kz : 0;
             const int KB = (a2 >= 0) ? b->legs[a2].Kz : 0;
             std::vector<int> grid;
+            std::map<int, int> grid_of;
             for (int p : kv.second) {
                 const int Kt = b->legs[kind == 1 ? pleg[p][0] : pleg[p][polys[p].n - 1]].Kz;
                 grid.push_back(alias_free_grid((kind == 1 ? 2 * Kt : KA + KB + Kt) + 1, N));
+                grid_of[p] = grid.back();
             }
             const int npre = (kind == 3) ? (a == a2 ? 1 : 2) : (kind == 2 ? 1 : 0);
             classes = split_by_grid(kv.second, grid, npre);
+            // a small grid has few fused-pass tiles (work items), each running
+            // through every leg of its group: there its suffixes are split
+            // into more groups (each recomputing the prefix) so that about
+            // 256 items run side by side
+            class_chunk.clear();
+            for (const std::vector<int> &cl : classes) {
+                int M = 0;
+                for (int p : cl) M = std::max(M, grid_of[p]);
+                const int64_t tiles = (int64_t)M * (M / fused_tile_rows(M, c->fp64));
+                const int split = (int)std::min<int64_t>((int64_t)cl.size(), std::max<int64_t>(1, (256 + tiles - 1) / tiles));
+                class_chunk.push_back(std::min<int>(MAX_SUFFIX, (int)((cl.size() + split - 1) / split)));
+            }
         }
         std::vector<std::vector<int>> chunks;  // at most MAX_SUFFIX polygons each
-        for (const std::vector<int> &cl : classes)
-            for (size_t s0 = 0; s0 < cl.size(); s0 += MAX_SUFFIX)
-                chunks.emplace_back(cl.begin() + s0, cl.begin() + std::min(cl.size(), s0 + MAX_SUFFIX));
+        for (size_t ci = 0; ci < classes.size(); ++ci) {
+            const std::vector<int> &cl = classes[ci];
+            const size_t cs = (size_t)class_chunk[ci];
+            for (size_t s0 = 0; s0 < cl.size(); s0 += cs)
+                chunks.emplace_back(cl.begin() + s0, cl.begin() + std::min(cl.size(), s0 + cs));
+        }
         for (const std::vector<int> &ps_ : chunks) {
             GroupDev G{};
             if (kind == 3) {
