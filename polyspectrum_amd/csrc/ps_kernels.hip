@@ -110,9 +110,9 @@ __global__ void __launch_bounds__(NT) k_c2c_strided(cplx<T> *__restrict__ data, 
 
 // ================================================================ inverse
 
-// G1: masked gather + C2C along ky.  Tile = (leg, kz, band tile of kx
-// lines); LDS holds TLC D-lines (mask * spectrum) and TLC I-lines (mask), or
-// (dmode: D parts only) 2 TLC D-lines.
+// G1: masked gather + C2C along ky.  Tile = (leg, kz, band tile of TLC kx
+// lines); LDS holds TLC D-lines (mask * spectrum) and TLC I-lines (mask).
+// (D parts only: k_inv_g1d below.)
 // 4 wavefronts per SIMD where the transform's registers allow (N < 384): the
 // gather and band passes are latency bound (measured: G1 0.57 -> 0.44 ms, G2
 // 0.89 -> 0.77 ms on the 512^3 sweep)
@@ -120,15 +120,14 @@ template <typename T, int N> constexpr int band_pass_waves() { return (N >= 384)
 
 template <typename T, int N>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(band_pass_waves<T, N>())))
-k_inv_g1(const LegDev *__restrict__ legs, int nlegs, int64_t t0, int64_t t1, int dmode,
-         const cplx<T> *__restrict__ tw)
+k_inv_g1(const LegDev *__restrict__ legs, int nlegs, int64_t t0, int64_t t1, const cplx<T> *__restrict__ tw)
 {
     constexpr int TLC = tile_lines<T, N>() / 2;
     constexpr int TL2 = 2 * TLC;
     __shared__ cplx<T> lds[N * TL2];
     __shared__ cplx<T> stw[N];
     const int tid = threadIdx.x;
-    const int lpt = dmode ? TL2 : TLC;  // kx lines per tile
+    constexpr int lpt = TLC;  // kx lines per tile
     stage_twiddles<T, N>(stw, tw, tid);
     for (int64_t tile = t0 + blockIdx.x; tile < t1; tile += gridDim.x) {
         const int li = find_owner(legs, nlegs, tile, &LegDev::g1_begin);
@@ -163,10 +162,9 @@ k_inv_g1(const LegDev *__restrict__ legs, int nlegs, int64_t t0, int64_t t1, int
                 }
             }
             lds[lidx<TL2>(ky, l)] = d;
-            if (!dmode) lds[lidx<TL2>(ky, l + TLC)] = u;
+            lds[lidx<TL2>(ky, l + TLC)] = u;
         };
-        if (dmode) tile_loop_u<TL2 * N, 4>(tid, gather);
-        else tile_loop_u<TLC * N, 4>(tid, gather);
+        tile_loop_u<TLC * N, 4>(tid, gather);
         __syncthreads();
         fft_lds<T, N, TL2, +1>(lds, stw, tid);
         cplx<T> *gd = reinterpret_cast<cplx<T> *>(L.g1d);
@@ -177,11 +175,10 @@ k_inv_g1(const LegDev *__restrict__ legs, int nlegs, int64_t t0, int64_t t1, int
             if (j < L.Bx) {
                 const int64_t o = ((int64_t)kz * L.Bx + j) * N + y;
                 if (gd) gd[o] = lds[lidx<TL2>(y, l)];
-                if (!dmode) gi[o] = lds[lidx<TL2>(y, l + TLC)];
+                gi[o] = lds[lidx<TL2>(y, l + TLC)];
             }
         };
-        if (dmode) tile_loop_u<TL2 * N, 4>(tid, put);
-        else tile_loop_u<TLC * N, 4>(tid, put);
+        tile_loop_u<TLC * N, 4>(tid, put);
         __syncthreads();
     }
 }
@@ -970,7 +967,7 @@ static int launch_inverse_g_n(const LegDev *legs_dev, int nlegs, int64_t g1a, in
         if (dmode)
             hipLaunchKernelGGL((k_inv_g1d<T, N>), dim3(g), dim3(NT), 0, st, legs_dev, nlegs, g1a, g1b, twp);
         else
-            hipLaunchKernelGGL((k_inv_g1<T, N>), dim3(g), dim3(NT), 0, st, legs_dev, nlegs, g1a, g1b, dmode, twp);
+            hipLaunchKernelGGL((k_inv_g1<T, N>), dim3(g), dim3(NT), 0, st, legs_dev, nlegs, g1a, g1b, twp);
     }
     if (g2b > g2a) {
         const unsigned g = (unsigned)((g2b - g2a < grid_cap) ? g2b - g2a : grid_cap);
