@@ -208,10 +208,10 @@ template <int TOT, int U, class F> __device__ __forceinline__ void tile_loop_u(i
 // their twiddles inside the dependency chain of every butterfly, where a
 // global (cache) read would stall the stage.  The caller's next barrier
 // publishes it.
-template <typename T, int N>
+template <typename T, int N, int NTH = NT>
 __device__ __forceinline__ void stage_twiddles(cplx<T> *stw, const cplx<T> *__restrict__ tw, int tid)
 {
-    for (int i = tid; i < N; i += NT) stw[i] = tw[i];
+    for (int i = tid; i < N; i += NTH) stw[i] = tw[i];
 }
 
 template <typename T, int N, int TLC, int S, int ST = 0>

@@ -508,8 +508,9 @@ __global__ void __launch_bounds__(NT) k_inv_zc_plain(const cplx<T> *__restrict__
 template <typename T, int N, bool DM = false> struct ZcCfg {
     static constexpr int LW0 = wave_lines<T, N>();  // y rows per wavefront
     static constexpr int LW = DM ? LW0 / 2 : LW0;   // transformed lines per wavefront
-    // 4 wavefronts: measured faster at 512^3 than 8 (whose 16-row tiles read
-    // whole 128 B lines per plane but synchronise twice as many wavefronts)
+    // 4 wavefronts: measured faster on the 512^3 sweep than 8 (whose 16-row
+    // tiles read whole 128 B lines per plane but synchronise twice as many
+    // wavefronts), 2 (Zc 628 -> 676 us) or 1 (853 us)
     static constexpr int NWV = 4;
     static constexpr int BT = 64 * NWV;  // threads per workgroup
     static constexpr int ROWS = NWV * LW0;  // y rows of a tile
@@ -555,7 +556,7 @@ __global__ void __launch_bounds__((ZcCfg<T, N, DM>::BT)) __attribute__((amdgpu_w
     constexpr bool CMP = zc_compact<T, N>();
     __shared__ cplx<T> stw[CMP ? N : 1];
     WaveTw<T, N, LW> twr;
-    if constexpr (CMP) stage_twiddles<T, N>(stw, twg, tid);
+    if constexpr (CMP) stage_twiddles<T, N, BT>(stw, twg, tid);
     else twr.template load<+1>(twg, lane);
     cplx<T> *wb = wbuf + w * LW * LP;
     constexpr int tiles_per_x = N / ROWS;
