@@ -5,10 +5,15 @@
 * 256^3 binary ionisation map (float64 input, 50% filling), L = 200: equilateral
   triangles k = m k_F for m = 1..64, Npolygons written.
 
+* 512^3 log-normal, L = 300 (the benchmark sweep's field): k1 = k2 = 0.4 and
+  1.0 h/Mpc, 6 theta bins each, GIVEN binning (grid classes and reduced grids).
+
 The oracle (FP64, one core) is run on sampled configurations; every
 configuration is checked through size-independent properties: integer polygon
 counts, reference-shaped FP64 path = batched FP64 path, mixed within 1e-5 of
-FP64 (relative to the largest |B| of the sweep where B crosses zero).
+FP64 (relative to the largest |B| of the sweep where B crosses zero), and at
+512^3 B(2 delta) = 8 B(delta) (a power-of-two scale is exact in binary floating
+point, so every transform and product scales exactly).
 """
 import math
 
@@ -90,3 +95,46 @@ def test_config_256_equilateral_xhii(gpu_lib):
     # B(m) of a binary map is small at some m: bound those relative to the sweep
     assert np.median(err) < 1e-6 and scale_err.max() < 1e-5
     assert np.mean(err <= 1e-5) >= 0.95
+
+
+def test_config_512_sweep_properties(gpu_lib):
+    """The benchmark's 512^3 field: batched FP64 = reference-shaped FP64 path
+    (full grid) to 1e-10, mixed = FP64 within 1e-5 of each k1 = k2 group's
+    largest |B|, integer polygon counts (FP64), and exact cubic scaling."""
+    P = gpu_lib
+    N, L = 512, 300.0
+    field = lognormal_field(N, 2000, sigma=1.0, alpha=1.5)
+    polys = []
+    for k in (0.4, 1.0):
+        for i in range(6):
+            th = PI * (3 * i + 1) / 18
+            k3 = math.sqrt(2 * k * k - 2 * k * k * math.cos(th))
+            lo = math.sqrt(2 * k * k - 2 * k * k * min(1., math.cos(th) + 0.025))
+            hi = math.sqrt(2 * k * k - 2 * k * k * max(-1., math.cos(th) - 0.025))
+            polys.append({"n": 3, "k": [k, k, k3], "kmin": [k, k, lo], "kmax": [k, k, hi]})
+    case = P.BINNING["GIVEN_BINNING"]
+    with P.Context(N, L, P.PREC_FP64) as ctx:
+        ctx.load_field(field)
+        b = ctx.batch(polys, 1.0, case)
+        v64, n64, _ = b.execute()
+        assert len({w[0] for w in b.waves()}) >= 2
+        b.close()
+        for i in (0, 11):
+            p = polys[i]
+            plain = ctx.polyspectrum(3, p["k"], p["kmin"], p["kmax"], 1.0, case)[0]
+            assert abs(v64[i] - plain) <= 1e-10 * abs(plain), (i, v64[i], plain)
+    assert np.all(np.abs(n64 - np.round(n64)) < 1e-6 * np.maximum(1.0, n64))
+    with P.Context(N, L, P.PREC_MIXED) as ctx:
+        ctx.load_field(field)
+        b = ctx.batch(polys, 1.0, case)
+        v32, n32, _ = b.execute()
+        ctx.load_field((2.0 * field).astype(field.dtype))
+        v2, _, _ = b.execute()
+        b.close()
+    # mixed precision counts the polygons with FP32 transforms of the bare
+    # masks: within 1e-6 of the exact FP64 integers (up to 3.5e7 here)
+    np.testing.assert_allclose(n32, n64, rtol=1e-6, atol=0)
+    for g in (slice(0, 6), slice(6, 12)):
+        scale = np.max(np.abs(v64[g]))
+        assert np.max(np.abs(v32[g] - v64[g])) <= 1e-5 * scale
+    np.testing.assert_allclose(v2, 8.0 * v32, rtol=1e-13, atol=0)
