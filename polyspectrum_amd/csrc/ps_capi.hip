@@ -590,11 +590,109 @@ static int record_wave(ps_batch *b, Wave &w, hipStream_t ws)
     return PS_OK;
 }
 
+// Mixed precision: the I sums taken with FP64 transforms of the bare masks on
+// temporary FP64 buffers, wave by wave, so that the polygon counts are exact
+// integers as the reference's (FP32 transforms leave a relative error of about
+// 1e-7: a few polygons in 3.5e7).  The FP64 tiles and work items are numbered
+// here; the I sums go to dsum[nout ..].
+static int batch_i_sums_fp64(ps_batch *b)
+{
+    ps_ctx *c = b->c;
+    const size_t cs = sizeof(double) * 2;
+    int st = PS_OK;
+    void *partials = nullptr, *dcounts = nullptr, *dsumD = nullptr;
+    // work items per group in the FP64 numbering, and per output its count
+    std::vector<int> counts(b->nout, 0);
+    int max_chunks = 1;
+    for (Wave &w : b->waves) {
+        const int M = w.M;
+        const int64_t tiles = (int64_t)M * (M / fused_tile_rows(M, 1));
+        const int nchunks = (int)((tiles + b->chunk_tiles - 1) / b->chunk_tiles);
+        for (const GroupDev &G : w.groups)
+            for (int o = 0; o < G.nsuf; ++o) counts[G.out_begin + o] = nchunks;
+        max_chunks = std::max(max_chunks, nchunks);
+    }
+    if (hipMalloc(&partials, sizeof(double2) * 2 * (size_t)b->nout * max_chunks) != hipSuccess ||
+        hipMalloc(&dcounts, sizeof(int) * counts.size()) != hipSuccess ||
+        hipMalloc(&dsumD, sizeof(double) * (size_t)b->nout) != hipSuccess ||
+        hipMemcpy(dcounts, counts.data(), sizeof(int) * counts.size(), hipMemcpyHostToDevice) != hipSuccess)
+        st = PS_ERR_NOMEM;
+    for (size_t wi = 0; wi < b->waves.size() && st == PS_OK; ++wi) {
+        const Wave &w = b->waves[wi];
+        const int M = w.M, nl = (int)w.legs.size(), ng = (int)w.groups.size();
+        void *tw = nullptr, *arena = nullptr, *dlegs = nullptr, *dgroups = nullptr;
+        std::vector<LegDev> legs(w.legs);
+        std::vector<GroupDev> groups(w.groups);
+        const int tl = g_tile_lines(M, 1);
+        const int64_t g2_per_kz = M / g2_tile_cols(M, 1, 1);
+        int64_t g1 = 0, g2 = 0, items = 0;
+        size_t bytes = 0;
+        std::vector<size_t> off(nl);
+        for (int i = 0; i < nl; ++i) {
+            LegDev &L = legs[i];
+            const int K = L.Kz;
+            L.g1_begin = g1;
+            L.g2_begin = g2;
+            g1 += (int64_t)(K + 1) * ((L.Bx + tl - 1) / tl);
+            g2 += 2 * (int64_t)(K + 1) * g2_per_kz;
+            off[i] = bytes;
+            bytes += cs * ((size_t)(K + 1) * L.Bx * M + (size_t)(K + 1) * M * M);
+        }
+        const int64_t tiles = (int64_t)M * (M / fused_tile_rows(M, 1));
+        const int nchunks = (int)((tiles + b->chunk_tiles - 1) / b->chunk_tiles);
+        for (GroupDev &G : groups) {
+            G.item_begin = items;
+            items += nchunks;
+        }
+        if (make_twiddles<double>(M, &tw) != PS_OK || hipMalloc(&arena, std::max<size_t>(bytes, 16)) != hipSuccess ||
+            hipMalloc(&dlegs, sizeof(LegDev) * std::max(nl, 1)) != hipSuccess ||
+            hipMalloc(&dgroups, sizeof(GroupDev) * std::max(ng, 1)) != hipSuccess) {
+            st = PS_ERR_NOMEM;
+        } else {
+            for (int i = 0; i < nl; ++i) {
+                LegDev &L = legs[i];
+                const size_t n1 = (size_t)(L.Kz + 1) * L.Bx * M;
+                char *p = (char *)arena + off[i];
+                L.spec = nullptr;
+                L.g1d = L.g2d = nullptr;
+                L.g1i = p;
+                L.g2i = p + cs * n1;
+                L.scale = nullptr;
+            }
+            if (hipMemcpy(dlegs, legs.data(), sizeof(LegDev) * nl, hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(dgroups, groups.data(), sizeof(GroupDev) * ng, hipMemcpyHostToDevice) != hipSuccess)
+                st = PS_ERR_HIP;
+            if (st == PS_OK)
+                st = run_inverse_g(1, M, (const LegDev *)dlegs, nl, 0, g1, 0, g2, b->grid_cap, 1, 0, tw, c->st);
+            if (st == PS_OK)
+                st = run_zc_fused(1, M, (const GroupDev *)dgroups, ng, (const LegDev *)dlegs, w.dsuffix, 0, items,
+                                  (double2 *)partials, max_chunks, b->grid_cap, w.kz_cap, 0, tw, c->st);
+            if (st > 0) st = PS_ERR_HIP;
+            if (hipStreamSynchronize(c->st) != hipSuccess && st == PS_OK) st = PS_ERR_HIP;
+        }
+        void *bufs[] = {tw, arena, dlegs, dgroups};
+        for (void *q : bufs)
+            if (q) (void)hipFree(q);
+    }
+    if (st == PS_OK) {
+        st = run_reduce_partials((const double2 *)partials, max_chunks, (const int *)dcounts, 0, b->nout,
+                                 (double *)dsumD, b->dsum + b->nout, c->st);
+        if (st > 0) st = PS_ERR_HIP;
+        if (hipStreamSynchronize(c->st) != hipSuccess && st == PS_OK) st = PS_ERR_HIP;
+    }
+    void *bufs[] = {partials, dcounts, dsumD};
+    for (void *q : bufs)
+        if (q) (void)hipFree(q);
+    return st;
+}
+
 // Direct method: the I sums (polygon counts, field independent) of every
-// group, from the legs' bare masks, once when the batch is planned.
+// group, from the legs' bare masks, once when the batch is planned (in FP64
+// for mixed-precision contexts too).
 static int batch_i_sums(ps_batch *b)
 {
     ps_ctx *c = b->c;
+    if (!c->fp64) return batch_i_sums_fp64(b);
     for (Wave &w : b->waves) {
         const int nl = (int)w.legs.size();
         PSCK(run_inverse_g(c->fp64, w.M, w.dlegs_i, nl, 0, w.g1_tiles, 0, w.g2_tiles, b->grid_cap, b->g2_div, 0,
