@@ -227,6 +227,20 @@ int ps_forward_profile(ps_ctx *ctx, int slot, int reps, double *ms, double *byte
  * counterpart in the reference. */
 int ps_batch_adjoint(ps_batch *b, int slot, const double *weights, void *grad, int grad_is_double, int on_device);
 
+/* Transform-free check of polygon values: the same sums taken directly over
+ * the closed lattice polygons m_1 + ... + m_n = 0 (mod N) with every m_i in
+ * its leg's shell (the masks ps_batch_create uses; DERIVED kmin/kmax are
+ * written back), from the resident spectra:
+ *   sum prod D = N^3 * sum over polygons of prod H(m_i),
+ *   sum prod I = N^3 * number of polygons.
+ * values, npolygons and sums as ps_batch_execute; the counts are exact.
+ * Cost: |shell 1| x |shell 2| cell pairs per pair of first shells
+ * (bispectra), |shell 1| cells (power spectra); temporary device buffers.
+ * Replaces: nothing in the reference; an independent check of
+ * polyspectrum()/num_polygons() (polyspectrum.c:51-252). */
+int ps_direct_check(ps_ctx *ctx, ps_polygon *polys, int npoly, double kbinwidth, int kbinning_case, double *values,
+                    double *npolygons, double *sums);
+
 /* Re-run the forward transform of a slot from the last host upload (timing
  * of the FFT passes alone). */
 int ps_forward_again(ps_ctx *ctx, int slot);

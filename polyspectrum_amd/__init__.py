@@ -142,6 +142,8 @@ def load_library():
     L.ps_batch_autotune.argtypes = [_vp, ctypes.POINTER(Polygon), ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                     ctypes.c_int, ctypes.POINTER(LaunchParams), _dp]
     L.ps_batch_params.argtypes = [_vp, ctypes.POINTER(LaunchParams)]
+    L.ps_direct_check.argtypes = [_vp, ctypes.POINTER(Polygon), ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                  _dp, _dp, _dp]
     L.ps_multi_create.argtypes = [ctypes.POINTER(_vp), ctypes.c_int, ctypes.POINTER(_vp)]
     L.ps_multi_execute.argtypes = [_vp]
     L.ps_multi_last_ms.restype = ctypes.c_double
@@ -294,6 +296,20 @@ class Context:
         _check(load_library().ps_batch_autotune(self._h, arr, len(polygons), float(kbinwidth), int(kbinning_case),
                                                 int(reps), ctypes.byref(lp), ctypes.byref(ms)))
         return _params_dict(lp), ms.value
+
+    def direct_check(self, polygons, kbinwidth=1.0, kbinning_case=0):
+        """The polygon values taken without transforms, over the closed
+        lattice polygons of the shells (an independent check of a batch).
+        Returns (values, npolygons, sums[n,2]) as Batch.execute."""
+        arr = _polygon_array(polygons)
+        n = len(polygons)
+        v = np.zeros(n)
+        npol = np.zeros(n)
+        s = np.zeros(2 * n)
+        _check(load_library().ps_direct_check(self._h, arr, n, float(kbinwidth), int(kbinning_case),
+                                              v.ctypes.data_as(_dp), npol.ctypes.data_as(_dp),
+                                              s.ctypes.data_as(_dp)))
+        return v, npol, s.reshape(n, 2)
 
     def batch_for_field(self, polygons, field, kbinwidth=1.0, kbinning_case=0, slot=0):
         """Load a field into the slot, then plan a batch on it."""

@@ -2016,6 +2016,141 @@ extern "C" int ps_forward_profile(ps_ctx *c, int slot, int reps, double *ms, dou
 }
 extern "C" double ps_batch_model_bytes(const ps_batch *b) { return b ? b->model_bytes : -1.0; }
 
+// ---------------------------------------------------------------- transform-free check
+// Cells of one shell, packed 10 bits per coordinate (offset N/2): the band
+// |m_i| <= K of the shell scanned with the estimator's membership test.  K
+// bounds every selected cell: each test implies a cube-corner norm or |m| of
+// at most r = shell_band's radius, and a cube-corner norm is at least
+// max_i |m_i| - bw/2.
+static void shell_cells(const MaskParams &mp, int N, std::vector<int> &out)
+{
+    out.clear();
+    const int h = N / 2;
+    const int K = std::min(h, shell_band(mp, N) + 1);
+    const int lo = -K, hi = std::min(K, h - 1);
+    for (int mx = lo; mx <= hi; ++mx)
+        for (int my = lo; my <= hi; ++my)
+            for (int mz = lo; mz <= hi; ++mz)
+                if (mask_eval(mp, mx, my, mz)) out.push_back((mx + h) | ((my + h) << 10) | ((mz + h) << 20));
+}
+
+// device buffers of one call, freed on every return path
+struct DevTemps {
+    std::vector<void *> p;
+    template <typename T> hipError_t alloc(T **q, size_t count)
+    {
+        const hipError_t e = hipMalloc((void **)q, sizeof(T) * std::max<size_t>(count, 1));
+        if (e == hipSuccess) p.push_back(*q);
+        return e;
+    }
+    void release()
+    {
+        for (void *q : p) (void)hipFree(q);
+        p.clear();
+    }
+    ~DevTemps() { release(); }
+};
+
+extern "C" int ps_direct_check(ps_ctx *c, ps_polygon *polys, int npoly, double bw, int kcase, double *values,
+                               double *npolygons, double *sums)
+{
+    if (!c || !polys || npoly <= 0 || !values) return PS_ERR_INVALID;
+    if (c->N > 1024) return PS_ERR_UNSUPPORTED_N;  // cells are packed 10 bits per coordinate
+    HIPCK(hipSetDevice(c->device));
+    // polygons sharing their enumerated shells (bispectra: legs 1 and 2,
+    // power spectra: leg 1) share one pass over the cells
+    struct Pairing {
+        int n = 0, slot = 0;
+        MaskParams m1{}, m2{};
+        std::vector<int> poly;          // member polygons
+        std::vector<MaskParams> last;   // their closing shells
+    };
+    std::vector<Pairing> pg;
+    for (int p = 0; p < npoly; ++p) {
+        ps_polygon &P = polys[p];
+        if (P.n != 2 && P.n != 3) return PS_ERR_INVALID;
+        if (P.slot < 0 || P.slot >= c->nslots) return PS_ERR_INVALID;
+        if (!c->loaded[P.slot]) return PS_ERR_STATE;
+        MaskParams mp[3];
+        for (int i = 0; i < P.n; ++i) PSCK(leg_mask(c, i, P.n, P.k, P.kmin, P.kmax, bw, kcase, &mp[i]));
+        size_t g = 0;
+        while (g < pg.size() && !(pg[g].n == P.n && pg[g].slot == P.slot && same_mask(pg[g].m1, mp[0]) &&
+                                  (P.n == 2 || same_mask(pg[g].m2, mp[1]))))
+            ++g;
+        if (g == pg.size()) {
+            Pairing q;
+            q.n = P.n;
+            q.slot = P.slot;
+            q.m1 = mp[0];
+            if (P.n == 3) q.m2 = mp[1];
+            pg.push_back(q);
+        }
+        pg[g].poly.push_back(p);
+        pg[g].last.push_back(mp[P.n - 1]);
+    }
+    const int N = c->N;
+    std::vector<double> S(npoly, 0.0), C(npoly, 0.0);
+    std::vector<int> cells1, cells2;
+    DevTemps tmp;
+    for (const Pairing &q : pg) {
+        shell_cells(q.m1, N, cells1);
+        if (q.n == 3) shell_cells(q.m2, N, cells2);
+        const int n1 = (int)cells1.size(), n2 = (q.n == 3) ? (int)cells2.size() : 0;
+        if (n1 == 0 || (q.n == 3 && n2 == 0)) continue;  // no polygon: both sums 0
+        const void *H = c->spec[q.slot];
+        const int nb = direct_sum_blocks(n1, n2);
+        int *dc1 = nullptr, *dc2 = nullptr;
+        double2 *dv1 = nullptr, *dv2 = nullptr, *dpart = nullptr;
+        HIPCK(tmp.alloc(&dc1, n1));
+        HIPCK(tmp.alloc(&dv1, n1));
+        HIPCK(tmp.alloc(&dpart, (size_t)nb * DIRECT_MEMBERS));
+        HIPCK(hipMemcpyAsync(dc1, cells1.data(), sizeof(int) * n1, hipMemcpyHostToDevice, c->st));
+        PSCK(run_direct_values(c->fp64, H, N, c->phase_only, dc1, n1, dv1, c->st));
+        if (n2 > 0) {
+            HIPCK(tmp.alloc(&dc2, n2));
+            HIPCK(tmp.alloc(&dv2, n2));
+            HIPCK(hipMemcpyAsync(dc2, cells2.data(), sizeof(int) * n2, hipMemcpyHostToDevice, c->st));
+            PSCK(run_direct_values(c->fp64, H, N, c->phase_only, dc2, n2, dv2, c->st));
+        }
+        std::vector<double2> hp((size_t)nb * DIRECT_MEMBERS);
+        for (size_t m0 = 0; m0 < q.last.size(); m0 += DIRECT_MEMBERS) {
+            DirectMembers mem{};
+            mem.nm = (int)std::min<size_t>(DIRECT_MEMBERS, q.last.size() - m0);
+            mem.bw = bw;
+            for (int j = 0; j < mem.nm; ++j) {
+                mem.mask[j] = q.last[m0 + j];
+                if (!mask_is_radial(mem.mask[j].kind)) mem.corners = 1;
+            }
+            PSCK(run_direct_sums(c->fp64, H, N, c->phase_only, dc1, dv1, n1, dc2, dv2, n2, mem, dpart, c->st));
+            HIPCK(hipMemcpyAsync(hp.data(), dpart, sizeof(double2) * hp.size(), hipMemcpyDeviceToHost, c->st));
+            HIPCK(hipStreamSynchronize(c->st));
+            for (int j = 0; j < mem.nm; ++j) {
+                long double s = 0.L, cnt = 0.L;  // fixed order over the workgroups
+                for (int k = 0; k < nb; ++k) {
+                    s += hp[(size_t)k * DIRECT_MEMBERS + j].x;
+                    cnt += hp[(size_t)k * DIRECT_MEMBERS + j].y;
+                }
+                S[q.poly[m0 + j]] = (double)s;
+                C[q.poly[m0 + j]] = (double)cnt;
+            }
+        }
+        tmp.release();
+    }
+    const double volume = c->L * c->L * c->L;
+    const double cells = (double)c->cells();
+    for (int p = 0; p < npoly; ++p) {
+        const int n = polys[p].n;
+        const double sD = cells * S[p], sI = cells * C[p];
+        values[p] = pow(volume, n - 1) * pow(c->N, -n * 3) * sD / sI;  // polyspectrum.c:155
+        if (npolygons) npolygons[p] = C[p];
+        if (sums) {
+            sums[2 * p] = sD;
+            sums[2 * p + 1] = sI;
+        }
+    }
+    return PS_OK;
+}
+
 // ---------------------------------------------------------------- drop-in symbols
 // A process-wide FP64 context per (nbins, boxsize), as the reference's
 // functions carry no handle.

@@ -91,6 +91,16 @@ struct SpecGroupDev {
     int64_t dot_begin;       // first dot work item
 };
 
+// Transform-free check: third-leg (power spectra: second-leg) shells tested
+// together against the cells that close the polygons of one pair of shells.
+constexpr int DIRECT_MEMBERS = 16;
+struct DirectMembers {
+    MaskParams mask[DIRECT_MEMBERS];
+    int nm;       // members in use
+    int corners;  // some member is a cube-corner (WATKINSON) mask
+    double bw;    // binwidth of every member
+};
+
 int supported_grid(int N);
 int tile_rows(int N, int fp64);
 int g2_tile_cols(int N, int fp64, int g2_div);  // columns of a G2 tile
@@ -115,6 +125,11 @@ int zc_dmode_supported(int N, int fp64);
 int run_reduce_partials(const double2 *partials, int64_t stride, const int *counts, int64_t uniform_count,
                         int64_t nout, double *outD, double *outI, hipStream_t st);
 int run_fill(double *p, int64_t n, double v, hipStream_t st);
+int run_direct_values(int fp64, const void *H, int N, int phase, const int *cells, int n, double2 *out, hipStream_t st);
+int direct_sum_blocks(int n1, int n2);  // partials (of DIRECT_MEMBERS each) run_direct_sums writes
+int run_direct_sums(int fp64, const void *H, int N, int phase, const int *c1, const double2 *v1, int n1,
+                    const int *c2, const double2 *v2, int n2, const DirectMembers &mem, double2 *partials,
+                    hipStream_t st);
 int run_leg_scale(int fp64, int N, const LegDev *legs, int nlegs, hipStream_t st);
 // adjoint helpers: inverse C2R of a half spectrum (destroyed) into a real
 // double field; H *= sum_j w_j mask_j; a += b; out = alpha a b

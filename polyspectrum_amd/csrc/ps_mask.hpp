@@ -32,7 +32,7 @@ __host__ __device__ __forceinline__ int lattice(int i, int n) { return (i < n / 
 
 // the 8 squared cube-corner norms of filter.c:257-289, started from
 // (mmin, mmax) = (1e30, 0) as the WATKINSON builders do (filter.c:201-203)
-__device__ __forceinline__ void corner_extremes(int mx, int my, int mz, double bw, double &lo2, double &hi2)
+__host__ __device__ __forceinline__ void corner_extremes(int mx, int my, int mz, double bw, double &lo2, double &hi2)
 {
 #pragma clang fp contract(off)
     const double x = (double)mx, y = (double)my, z = (double)mz;
@@ -59,7 +59,7 @@ __device__ __forceinline__ void corner_extremes(int mx, int my, int mz, double b
     hi2 = hi;
 }
 
-__device__ __forceinline__ bool mask_eval(const MaskParams &p, int mx, int my, int mz)
+__host__ __device__ __forceinline__ bool mask_eval(const MaskParams &p, int mx, int my, int mz)
 {
 #pragma clang fp contract(off)
     switch (p.kind) {
@@ -85,6 +85,25 @@ __device__ __forceinline__ bool mask_eval(const MaskParams &p, int mx, int my, i
         const double m = sqrt((double)(mx * mx + my * my + mz * mz));
         return fabs(p.kkf - m) <= 0.5 * p.bw;
     }
+    }
+}
+
+// mask_eval from the cell's norm m = sqrt(mx^2 + my^2 + mz^2) (int sum) and
+// its cube-corner extremes pmin, pmax (corner_extremes, then sqrt; read by
+// the WATKINSON kinds only), taken once for several masks of one binwidth
+__device__ __forceinline__ bool mask_eval_norms(const MaskParams &p, double m, double pmin, double pmax)
+{
+#pragma clang fp contract(off)
+    switch (p.kind) {
+    case MASK_RANGE: {
+        const double emin = p.mmin - m, emax = m - p.mmax;
+        return (emin <= 0.5 * p.bw && emax <= 0.5 * p.bw);
+    }
+    case MASK_WATKINSON: return (pmin <= p.kkf && p.kkf <= pmax);
+    case MASK_WATKINSON_KN:
+        return ((pmax >= p.mmax && pmin <= p.mmax) || (pmax <= p.mmax && pmin >= p.mmin) ||
+                (pmax >= p.mmin && pmin <= p.mmin));
+    default: return fabs(p.kkf - m) <= 0.5 * p.bw;
     }
 }
 

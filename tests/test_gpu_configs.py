@@ -131,8 +131,8 @@ def test_config_512_sweep_properties(gpu_lib):
         ctx.load_field((2.0 * field).astype(field.dtype))
         v2, _, _ = b.execute()
         b.close()
-    # the counts (FP64 transforms of the bare masks in both precisions) are
-    # the same integers (up to 3.5e7 here)
+    # the counts come from FP64 transforms of the bare masks in both
+    # precisions: the same values (integers up to 3.5e7 to FP64 rounding)
     np.testing.assert_array_equal(n32, n64)
     for g in (slice(0, 6), slice(6, 12)):
         scale = np.max(np.abs(v64[g]))
