@@ -21,6 +21,8 @@ script measures:
   * the reference-shaped FP64 path (ps_polyspectrum, one triangle per call)
     on a sample of triangles, and the batched FP64 path on all 128 (the
     accuracy reference for the mixed path);
+  * the transform-free sums over the closed lattice triangles
+    (ps_direct_check) of all 128, an independent reference for both;
   * per-stage HIP-event times of the batch (the roofline of the dominant
     kernel) and of the forward transform;
   * the CPU baseline: the oracle's FP64 restatement of the reference
@@ -326,6 +328,10 @@ def main():
     vals64, npol64, _ = b64.execute()
     b64_ms = b64.last_ms
     b64.close()
+    # transform-free sums over the closed lattice triangles (independent check)
+    t0 = time.perf_counter()
+    valsd, npold, _ = c64.direct_check(polys, 1.0, P.BINNING["GIVEN_BINNING"])
+    direct_s = time.perf_counter() - t0
     sample = [polys[i] for i in np.linspace(0, ntri - 1, args.plain_samples).astype(int)]
     plain = []
     t0 = time.perf_counter()
@@ -374,7 +380,13 @@ def main():
                      "fraction_le_1e-5": float(np.mean(rel <= 1e-5)),
                      "max_err_over_group_scale": scale_err(vals32, vals64),
                      "npolygons_rel_err_max": float(np.max(np.abs(npol32 - npol64) / np.abs(npol64))),
-                     "plain_fp64_vs_batched_fp64_max": plain_vs_b64},
+                     "plain_fp64_vs_batched_fp64_max": plain_vs_b64,
+                     "vs_direct_lattice_sums": {
+                         "fp64_batched_rel_err_max": float(np.max(np.abs(vals64 - valsd) / np.abs(valsd))),
+                         "mixed_rel_err_max": float(np.max(np.abs(vals32 - valsd) / np.abs(valsd))),
+                         "mixed_max_err_over_group_scale": scale_err(vals32, valsd),
+                         "npolygons_rounded_equal": bool(np.array_equal(np.round(npol32), npold)),
+                         "seconds": round(direct_s, 3)}},
         "fp64_plain": {"triangles_per_s": round(1.0 / plain_s, 3), "sampled_triangles": len(sample),
                        "speedup_mixed_batched": round(value * plain_s, 1)},
         "fp64_batched": {"triangles_per_s": round(ntri / (b64_ms / 1e3), 1)},
