@@ -795,11 +795,35 @@ extern "C" int ps_batch_create(ps_ctx *c, ps_polygon *polys, int npoly, double b
 // smallest supported transform grid m >= need, at most the spectrum's N
 // (216 is supported but not used as a reduced grid: its fused pass holds one
 // line per wavefront and is slower than 256 on the 512^3 sweep)
-static int alias_free_grid(int need, int N)
+// Euclidean bound of a shell: |m| <= r for every cell its test selects (the
+// cube-corner tests reach half a cell diagonal beyond their radius, since a
+// corner norm is at least |m| - sqrt(3)/2 bw)
+static double shell_radius(const MaskParams &p)
+{
+    const double e = 0.5 * fabs(p.bw), ec = 0.8660254037844387 * fabs(p.bw);
+    switch (p.kind) {
+    case MASK_RANGE: return p.mmax + e;
+    case MASK_WATKINSON: return p.kkf + ec;
+    case MASK_WATKINSON_KN: return std::max(p.mmin, p.mmax) + ec;
+    default: return p.kkf + e;
+    }
+}
+
+// Smallest supported grid M <= N on which the sums of a group do not alias.
+// A closed combination S = m_1 + ... + m_n of the legs' cells with S = 0
+// (mod M) but S != 0 has a component of size >= M, so |S| >= M; it cannot
+// occur when M exceeds the sum of the legs' per-axis bands (ksum) or of their
+// Euclidean radii (rsum, with a margin for the rounding of the tests).  A leg
+// whose band exceeds M/2 is cut to the grid: its cells of a closed polygon
+// have |m_i| <= (sum of the other radii) < M - r_i, hence |m_i| < M/2, and
+// lie inside the cut.
+static int alias_free_grid(int ksum, double rsum, int N)
 {
     static const int kGrids[] = {16, 24, 32, 48, 64, 96, 128, 144, 192, 256, 384, 512};
-    for (int m : kGrids)
-        if (m >= need && m <= N) return m;
+    for (int m : kGrids) {
+        if (m > N) break;
+        if (m > ksum || (double)m > rsum + 1e-6) return m;
+    }
     return N;
 }
 
@@ -893,7 +917,11 @@ static int plan_waves(ps_batch *b, std::vector<int> &counts)
         const int KB = (G.npre == 2) ? b->legs[G.preB].Kz : (G.pre_sq ? KA : 0);
         if (!fourier) {
             // the product of all legs must not alias onto m = 0
-            gM[gi] = alias_free_grid((G.npre == 0 ? 2 * Kt : KA + KB + Kt) + 1, N);
+            double rt = 0.0;
+            for (int o = 0; o < G.nsuf; ++o) rt = std::max(rt, shell_radius(b->legs[b->suffix[G.suf_begin + o]].mask));
+            const double rA = (G.npre >= 1) ? shell_radius(b->legs[G.preA].mask) : 0.0;
+            const double rB = (G.npre == 2) ? shell_radius(b->legs[G.preB].mask) : (G.pre_sq ? rA : 0.0);
+            gM[gi] = (G.npre == 0) ? alias_free_grid(2 * Kt, 2.0 * rt, N) : alias_free_grid(KA + KB + Kt, rA + rB + rt, N);
             continue;
         }
         SpecGroupDev S{};
@@ -908,7 +936,7 @@ static int plan_waves(ps_batch *b, std::vector<int> &counts)
             need = 2 * Kq + 1;
         }
         Kq = std::min(Kq, N / 2);
-        const int M = alias_free_grid(need, N);
+        const int M = alias_free_grid(need - 1, need - 1.0, N);
         gM[gi] = M;
         S.Kq = Kq;
         S.Kqz = std::min(Kq, M / 2);
@@ -1392,11 +1420,16 @@ extern "C" int ps_batch_create_ex(ps_ctx *c, ps_polygon *polys, int npoly, doubl
             const int a = std::get<1>(kv.first), a2 = std::get<2>(kv.first);
             const int KA = (a >= 0) ? b->legs[a].Kz : 0;
             const int KB = (a2 >= 0) ? b->legs[a2].Kz : 0;
+            const double rA = (a >= 0) ? shell_radius(b->legs[a].mask) : 0.0;
+            const double rB = (a2 >= 0) ? shell_radius(b->legs[a2].mask) : 0.0;
             std::vector<int> grid;
             std::map<int, int> grid_of;
             for (int p : kv.second) {
-                const int Kt = b->legs[kind == 1 ? pleg[p][0] : pleg[p][polys[p].n - 1]].Kz;
-                grid.push_back(alias_free_grid((kind == 1 ? 2 * Kt : KA + KB + Kt) + 1, N));
+                const LegDev &Lt = b->legs[kind == 1 ? pleg[p][0] : pleg[p][polys[p].n - 1]];
+                const int Kt = Lt.Kz;
+                const double rt = shell_radius(Lt.mask);
+                grid.push_back(kind == 1 ? alias_free_grid(2 * Kt, 2.0 * rt, N)
+                                         : alias_free_grid(KA + KB + Kt, rA + rB + rt, N));
                 grid_of[p] = grid.back();
             }
             const int npre = (kind == 3) ? (a == a2 ? 1 : 2) : (kind == 2 ? 1 : 0);

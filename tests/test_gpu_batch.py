@@ -261,6 +261,36 @@ def test_reduced_grid_3x3(gpu_lib, N, L, m, grid):
     assert np.all(rel < 1e-5), rel
 
 
+def test_reduced_grid_by_euclidean_radii(gpu_lib):
+    """k1 = k2 = 10.2 k_F, k3 = 9.9 k_F: the per-axis bands (11 + 11 + 11 =
+    33) would need M = 48, the shells' radii (10.7 + 10.7 + 10.4 = 31.8) allow
+    M = 32, where no nonzero closed combination of their cells can alias onto
+    m = 0.  FP64 equals the oracle, mixed equals FP64 to the mixed bar, and the
+    counts are the oracle's."""
+    P = gpu_lib
+    N, L = 64, 100.0
+    f = lognormal_field(N, 41)
+    kf = 2 * math.pi / L
+    polys = [{"n": 3, "k": [10.2 * kf, 10.2 * kf, 9.9 * kf], "kmin": [0.0] * 3, "kmax": [0.0] * 3},
+             {"n": 3, "k": [10.2 * kf, 10.2 * kf, 6.0 * kf], "kmin": [0.0] * 3, "kmax": [0.0] * 3}]
+    res = {}
+    for prec in (P.PREC_FP64, P.PREC_MIXED):
+        with P.Context(N, L, prec) as ctx:
+            ctx.load_field(f)
+            b = ctx.batch(polys, 1.0, 0, params={"grid_cap": 1024, "chunk_tiles": 1, "method": P.METHOD_DIRECT})
+            grids = [w[0] for w in b.waves()]
+            res[prec] = b.execute()
+            b.close()
+        assert grids == [32], grids
+    spec = O.forward_spectrum(f)
+    for i, p in enumerate(polys):
+        ref, _, _, rs = O.polyspectrum(spec, 3, p["k"], p["kmin"], p["kmax"], 1.0, 0, L)
+        assert abs(res[P.PREC_FP64][0][i] - ref) <= 1e-10 * abs(ref), (i, res[P.PREC_FP64][0][i], ref)
+        assert round(res[P.PREC_FP64][1][i]) == round(rs[1].real / N ** 3) > 0
+    rel = np.abs(res[P.PREC_MIXED][0] - res[P.PREC_FP64][0]) / np.abs(res[P.PREC_FP64][0])
+    assert np.all(rel < 1e-5), rel
+
+
 @pytest.mark.parametrize("method", ["METHOD_DIRECT", "METHOD_FOURIER"])
 def test_open_polygons_on_reduced_grids(gpu_lib, method):
     """Polygons that cannot close (k3 > k1 + k2; a cross spectrum of two
