@@ -248,7 +248,9 @@ static MaskParams mp_range(int kind, double kmin, double kmax, double bw, double
     return p;
 }
 
-// largest |m_i| a cell selected by p can have (+1 margin), capped at N/2
+// largest |m_i| a cell selected by p can have, capped at N/2: every test
+// implies |m_i| <= r (a cube-corner norm is at least max_i |m_i| - bw/2); the
+// 1e-9 relative margin covers the rounding of the tests' FP64 expressions
 static int shell_band(const MaskParams &p, int N)
 {
     double r;
@@ -260,7 +262,7 @@ static int shell_band(const MaskParams &p, int N)
     r += 0.5 * fabs(p.bw);
     if (!(r < (double)N)) return N / 2;
     if (r < 0.0) return 0;
-    int K = (int)floor(r) + 1;
+    const int K = (int)floor(r + 1e-9 * std::max(1.0, r));
     return std::min(K, N / 2);
 }
 
