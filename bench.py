@@ -12,7 +12,7 @@ mixed-precision path (FP32 FFTs, FP64 compensated sums) with the reference's
 method: every leg's filtered field inverse transformed, the pointwise product
 and its sum.  The triangles of each k1 = k2 pair run in classes on the
 smallest grids M on which their products do not alias (M > K1 + K2 + K3 in
-cells; M = 16 .. 256 here, the sums scaled by (512/M)^3, exact).  The step replays the batch's captured
+cells; M = 16 .. 192 here, the sums scaled by (512/M)^3, exact).  The step replays the batch's captured
 hipGraph with the field's spectrum already resident in HBM.  Beside it the
 script measures:
   * the opt-in Fourier-space method (same sums in another order: the product
