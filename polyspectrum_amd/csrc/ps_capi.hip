@@ -1409,6 +1409,10 @@ extern "C" int ps_batch_create_ex(ps_ctx *c, ps_polygon *polys, int npoly, doubl
         L.Kx = K;
         L.Bx = std::min(2 * K + 1, N);
         L.ns = N;
+        // every selected cell has |m| <= r, so a (kz, kx) line with
+        // mz^2 + mx^2 > r^2 holds none (the first passes skip it)
+        const double r = shell_radius(kv.second);
+        if (r >= 0.0 && r < (double)N) L.r2 = (int)floor(r * r * (1.0 + 1e-9) + 1e-9);
         b->legs.push_back(L);
     }
     // 3. group descriptors

@@ -33,6 +33,7 @@ struct LegDev {
     double *scale;     // device: rms |spectrum| over the shell (D part enters G1 divided by it)
     int ivlo = -1;     // radial mask: the cells with ivlo <= m^2 <= ivhi ((1, 0): none);
     int ivhi = -1;     // ivlo < 0: test every cell with mask_eval
+    int r2 = -1;       // the shell's cells have mx^2 + mz^2 <= r2: (kz, kx) lines beyond are empty (-1: no bound)
 };
 
 // A batch of polygons sharing a prefix product: output s of the group is

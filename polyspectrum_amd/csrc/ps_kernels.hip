@@ -317,6 +317,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(band_pa
         const cplx<T> *g;
         cplx<T> *o;
         int kz, y0, Kx, Bx;
+        int kx2;  // band lines with mx^2 > kx2 hold no cell of the shell (read as zero)
     };
     // Tiles narrower than a 128-byte line share each line with the next y
     // tile.  Workgroups are dispatched to the 8 XCDs (8 separate L2 caches)
@@ -339,12 +340,15 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(band_pa
         t.y0 = (int)(rest % nyt) * TLC;
         t.Kx = L.Kx;
         t.Bx = L.Bx;
+        const int mz = lattice(t.kz, N);
+        t.kx2 = (L.r2 < 0) ? N * N : L.r2 - mz * mz;
         return t;
     };
     auto band = [&](const Tile &t, int idx) -> cplx<T> {
         const int l = idx % TLC, kx = idx / TLC;
         const int j = k_to_band(kx, t.Kx, t.Bx, N);
-        return (j >= 0 && t.g) ? t.g[((int64_t)t.kz * t.Bx + j) * N + t.y0 + l] : mk<T>(0, 0);
+        const int mx = lattice(kx, N);
+        return (j >= 0 && t.g && mx * mx <= t.kx2) ? t.g[((int64_t)t.kz * t.Bx + j) * N + t.y0 + l] : mk<T>(0, 0);
     };
     auto put = [&](const Tile &t) {
         if (t.o) {
