@@ -208,13 +208,27 @@ k_inv_g1d(const LegDev *__restrict__ legs, int nlegs, int64_t t0, int64_t t1, co
     stage_twiddles<T, N>(stw, tw, tid);
     struct Tile {
         int li, kz, j0;
+        int empty;  // no line of the tile meets the shell's disk: nothing to transform or store
     };
     auto decode = [&](int64_t tile) -> Tile {
         const int li = find_owner(legs, nlegs, tile, &LegDev::g1_begin);
         const LegDev &L = legs[li];
         const int nbt = (L.Bx + LPD - 1) / LPD;
         const int64_t local = tile - L.g1_begin;
-        return Tile{li, (int)(local / nbt), (int)(local % nbt) * LPD};
+        Tile t{li, (int)(local / nbt), (int)(local % nbt) * LPD, 0};
+        if (L.r2 >= 0) {
+            // |mx| rises with the band line j up to the split s, then falls
+            // as P - j (the band's negative half): its least value over the
+            // tile's lines [j0, j1] is at one of the two ends
+            const int j1 = min(t.j0 + LPD, L.Bx) - 1;
+            const int split = (L.Bx == N) ? N / 2 : L.Kx + 1;
+            int mn = N;
+            if (t.j0 < split) mn = t.j0;
+            if (j1 >= split) mn = min(mn, L.Bx - j1);
+            const int mz = lattice(t.kz, N);
+            t.empty = mn * mn > L.r2 - mz * mz;
+        }
+        return t;
     };
     // masked spectrum cell idx = (line l, ky) of a tile, as in k_inv_g1
     auto gather = [&](const Tile &t, int idx) -> cplx<T> {
@@ -278,8 +292,10 @@ k_inv_g1d(const LegDev *__restrict__ legs, int nlegs, int64_t t0, int64_t t1, co
                     r[m] = (TOT % NT == 0 || idx < TOT) ? gather(nx, idx) : mk<T>(0, 0);
                 }
             }
-            fft_lds<T, N, LPD, +1>(lds, stw, tid);
-            put(cur);
+            if (!cur.empty) {  // uniform over the workgroup
+                fft_lds<T, N, LPD, +1>(lds, stw, tid);
+                put(cur);
+            }
             __syncthreads();
             if (nt >= t1) break;
             tile = nt;
@@ -288,6 +304,7 @@ k_inv_g1d(const LegDev *__restrict__ legs, int nlegs, int64_t t0, int64_t t1, co
     } else {
         for (int64_t tile = t0 + blockIdx.x; tile < t1; tile += gridDim.x) {
             const Tile cur = decode(tile);
+            if (cur.empty) continue;  // uniform over the workgroup
             tile_loop_u<TOT, 4>(tid, [&](int idx) { lds[lidx<LPD>(idx % N, idx / N)] = gather(cur, idx); });
             __syncthreads();
             fft_lds<T, N, LPD, +1>(lds, stw, tid);
