@@ -1076,9 +1076,19 @@ static int plan_waves(ps_batch *b, std::vector<int> &counts)
                 w.g1_tiles_d += (int64_t)(K + 1) * ((L.Bx + ld - 1) / ld);
                 w.g2_tiles_d += (int64_t)(K + 1) * g2_per_kz;
             }
-            // G1 reads the spectrum cells of the band box and writes the band
-            // lines (D, I or D only); G2 reads them and writes the planes
-            const double n1 = (double)(K + 1) * L.Bx * M, n2 = (double)(K + 1) * M * M;
+            // G1 reads the spectrum cells of the band box (an upper bound: the
+            // gather reads the shell's cells) and writes the band lines (D, I
+            // or D only) that meet the shell's disk mx^2 + mz^2 <= r^2; G2
+            // reads those and writes the planes
+            double disk = 0.0;
+            for (int kz = 0; kz <= K; ++kz) {
+                const int mz = lattice(kz, M), lim = (L.r2 < 0) ? M * M : L.r2 - mz * mz;
+                for (int j = 0; j < L.Bx; ++j) {
+                    const int mx = lattice((L.Bx == M || j <= L.Kx) ? j : j + M - L.Bx, M);
+                    if (mx * mx <= lim) disk += 1.0;
+                }
+            }
+            const double n1 = disk * M, n2 = (double)(K + 1) * M * M;
             b->stage_bytes[0] += (double)cs * ((double)(K + 1) * L.Bx * L.Bx + parts * n1);
             b->stage_bytes[1] += (double)cs * (parts * n1 + parts * n2);
         }
