@@ -526,8 +526,12 @@ __global__ void __launch_bounds__(NT) k_inv_zc_plain(const cplx<T> *__restrict__
 // do not depend on the field, so their sums are taken once when a batch is
 // planned; every execution then transforms the D parts only, two adjacent y
 // rows per complex line (D(a) + i D(b)), so half as many lines per tile.
+// y rows per wavefront of the fused pass: wave_lines, except FP32 at 144
+// (6 rows of 144 = 3 paired lines held 212 VGPRs, two waves per SIMD)
+template <typename T, int N> constexpr int zc_wave_rows() { return (sizeof(T) == 4 && N == 144) ? 4 : wave_lines<T, N>(); }
+
 template <typename T, int N, bool DM = false> struct ZcCfg {
-    static constexpr int LW0 = wave_lines<T, N>();  // y rows per wavefront
+    static constexpr int LW0 = zc_wave_rows<T, N>();  // y rows per wavefront
     static constexpr int LW = DM ? LW0 / 2 : LW0;   // transformed lines per wavefront
     // 4 wavefronts: measured faster on the 512^3 sweep than 8 (whose 16-row
     // tiles read whole 128 B lines per plane but synchronise twice as many
@@ -542,7 +546,7 @@ template <typename T, int N, bool DM = false> struct ZcCfg {
     // staged without overlap)
     static constexpr int MPT = ((NH * SL + BT - 1) / BT < 4) ? (NH * SL + BT - 1) / BT : 4;
 };
-template <typename T, int N> constexpr bool zc_dmode_ok() { return wave_lines<T, N>() % 2 == 0; }
+template <typename T, int N> constexpr bool zc_dmode_ok() { return zc_wave_rows<T, N>() % 2 == 0; }
 // Compact form of the fused pass: twiddles read from an LDS table and one
 // staging buffer (the loads of the next step still overlap the transforms, in
 // registers), which leaves registers and LDS for four workgroups per CU where
