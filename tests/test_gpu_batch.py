@@ -41,18 +41,22 @@ def _oracle_values(spec, polys, bw, case, L):
     return np.array(vals), np.array(counts)
 
 
-@pytest.mark.parametrize("N,L,case", [(32, 100.0, 0), (32, 100.0, 2), (48, 120.0, 1), (64, 100.0, 3)])
-def test_batch_fp64_matches_oracle(gpu_lib, N, L, case):
+# wider bins (bw = 1.5, 2) widen the shells and their radius bounds (band,
+# disk and alias-free grid), including the cube-corner WATKINSON tests
+@pytest.mark.parametrize("N,L,case,bw", [(32, 100.0, 0, 1.0), (32, 100.0, 2, 1.0), (48, 120.0, 1, 1.0),
+                                         (64, 100.0, 3, 1.0), (64, 100.0, 3, 2.0), (48, 120.0, 2, 1.5),
+                                         (32, 100.0, 1, 2.0)])
+def test_batch_fp64_matches_oracle(gpu_lib, N, L, case, bw):
     P = gpu_lib
     f = lognormal_field(N, 2000 + N)
     spec = O.forward_spectrum(f)
     kf = 2 * math.pi / L
     thetas = [math.pi * i / 6 for i in range(6)]
     polys = _polys_triangles(kf, [(4.0, 4.0), (3.0, 5.0)], thetas, case)
-    ref, cnt = _oracle_values(spec, polys, 1.0, case, L)
+    ref, cnt = _oracle_values(spec, polys, bw, case, L)
     with P.Context(N, L, P.PREC_FP64) as ctx:
         ctx.load_field(f)
-        b = ctx.batch(polys, 1.0, case)
+        b = ctx.batch(polys, bw, case)
         v, npg, sums = b.execute()
         v2, _, _ = b.execute()  # graph replay
         b.close()
