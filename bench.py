@@ -376,6 +376,9 @@ def main():
         "forward_fft": {"ms": round(fwd_ms, 4), "GB": round(fwd_bytes / 1e9, 4),
                         "GB/s": round(fwd_bytes / fwd_ms / 1e6, 1),
                         "frac": round(fwd_bytes / fwd_ms / 1e6 / HBM_PEAK_GBS, 4)},
+        # a step that also transforms the field (a new field per step)
+        "with_forward_fft": {"ms_per_step": round(ms_per_step + fwd_ms, 3),
+                             "triangles_per_s": round(ntri / ((ms_per_step + fwd_ms) / 1e3), 1)},
         "accuracy": {"mixed_vs_fp64_rel_err_max": float(rel.max()), "median": float(np.median(rel)),
                      "fraction_le_1e-5": float(np.mean(rel <= 1e-5)),
                      "max_err_over_group_scale": scale_err(vals32, vals64),
