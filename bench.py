@@ -289,7 +289,7 @@ def main():
     tuned, tuned_ms = None, None
     if not args.no_autotune:
         print("[bench] autotuning the launch shape", file=sys.stderr, flush=True)
-        tuned, tuned_ms = ctx.autotune(polys, 1.0, P.BINNING["GIVEN_BINNING"], reps=2)
+        tuned, tuned_ms = ctx.autotune(polys, 1.0, P.BINNING["GIVEN_BINNING"], reps=5)
     batch = ctx.batch(polys, 1.0, P.BINNING["GIVEN_BINNING"])
     for _ in range(args.warmup):
         batch.execute()

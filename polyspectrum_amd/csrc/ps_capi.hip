@@ -1855,20 +1855,21 @@ extern "C" int ps_batch_autotune(ps_ctx *c, ps_polygon *polys, int npoly, double
     const int divs[] = {1, 2};
     std::vector<ps_polygon> work(polys, polys + npoly);
     std::vector<double> v(npoly);
-    // mean replay time of one candidate (after a warm-up replay)
+    // fastest of reps replays of one candidate (after a warm-up replay): the
+    // least time is the least disturbed by other work on the device
     auto trial = [&](const ps_launch_params &lp, double *ms) -> int {
         ps_batch *b = nullptr;
         std::copy(polys, polys + npoly, work.begin());
         int s = ps_batch_create_ex(c, work.data(), npoly, bw, kcase, &lp, &b);
         if (s != PS_OK) return s;
-        double t = 0.0;
+        double t = 1e300;
         s = ps_batch_execute(b, v.data(), nullptr, nullptr);
         for (int r = 0; r < reps && s == PS_OK; ++r) {
             s = ps_batch_execute(b, v.data(), nullptr, nullptr);
-            t += b->last_ms;
+            t = std::min(t, b->last_ms);
         }
         ps_batch_destroy(b);
-        *ms = t / reps;
+        *ms = t;
         return s;
     };
     ps_launch_params bp = default_params(c);
