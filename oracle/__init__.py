@@ -51,6 +51,9 @@ def lib():
                                       ctypes.c_double, ctypes.c_int, ctypes.c_double]
         L.or_direct_polygon_sum.restype = ctypes.c_int64
         L.or_direct_polygon_sum.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _u8p, _u8p, _u8p, _dp]
+        L.or_polygon_gradient.restype = ctypes.c_double
+        L.or_polygon_gradient.argtypes = [ctypes.c_int, _dp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_double,
+                                          ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_double, _dp]
         L.or_product.argtypes = [ctypes.c_int64, _dp, _dp, _dp]
         L.or_sum.argtypes = [ctypes.c_int64, _dp, _dp]
         _lib = L
@@ -145,6 +148,26 @@ def num_polygons(N, n, k, kmin, kmax, kbinwidth, kbinning_case, boxsize):
     lo = np.asarray(kmin, dtype=np.float64).copy()
     hi = np.asarray(kmax, dtype=np.float64).copy()
     return lib().or_num_polygons(N, n, _d(k), _d(lo), _d(hi), kbinwidth, kbinning_case, boxsize)
+
+
+def polygon_gradient(field, polys, kbinwidth, kbinning_case, boxsize, weights, phase=False):
+    """Values of the polygons (dicts n, k, kmin, kmax) on a real N^3 field and
+    sum_p weights[p] * d value_p / d field (N^3 float64), one polygon at a time."""
+    f = np.ascontiguousarray(field, dtype=np.float64)
+    N = f.shape[0]
+    grad = np.zeros((N, N, N))
+    vals = []
+    for p, w in zip(polys, weights):
+        k = np.zeros(3)
+        lo = np.zeros(3)
+        hi = np.zeros(3)
+        n = int(p["n"])
+        k[:n] = p["k"][:n]
+        lo[:len(p.get("kmin", []))] = p.get("kmin", [])[:3]
+        hi[:len(p.get("kmax", []))] = p.get("kmax", [])[:3]
+        vals.append(lib().or_polygon_gradient(N, _d(f), n, _d(k), _d(lo), _d(hi), kbinwidth, kbinning_case,
+                                              boxsize, int(bool(phase)), float(w), _d(grad)))
+    return np.array(vals), grad
 
 
 def direct_polygon_sum(spec, masks):

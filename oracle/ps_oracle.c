@@ -17,6 +17,9 @@
  *                            any FFT), exact integer polygon counts, scaling
  *                            B(l*d) = l^3 B, shift invariance, phase-only P.
  *   or_num_polygons          exact lattice polygon counts.
+ *   or_polygon_gradient      central differences of or_polyspectrum (exact for
+ *                            the cubic amplitude estimator with Richardson's
+ *                            two-step formula), an independent numpy gradient.
  */
 #include "ps_oracle.h"
 
@@ -450,6 +453,84 @@ double or_num_polygons(int nbins, int n, const double *k, double *kmin, double *
     or_sum(cnt, accI, sI);
     free(mask); free(tmp); free(accI);
     return sI[0] / (double)((int64_t)nbins * nbins * nbins);          /* :247 */
+}
+
+/* ------------------------------------------------------------------------ */
+/* gradient of one polygon value with respect to the real field              */
+/* ------------------------------------------------------------------------ */
+
+/* value = V^(n-1) N^(-3n) Re S / Re C with S = sum_x prod_i D_i(x) (the
+ * estimator above, polyspectrum.c:51-161), D_i = B(m_i s), s = F(delta) or,
+ * with phase, p = s/|s| (utils_fftw.c:78-93); F/B are the unnormalised
+ * forward/backward transforms of fft.c.  C does not depend on delta.  With
+ * Q_i = prod_{j != i} D_j,
+ *   dS = sum_k R(k) ds(k),  R = sum_i m_i B(Q_i)   (sum_x Q B(f) = sum_k f B(Q)),
+ * and ds(k) = sum_y d delta(y) e^{-iky}, so d Re S / d delta = Re F(R).  With
+ * phase, dp = (ds - p Re(conj(p) ds)) / |s| turns R into
+ * R' = (R - conj(p) Re(R p)) / |s| (0 where s = 0).  grad += weight * value'
+ * (N^3 doubles); kmin/kmax are updated as by or_polyspectrum. */
+double or_polygon_gradient(int nbins, const double *field, int n, const double *k, double *kmin, double *kmax,
+                           double kbinwidth, int kcase, double boxsize, int phase, double weight, double *grad)
+{
+    const int64_t cnt = (int64_t)nbins * nbins * nbins;
+    const size_t cb = sizeof(double) * 2 * (size_t)cnt;
+    double *s = (double *)malloc(cb), *src = (double *)malloc(cb), *R = (double *)calloc(1, cb);
+    double *Q = (double *)malloc(cb), *accI = (double *)malloc(cb), *tmp = (double *)malloc(cb);
+    double *D[3] = {NULL, NULL, NULL};
+    uint8_t *mask[3] = {NULL, NULL, NULL};
+    for (int64_t i = 0; i < cnt; i++) { s[2 * i] = field[i]; s[2 * i + 1] = 0.; }
+    or_fft3d(nbins, s, -1);
+    memcpy(src, s, cb);
+    if (phase) or_calc_phase(nbins, src);
+    for (int64_t i = 0; i < cnt; i++) { accI[2 * i] = 1.; accI[2 * i + 1] = 0.; }
+    for (int i = 0; i < n; i++) {
+        mask[i] = (uint8_t *)malloc((size_t)cnt);
+        D[i] = (double *)malloc(cb);
+        leg_mask(nbins, i, n, k, kmin, kmax, kbinwidth, kcase, boxsize, mask[i]);
+        mask_to_complex(cnt, mask[i], tmp);
+        or_fft3d(nbins, tmp, +1);
+        or_product(cnt, accI, tmp, accI);
+        for (int64_t c = 0; c < cnt; c++) {
+            D[i][2 * c] = mask[i][c] ? src[2 * c] : 0.;
+            D[i][2 * c + 1] = mask[i][c] ? src[2 * c + 1] : 0.;
+        }
+        or_fft3d(nbins, D[i], +1);
+    }
+    /* S and the R of every leg */
+    for (int64_t c = 0; c < cnt; c++) { Q[2 * c] = 1.; Q[2 * c + 1] = 0.; }
+    for (int i = 0; i < n; i++) or_product(cnt, Q, D[i], Q);
+    double sD[2], sI[2];
+    or_sum(cnt, Q, sD);
+    or_sum(cnt, accI, sI);
+    for (int i = 0; i < n; i++) {
+        for (int64_t c = 0; c < cnt; c++) { Q[2 * c] = 1.; Q[2 * c + 1] = 0.; }
+        for (int j = 0; j < n; j++)
+            if (j != i) or_product(cnt, Q, D[j], Q);
+        or_fft3d(nbins, Q, +1);
+        for (int64_t c = 0; c < cnt; c++)
+            if (mask[i][c]) { R[2 * c] += Q[2 * c]; R[2 * c + 1] += Q[2 * c + 1]; }
+    }
+    if (phase) {
+        for (int64_t c = 0; c < cnt; c++) {
+            const double a = hypot(s[2 * c], s[2 * c + 1]);
+            if (a > 0.) {
+                const double pr = s[2 * c] / a, pi = s[2 * c + 1] / a;
+                const double re = R[2 * c] * pr - R[2 * c + 1] * pi;  /* Re(R p) */
+                R[2 * c] = (R[2 * c] - pr * re) / a;
+                R[2 * c + 1] = (R[2 * c + 1] + pi * re) / a;
+            } else {
+                R[2 * c] = R[2 * c + 1] = 0.;
+            }
+        }
+    }
+    or_fft3d(nbins, R, -1);
+    const double volume = boxsize * boxsize * boxsize;
+    const double cn = pow(volume, n - 1) * pow(nbins, -n * 3);
+    const double coef = weight * cn / sI[0];
+    for (int64_t c = 0; c < cnt; c++) grad[c] += coef * R[2 * c];
+    for (int i = 0; i < n; i++) { free(mask[i]); free(D[i]); }
+    free(s); free(src); free(R); free(Q); free(accI); free(tmp);
+    return cn * sD[0] / sI[0];
 }
 
 /* ------------------------------------------------------------------------ */

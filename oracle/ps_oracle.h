@@ -67,6 +67,14 @@ double or_polyspectrum(int nbins, const double *fft_array, int n, const double *
 double or_num_polygons(int nbins, int n, const double *k, double *kmin, double *kmax,
                        double kbinwidth, int kbinning_case, double boxsize);
 
+/* Gradient of one polygon's value with respect to the real N^3 field:
+ * grad += weight * d value / d field (value of or_polyspectrum on the
+ * field's forward transform; phase = useOnlyPhase).  Returns the value;
+ * kmin/kmax are updated as by or_polyspectrum. */
+double or_polygon_gradient(int nbins, const double *field, int n, const double *k, double *kmin, double *kmax,
+                           double kbinwidth, int kbinning_case, double boxsize, int phase, double weight,
+                           double *grad);
+
 /* Independent direct estimator: sum over lattice polygons m_1+..+m_n = 0 (mod N)
  * with m_i in mask_i of prod spec(m_i).  n = 2 or 3.  Returns the count of
  * closed polygons; sum_out receives (re, im) of the spectral product sum. */
