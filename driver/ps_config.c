@@ -63,6 +63,8 @@ ps_config *ps_read_config(const char *path)
     }
     int32_t b;
     if (ps_ini_get_int32(ini, "batched", "GPU", &b)) c->batched = (b != 0);
+    c->quirks = 1;
+    if (ps_ini_get_int32(ini, "referenceQuirks", "GPU", &b)) c->quirks = (b != 0);
     ps_ini_close(ini);
     return c;
 }

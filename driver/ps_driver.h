@@ -35,9 +35,13 @@ typedef struct {
     char *output_basename;
     int write_numpolygons;
     /* optional [GPU] section (not in the reference): precision FP64|MIXED,
-     * batched 0|1 (1: every configuration in one captured launch sequence) */
+     * batched 0|1 (1: every configuration in one captured launch sequence),
+     * referenceQuirks 1|0 (1, the default: the reference's behaviour where it
+     * is a defect, e.g. n = 2 GIVEN/WATKINSON binning keeps the last leg's
+     * kmin/kmax at k[1], kvectors.c:92-97 with run.c:56-83; 0: corrected) */
     int precision;
     int batched;
+    int quirks;
 } ps_config;
 
 /* readConfObj (confObj.c:131-150): exits with the reference's FATAL messages */

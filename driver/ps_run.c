@@ -270,6 +270,12 @@ int ps_run(const ps_config *c)
     for (int i = 0; i < num; i++) {
         kv->kpolygon[n - 1] = kv->k[i];
         if (n == 2) kv->kpolygon[0] = kv->k[i];
+        if (n == 2 && !c->quirks) {
+            /* corrected: the last leg's GIVEN/WATKINSON range follows k[i]; the
+             * reference keeps kpolygonMin/Max[1] = k[1] from kvectors.c:92-97 */
+            kv->kpolygonMin[n - 1] = kv->k[i];
+            kv->kpolygonMax[n - 1] = kv->k[i];
+        }
         if (n == 3) {
             if (c->equilateral == 1) {
                 for (int j = 0; j < n; j++) kv->kpolygon[j] = kv->k[i];

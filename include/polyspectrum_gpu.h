@@ -130,6 +130,16 @@ int ps_polyspectrum(ps_ctx *ctx, int slot, int n, const double *k, double *kmin,
 int ps_num_polygons(ps_ctx *ctx, int n, const double *k, double *kmin, double *kmax, double kbinwidth,
                     int kbinning_case, double *result);
 
+/* Condition number of each polygon's sum, kappa[p] = sum_x |prod_i D_i(x)| /
+ * |sum_x prod_i D_i(x)|: the factor by which rounding errors of the filtered
+ * fields are amplified in the polyspectrum value (large where B crosses
+ * zero).  Evaluated through the reference-shaped path in the context's
+ * precision (one configuration at a time, full grid); kmin/kmax of polys are
+ * not changed.  No counterpart in the reference (polyspectrum.c:139-140 takes
+ * the sums only). */
+int ps_condition_numbers(ps_ctx *ctx, const ps_polygon *polys, int npoly, double kbinwidth, int kbinning_case,
+                         double *kappa);
+
 /* Batched evaluation of many configurations: shells shared by several
  * polygons are filtered and transformed once, only the k-cells a shell can
  * touch enter the first inverse passes, and the last inverse pass computes
@@ -210,8 +220,12 @@ const char *ps_batch_stage_name(const ps_batch *b, int stage);
 int ps_batch_wave_count(const ps_batch *b);
 int ps_batch_wave_info(const ps_batch *b, int w, int *grid, int *nlegs, int *ngroups, double *bytes);
 
-/* Mean time (ms) of reps forward transforms of a slot's last upload and the
- * algorithmic bytes of one (3 passes, each reading and writing its array). */
+/* Mean time (ms) of reps forward transforms of the field last uploaded from
+ * host memory into this slot, and the algorithmic bytes of one (3 passes,
+ * each reading and writing its array).  The transforms write a scratch half
+ * spectrum: the resident spectra do not change.  PS_ERR_STATE unless the
+ * slot's field is the last host upload of the context (a later host upload
+ * to another slot, or a load from device memory, replaces or bypasses it). */
 int ps_forward_profile(ps_ctx *ctx, int slot, int reps, double *ms, double *bytes);
 
 /* Adjoint: the gradient of sum_p weights[p] * values[p] (values of the last
@@ -224,8 +238,10 @@ int ps_forward_profile(ps_ctx *ctx, int slot, int reps, double *ms, double *byte
  * grad_is_double, else float32), host memory unless on_device.  The first call
  * allocates its work buffers (two half spectra and five float64 N^3 fields,
  * 6.4 GB at 512^3 in mixed precision) and the batch keeps them until
- * ps_batch_destroy.  Not available with useOnlyPhase (PS_ERR_INVALID).  No
- * counterpart in the reference. */
+ * ps_batch_destroy.  With useOnlyPhase the filtered fields are taken from
+ * p = z/|z| (utils_fftw.c:78-93) and the gradient in spectrum space Y is
+ * pulled back through the normalisation, Y' = (Y - p Re(conj(p) Y)) / |z|,
+ * before the final inverse transform.  No counterpart in the reference. */
 int ps_batch_adjoint(ps_batch *b, int slot, const double *weights, void *grad, int grad_is_double, int on_device);
 
 /* Transform-free check of polygon values: the same sums taken directly over
@@ -242,8 +258,9 @@ int ps_batch_adjoint(ps_batch *b, int slot, const double *weights, void *grad, i
 int ps_direct_check(ps_ctx *ctx, ps_polygon *polys, int npoly, double kbinwidth, int kbinning_case, double *values,
                     double *npolygons, double *sums);
 
-/* Re-run the forward transform of a slot from the last host upload (timing
- * of the FFT passes alone). */
+/* Re-run the forward transform of a slot from its field in the upload
+ * buffer (timing of the FFT passes alone); PS_ERR_STATE under the same
+ * condition as ps_forward_profile. */
 int ps_forward_again(ps_ctx *ctx, int slot);
 
 /* Copy the resident half spectrum of a slot to host memory: (N/2+1)*N*N
@@ -255,7 +272,11 @@ int ps_get_spectrum(ps_ctx *ctx, int slot, void *out);
 /* ---- drop-in symbols (reference signatures) ---------------------------- */
 /* polyspectrum.h:7.  fft_array is the full complex N^3 spectrum of a real
  * field (as get_FT_field returns it); local_n0 must equal nbins and
- * local_n0_start 0 (serial build).  Runs the FP64 path; returns NaN on error. */
+ * local_n0_start 0 (serial build).  Runs the FP64 path; returns NaN on error.
+ * Prints polyspectrum.c:142's line (rank 0, Im/Re of both sums: 0 here, the
+ * sums of real fields).  The spectrum is uploaded on the first call and kept
+ * while later calls pass the same array with the same contents (a hash of the
+ * whole array, taken in host memory on every call). */
 double polyspectrum(int nbins, int local_n0, int local_n0_start, ps_fftw_complex *fft_array, int n, double *k,
                     double *kmin, double *kmax, double kbinwidth, int kbinningCase, double boxsize);
 /* polyspectrum.h:8 */

@@ -144,6 +144,8 @@ def load_library():
     L.ps_batch_params.argtypes = [_vp, ctypes.POINTER(LaunchParams)]
     L.ps_direct_check.argtypes = [_vp, ctypes.POINTER(Polygon), ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                   _dp, _dp, _dp]
+    L.ps_condition_numbers.argtypes = [_vp, ctypes.POINTER(Polygon), ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                       _dp]
     L.ps_multi_create.argtypes = [ctypes.POINTER(_vp), ctypes.c_int, ctypes.POINTER(_vp)]
     L.ps_multi_execute.argtypes = [_vp]
     L.ps_multi_last_ms.restype = ctypes.c_double
@@ -310,6 +312,17 @@ class Context:
                                               v.ctypes.data_as(_dp), npol.ctypes.data_as(_dp),
                                               s.ctypes.data_as(_dp)))
         return v, npol, s.reshape(n, 2)
+
+    def condition_numbers(self, polygons, kbinwidth=1.0, kbinning_case=0):
+        """kappa = sum_x |prod D| / |sum_x prod D| of every polygon (the
+        amplification of rounding errors in its value), through the
+        reference-shaped path of the context's precision."""
+        arr = _polygon_array(polygons)
+        n = len(polygons)
+        k = np.zeros(n)
+        _check(load_library().ps_condition_numbers(self._h, arr, n, float(kbinwidth), int(kbinning_case),
+                                                   k.ctypes.data_as(_dp)))
+        return k
 
     def batch_for_field(self, polygons, field, kbinwidth=1.0, kbinning_case=0, slot=0):
         """Load a field into the slot, then plan a batch on it."""

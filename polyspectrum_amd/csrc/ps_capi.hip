@@ -19,6 +19,7 @@
 #include <cstring>
 #include <map>
 #include <string>
+#include <thread>
 #include <tuple>
 #include <vector>
 
@@ -67,6 +68,8 @@ struct ps_ctx {
     std::vector<int> loaded;
     void *staging = nullptr;             // field upload buffer (N^3 doubles)
     int staging_is_double = 0;
+    int staging_slot = -1;               // slot whose last load went through the upload buffer (-1: none)
+    void *fscratch = nullptr;            // half spectrum the forward profiler transforms into
     int phase_only = 0;
     int method = PS_METHOD_DIRECT;       // evaluation of batches created without explicit parameters
     // reference-shaped path buffers (lazily allocated)
@@ -125,7 +128,8 @@ extern "C" int ps_ctx_destroy(ps_ctx *c)
     if (c->st) (void)hipStreamSynchronize(c->st);
     for (void *p : c->spec) if (p) (void)hipFree(p);
     for (auto &kv : c->twm) (void)hipFree(kv.second);
-    void *bufs[] = {c->tw, c->staging, c->gd, c->gi, c->accD, c->accI, c->partials, c->dsum, c->dleg, c->dcorner};
+    void *bufs[] = {c->tw,       c->staging,  c->gd,   c->gi,   c->accD,    c->accI,
+                    c->partials, c->dsum,     c->dleg, c->dcorner, c->fscratch};
     for (void *p : bufs) if (p) (void)hipFree(p);
     for (hipStream_t b : c->br) {
         (void)hipStreamSynchronize(b);
@@ -192,16 +196,20 @@ extern "C" int ps_load_field(ps_ctx *c, int slot, const void *field, int field_i
         src = c->staging;
         c->staging_is_double = field_is_double ? 1 : 0;
     }
+    // the upload buffer holds this slot's field only after a host upload
+    c->staging_slot = on_device ? (c->staging_slot == slot ? -1 : c->staging_slot) : -1;
     PSCK(forward(c, slot, src, field_is_double));
     HIPCK(hipStreamSynchronize(c->st));
     c->loaded[slot] = 1;
+    if (!on_device) c->staging_slot = slot;
     return PS_OK;
 }
 
 extern "C" int ps_forward_again(ps_ctx *c, int slot)
 {
     if (!c || slot < 0 || slot >= c->nslots) return PS_ERR_INVALID;
-    if (!c->staging || !c->loaded[slot]) return PS_ERR_STATE;
+    if (!c->staging || !c->loaded[slot] || c->staging_slot != slot) return PS_ERR_STATE;
+    HIPCK(hipSetDevice(c->device));
     return forward(c, slot, c->staging, c->staging_is_double);
 }
 
@@ -395,8 +403,9 @@ static int plain_leg(ps_ctx *c, const MaskParams &mp, const void *spec, int mode
     return PS_OK;
 }
 
+// sumI receives sum prod I, or with want_abs (D parts only) sum |prod D|
 static int plain_run(ps_ctx *c, int slot, int n, const double *k, double *kmin, double *kmax, double bw, int kcase,
-                     double *sumD, double *sumI)
+                     double *sumD, double *sumI, int want_abs = 0)
 {
     PSCK(ensure_plain_buffers(c));
     const void *spec = (slot >= 0) ? c->spec[slot] : nullptr;
@@ -406,7 +415,7 @@ static int plain_run(ps_ctx *c, int slot, int n, const double *k, double *kmin, 
     for (int i = 0; i < n; ++i) {
         MaskParams mp;
         PSCK(leg_mask(c, i, n, k, kmin, kmax, bw, kcase, &mp));
-        PSCK(plain_leg(c, mp, spec, (i < n - 1) ? 1 : 2));
+        PSCK(plain_leg(c, mp, spec, (i < n - 1) ? 1 : (want_abs ? 3 : 2)));
     }
     const int64_t nb = (int64_t)c->N * c->N / tile_rows(c->N, c->fp64);
     PSCK(run_reduce_partials(c->partials, 0, nullptr, nb, 1, c->dsum, c->dsum + 1, c->st));
@@ -432,6 +441,25 @@ extern "C" int ps_polyspectrum(ps_ctx *c, int slot, int n, const double *k, doub
     if (sums) {
         sums[0] = sD;
         sums[1] = sI;
+    }
+    return PS_OK;
+}
+
+extern "C" int ps_condition_numbers(ps_ctx *c, const ps_polygon *polys, int npoly, double kbinwidth,
+                                    int kbinning_case, double *kappa)
+{
+    if (!c || !polys || npoly < 1 || !kappa) return PS_ERR_INVALID;
+    for (int p = 0; p < npoly; ++p) {
+        if (polys[p].n < 2 || polys[p].n > 3 || polys[p].slot < 0 || polys[p].slot >= c->nslots)
+            return PS_ERR_INVALID;
+        if (!c->loaded[polys[p].slot]) return PS_ERR_STATE;
+    }
+    HIPCK(hipSetDevice(c->device));
+    for (int p = 0; p < npoly; ++p) {
+        ps_polygon q = polys[p];  // DERIVED binning updates kmin/kmax of the copy
+        double sD = 0.0, sA = 0.0;
+        PSCK(plain_run(c, q.slot, q.n, q.k, q.kmin, q.kmax, kbinwidth, kbinning_case, &sD, &sA, 1));
+        kappa[p] = sA / fabs(sD);
     }
     return PS_OK;
 }
@@ -1679,9 +1707,11 @@ extern "C" int ps_batch_adjoint(ps_batch *b, int slot, const double *weights, vo
     if (!b || !weights || !grad) return PS_ERR_INVALID;
     ps_ctx *c = b->c;
     if (slot < 0 || slot >= c->nslots || !c->loaded[slot]) return PS_ERR_STATE;
-    for (const LegDev &L : b->legs)
-        if (L.phase_only) return PS_ERR_INVALID;  // the phase-only adjoint is not provided
     if (!b->have_sums) return PS_ERR_STATE;
+    // useOnlyPhase: the filtered fields are taken from p = z/|z| (utils_fftw.c:78-93)
+    int phase = 0;
+    for (size_t i = 0; i < b->legs.size(); ++i)
+        if (b->leg_slot[i] == slot && b->legs[i].phase_only) phase = 1;
     HIPCK(hipSetDevice(c->device));
     const int N = c->N;
     const int64_t cells = c->cells(), half = c->half_elems();
@@ -1766,11 +1796,25 @@ extern "C" int ps_batch_adjoint(ps_batch *b, int slot, const double *weights, vo
         ACK(hipMemcpyAsync(mW, &hm[2], sizeof(MaskParams) * nsw, hipMemcpyHostToDevice, c->st));
         ACK(hipMemcpyAsync(wW, &hw[2], sizeof(double) * nsw, hipMemcpyHostToDevice, c->st));
         const void *spec = c->spec[slot];
+        // H1 = the spectrum the filtered fields are taken from: z, or z/|z|
+        auto load_spec = [&]() -> int {
+            if (hipMemcpyAsync(H1, spec, hb, hipMemcpyDeviceToDevice, c->st) != hipSuccess) return PS_ERR_HIP;
+            return phase ? run_phase_normalize(c->fp64, H1, half, c->st) : PS_OK;
+        };
+        // G += IFFT(H1), H1 the gradient in spectrum space; with useOnlyPhase
+        // first pulled back through p = z/|z| (the gradient with respect to z)
+        auto finish = [&]() -> int {
+            if (phase) {
+                const int r = run_phase_pullback(c->fp64, H1, spec, half, c->st);
+                if (r != PS_OK) return r;
+            }
+            return run_inverse_real(c->fp64, N, H1, G, 1, c->tw, c->st);
+        };
         auto filtered = [&](const MaskParams *m, const double *w, int nm, double *out) -> int {
             // out = IFFT(mask * spectrum)
-            hipError_t e = hipMemcpyAsync(H1, spec, hb, hipMemcpyDeviceToDevice, c->st);
-            if (e != hipSuccess) return PS_ERR_HIP;
-            int r = run_apply_masks(c->fp64, N, H1, m, w, nm, c->st);
+            int r = load_spec();
+            if (r != PS_OK) return r;
+            r = run_apply_masks(c->fp64, N, H1, m, w, nm, c->st);
             if (r != PS_OK) return r;
             return run_inverse_real(c->fp64, N, H1, out, 0, c->tw, c->st);
         };
@@ -1783,15 +1827,15 @@ extern "C" int ps_batch_adjoint(ps_batch *b, int slot, const double *weights, vo
         if (g.npre == 0) {
             // S_t = sum D_t^2: dS_t = 2 L_t delta (masks are idempotent), the
             // factor 2 is in the weights
-            ACK(hipMemcpyAsync(H1, spec, hb, hipMemcpyDeviceToDevice, c->st));
+            RCK(load_spec());
             RCK(run_apply_masks(c->fp64, N, H1, mW, wW, nsw, c->st));
-            RCK(run_inverse_real(c->fp64, N, H1, G, 1, c->tw, c->st));
+            RCK(finish());
         } else if (g.npre == 1 && !g.pre_sq) {
             // S_t = sum D_A D_t: dS = L_A[E] + L_W[D_A] = 2 IFFT(F_A W delta)
-            ACK(hipMemcpyAsync(H1, spec, hb, hipMemcpyDeviceToDevice, c->st));
+            RCK(load_spec());
             RCK(run_apply_masks(c->fp64, N, H1, mA, wA, 1, c->st));
             RCK(run_apply_masks(c->fp64, N, H1, mW, wW, nsw, c->st));
-            RCK(run_inverse_real(c->fp64, N, H1, G, 1, c->tw, c->st));
+            RCK(finish());
         } else if (g.pre_sq) {
             // S_t = sum D_A^2 D_t: grad = L_A[2 D_A E] + L_W[D_A^2]
             RCK(filtered(mA, wA, 1, DA));
@@ -1801,7 +1845,7 @@ extern "C" int ps_batch_adjoint(ps_batch *b, int slot, const double *weights, vo
             RCK(run_mul_fields(Q, DA, nullptr, 1.0, cells, c->st));
             RCK(project(Q, H1, mW, wW, nsw));
             RCK(run_add_spec(c->fp64, H1, H2, half, c->st));
-            RCK(run_inverse_real(c->fp64, N, H1, G, 1, c->tw, c->st));
+            RCK(finish());
         } else {
             // S_t = sum D_A D_B D_t: grad = L_A[D_B E] + L_B[D_A E] + L_W[D_A D_B]
             RCK(filtered(mA, wA, 1, DA));
@@ -1815,7 +1859,7 @@ extern "C" int ps_batch_adjoint(ps_batch *b, int slot, const double *weights, vo
             RCK(run_mul_fields(Q, DA, DB, 1.0, cells, c->st));
             RCK(project(Q, H1, mW, wW, nsw));
             RCK(run_add_spec(c->fp64, H1, H2, half, c->st));
-            RCK(run_inverse_real(c->fp64, N, H1, G, 1, c->tw, c->st));
+            RCK(finish());
         }
         ACK(hipStreamSynchronize(c->st));  // the host mask lists are reused
     }
@@ -2040,14 +2084,17 @@ extern "C" int ps_batch_wave_info(const ps_batch *b, int w, int *grid, int *nleg
 extern "C" int ps_forward_profile(ps_ctx *c, int slot, int reps, double *ms, double *bytes)
 {
     if (!c || reps < 1 || !ms || slot < 0 || slot >= c->nslots) return PS_ERR_INVALID;
-    if (!c->staging || !c->loaded[slot]) return PS_ERR_STATE;
+    if (!c->staging || !c->loaded[slot] || c->staging_slot != slot) return PS_ERR_STATE;
     HIPCK(hipSetDevice(c->device));
+    // the transforms go to a scratch half spectrum: the resident spectra stay as they are
+    if (!c->fscratch) HIPCK(hipMalloc(&c->fscratch, c->csize() * (size_t)c->half_elems()));
     hipEvent_t e0, e1;
     HIPCK(hipEventCreate(&e0));
     HIPCK(hipEventCreate(&e1));
     (void)hipEventRecord(e0, c->st);
     int st = PS_OK;
-    for (int r = 0; r < reps && st == PS_OK; ++r) st = forward(c, slot, c->staging, c->staging_is_double);
+    for (int r = 0; r < reps && st == PS_OK; ++r)
+        st = run_forward(c->fp64, c->N, c->staging, c->staging_is_double, c->fscratch, c->tw, c->st);
     (void)hipEventRecord(e1, c->st);
     (void)hipEventSynchronize(e1);
     float t = 0.f;
@@ -2205,15 +2252,48 @@ extern "C" int ps_direct_check(ps_ctx *c, ps_polygon *polys, int npoly, double b
 // A process-wide FP64 context per (nbins, boxsize), as the reference's
 // functions carry no handle.
 static ps_ctx *g_legacy = nullptr;
+// The spectrum resident in the drop-in context: the caller's array and a hash
+// of its contents when it was uploaded.  run.c passes one array to every call
+// (run.c:47, :87), so each call after the first hashes the array in host
+// memory (several threads, about memory speed) instead of uploading it again.
+static const void *g_legacy_src = nullptr;
+static uint64_t g_legacy_hash = 0;
 
 static int legacy_ctx(int nbins, double boxsize)
 {
     if (g_legacy && (g_legacy->N != nbins || g_legacy->L != boxsize)) {
         ps_ctx_destroy(g_legacy);
         g_legacy = nullptr;
+        g_legacy_src = nullptr;
     }
     if (!g_legacy) return ps_ctx_create(nbins, boxsize, PS_PREC_FP64, 1, &g_legacy);
     return PS_OK;
+}
+
+// 64-bit hash of n words: four interleaved multiply-xor chains per thread,
+// the threads' results combined in a fixed order
+static uint64_t hash_words(const uint64_t *w, size_t n)
+{
+    const unsigned hw = std::thread::hardware_concurrency();
+    const size_t nth = std::max<size_t>(1, std::min<size_t>(hw ? hw : 1, 16));
+    const size_t per = (n + nth - 1) / nth;
+    std::vector<uint64_t> part(nth, 0);
+    auto work = [&](size_t t) {
+        const size_t a = std::min(n, t * per), e = std::min(n, a + per);
+        uint64_t h[4] = {0x9E3779B97F4A7C15ull, 0xC2B2AE3D27D4EB4Full, 0x165667B19E3779F9ull, 0x27D4EB2F165667C5ull};
+        size_t i = a;
+        for (; i + 4 <= e; i += 4)
+            for (int j = 0; j < 4; ++j) h[j] = (h[j] ^ w[i + j]) * 0x100000001B3ull + (h[j] >> 29);
+        for (; i < e; ++i) h[0] = (h[0] ^ w[i]) * 0x100000001B3ull + (h[0] >> 29);
+        part[t] = h[0] ^ (h[1] * 3) ^ (h[2] * 5) ^ (h[3] * 7) ^ (uint64_t)(e - a);
+    };
+    std::vector<std::thread> th;
+    for (size_t t = 1; t < nth; ++t) th.emplace_back(work, t);
+    work(0);
+    for (auto &x : th) x.join();
+    uint64_t r = 0;
+    for (size_t t = 0; t < nth; ++t) r = (r ^ part[t]) * 0x9E3779B97F4A7C15ull + t;
+    return r;
 }
 
 // full spectrum rows F[kx][ky][0..N/2] (pitch N/2+1) -> H[kz][kx][ky]
@@ -2245,18 +2325,28 @@ extern "C" double polyspectrum(int nbins, int local_n0, int local_n0_start, ps_f
     ps_ctx *c = g_legacy;
     const int N = nbins, NH = N / 2 + 1;
     if (ensure_spec(c, 0) != PS_OK || ensure_plain_buffers(c) != PS_OK) return nan;
-    // the work buffer gd holds exactly N^2 rows of N/2+1 complex doubles
-    double2 *rows = (double2 *)c->gd;
-    if (hipMemcpy2DAsync(rows, NH * 16, fft_array, (size_t)N * 16, NH * 16, (size_t)N * N, hipMemcpyHostToDevice,
-                         c->st) != hipSuccess)
-        return nan;
-    hipLaunchKernelGGL(k_full_to_half, dim3((NH + 31) / 32, (N + 31) / 32, N), dim3(32, 8), 0, c->st, rows,
-                       (double2 *)c->spec[0], N);
-    if (hipStreamSynchronize(c->st) != hipSuccess) return nan;
-    c->loaded[0] = 1;
+    const uint64_t h = hash_words(reinterpret_cast<const uint64_t *>(fft_array), 2 * (size_t)N * N * N);
+    if (!(c->loaded[0] && g_legacy_src == fft_array && g_legacy_hash == h)) {
+        g_legacy_src = nullptr;
+        // the work buffer gd holds exactly N^2 rows of N/2+1 complex doubles
+        double2 *rows = (double2 *)c->gd;
+        if (hipMemcpy2DAsync(rows, NH * 16, fft_array, (size_t)N * 16, NH * 16, (size_t)N * N,
+                             hipMemcpyHostToDevice, c->st) != hipSuccess)
+            return nan;
+        hipLaunchKernelGGL(k_full_to_half, dim3((NH + 31) / 32, (N + 31) / 32, N), dim3(32, 8), 0, c->st, rows,
+                           (double2 *)c->spec[0], N);
+        if (hipStreamSynchronize(c->st) != hipSuccess) return nan;
+        c->loaded[0] = 1;
+        g_legacy_src = fft_array;
+        g_legacy_hash = h;
+    }
     c->phase_only = 0;
-    double r = nan;
-    if (ps_polyspectrum(c, 0, n, k, kmin, kmax, kbinwidth, kbinningCase, &r, nullptr) != PS_OK) return nan;
+    double r = nan, sums[2] = {0.0, 0.0};
+    if (ps_polyspectrum(c, 0, n, k, kmin, kmax, kbinwidth, kbinningCase, &r, sums) != PS_OK) return nan;
+    // polyspectrum.c:142 (rank 0): Im/Re of both sums; the imaginary parts of
+    // sums of real filtered fields are exactly zero here
+    printf("rank %d: cimag(sumNumPolygons)/creal(sumNumPolygons) = %e\n\tcimag(sumValuesPolygons)/creal(sumValuesPolygons) = %e\n",
+           0, 0.0 / sums[1], 0.0 / sums[0]);
     return r;
 }
 

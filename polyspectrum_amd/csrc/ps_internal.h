@@ -138,6 +138,9 @@ int run_inverse_real(int fp64, int N, void *H, double *out, int accumulate, cons
 int run_apply_masks(int fp64, int N, void *H, const MaskParams *masks, const double *w, int nm, hipStream_t st);
 int run_add_spec(int fp64, void *a, const void *b, int64_t n, hipStream_t st);
 int run_mul_fields(double *out, const double *a, const double *b, double alpha, int64_t n, hipStream_t st);
+// phase-only adjoint: H = z/|z| in place; H = (H - p Re(conj(p) H)) / |z| with p = z/|z|, z = spec
+int run_phase_normalize(int fp64, void *H, int64_t n, hipStream_t st);
+int run_phase_pullback(int fp64, void *H, const void *spec, int64_t n, hipStream_t st);
 int run_corner_extremes(const MaskParams &mp, int N, int K, double bw, double2 *partials, int nblocks, hipStream_t st);
 
 // Fourier-space groups: tiles [t0, t1) of the Zf and Fx passes, work items

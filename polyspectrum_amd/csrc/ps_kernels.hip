@@ -451,6 +451,7 @@ __device__ __forceinline__ void load_zline_pair(cplx<T> *lds, const cplx<T> *__r
 //   mode 0: accD = D, accI = I
 //   mode 1: accD *= D, accI *= I                   (utils_fftw.c:32-44)
 //   mode 2: partial sums of accD*D and accI*I       (utils_fftw.c:60-76)
+//   mode 3: partial sums of accD*D and |accD*D|      (condition number)
 // Accumulators are FP64 real N^3 grids.
 template <typename T, int N>
 __global__ void __launch_bounds__(NT) k_inv_zc_plain(const cplx<T> *__restrict__ gd, const cplx<T> *__restrict__ gi,
@@ -480,12 +481,22 @@ __global__ void __launch_bounds__(NT) k_inv_zc_plain(const cplx<T> *__restrict__
         } else if (mode == 1) {
             if (accD) accD[o] *= (double)v.x;
             accI[o] *= (double)v.y;
-        } else {
+        } else if (mode == 2) {
             if (accD) sd.add(accD[o] * (double)v.x);
             si.add(accI[o] * (double)v.y);
         }
     }
-    if (mode == 2) {
+    if (mode == 3) {
+        // the sum of the D products and the sum of their magnitudes (the
+        // numerator of the sum's condition number); the I part is not summed
+        for (int idx = tid; idx < TLC * N; idx += NT) {
+            const int l = idx / N, z = idx % N;
+            const double p = accD[base + idx] * (double)lds[lidx<TLC>(z, l)].x;
+            sd.add(p);
+            si.add(fabs(p));
+        }
+    }
+    if (mode >= 2) {
         double a = sd.s, b = sd.c, c = si.s, d = si.c;
         wave_sum2(a, b);
         wave_sum2(c, d);
@@ -848,6 +859,38 @@ __global__ void __launch_bounds__(NT) k_mul_fields(double *__restrict__ out, con
     for (int64_t c = (int64_t)blockIdx.x * NT + threadIdx.x; c < n; c += (int64_t)gridDim.x * NT) {
         const double x = a[c];
         out[c] = alpha * x * (b ? b[c] : x);
+    }
+}
+
+// z -> z/|z| over a half spectrum, in place (utils_fftw.c:78-93); an exact
+// zero stays zero (it is never selected by a shell of a closing polygon)
+template <typename T>
+__global__ void __launch_bounds__(NT) k_phase_normalize(cplx<T> *__restrict__ H, int64_t n)
+{
+    for (int64_t c = (int64_t)blockIdx.x * NT + threadIdx.x; c < n; c += (int64_t)gridDim.x * NT) {
+        const cplx<T> z = H[c];
+        const double a = hypot((double)z.x, (double)z.y);
+        H[c] = (a > 0.0) ? mk<T>((T)((double)z.x / a), (T)((double)z.y / a)) : mk<T>(0, 0);
+    }
+}
+
+// Pull a spectrum-space gradient Y back through p = z/|z|: with
+// dp = (dz - p Re(conj(p) dz)) / |z|, the gradient with respect to z is
+// Y' = (Y - p Re(conj(p) Y)) / |z| (zero where z = 0).  H holds Y, spec z.
+template <typename T>
+__global__ void __launch_bounds__(NT) k_phase_pullback(cplx<T> *__restrict__ H, const cplx<T> *__restrict__ spec,
+                                                       int64_t n)
+{
+    for (int64_t c = (int64_t)blockIdx.x * NT + threadIdx.x; c < n; c += (int64_t)gridDim.x * NT) {
+        const cplx<T> z = spec[c], y = H[c];
+        const double a = hypot((double)z.x, (double)z.y);
+        if (a > 0.0) {
+            const double px = (double)z.x / a, py = (double)z.y / a;
+            const double r = px * (double)y.x + py * (double)y.y;
+            H[c] = mk<T>((T)(((double)y.x - px * r) / a), (T)(((double)y.y - py * r) / a));
+        } else {
+            H[c] = mk<T>(0, 0);
+        }
     }
 }
 
@@ -1296,6 +1339,24 @@ int run_add_spec(int fp64, void *a, const void *b, int64_t n, hipStream_t st)
     else
         hipLaunchKernelGGL(k_add_spec<float>, dim3(grid_for(n)), dim3(NT), 0, st, (cplx<float> *)a,
                            (const cplx<float> *)b, n);
+    return (int)hipGetLastError();
+}
+
+int run_phase_normalize(int fp64, void *H, int64_t n, hipStream_t st)
+{
+    if (fp64) hipLaunchKernelGGL(k_phase_normalize<double>, dim3(grid_for(n)), dim3(NT), 0, st, (cplx<double> *)H, n);
+    else hipLaunchKernelGGL(k_phase_normalize<float>, dim3(grid_for(n)), dim3(NT), 0, st, (cplx<float> *)H, n);
+    return (int)hipGetLastError();
+}
+
+int run_phase_pullback(int fp64, void *H, const void *spec, int64_t n, hipStream_t st)
+{
+    if (fp64)
+        hipLaunchKernelGGL(k_phase_pullback<double>, dim3(grid_for(n)), dim3(NT), 0, st, (cplx<double> *)H,
+                           (const cplx<double> *)spec, n);
+    else
+        hipLaunchKernelGGL(k_phase_pullback<float>, dim3(grid_for(n)), dim3(NT), 0, st, (cplx<float> *)H,
+                           (const cplx<float> *)spec, n);
     return (int)hipGetLastError();
 }
 

@@ -208,3 +208,37 @@ def test_driver_outputs(tmp_path, gpu_lib, kw):
         np.testing.assert_allclose(got, ref, rtol=2e-6, atol=1e-300)
     assert "kbinningCase = %d" % CASES[kw["case"]] in r.stdout
     assert "Execution took" in r.stdout
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("quirks", [1, 0])
+def test_driver_power_spectrum_given_binning(tmp_path, gpu_lib, quirks):
+    """n = 2, N/2 bins, GIVEN_BINNING.  The reference keeps the last leg's
+    kmin/kmax at k[1] (set once, kvectors.c:92-97; run.c:56-83 updates them for
+    n = 3 only), so every bin but k[1] multiplies two disjoint shells: no
+    polygon closes and the value is a ratio of roundoff.  referenceQuirks = 1
+    (default) reproduces that: bin k[1] equals the oracle and the other bins
+    count no polygons.  referenceQuirks = 0 updates the range with k[i]: every
+    bin is the DEFAULT power spectrum, as the oracle gives it."""
+    import oracle as O
+    from polyspectrum_amd.synthetic import lognormal_field
+    N, L = 16, 100.0
+    dens = lognormal_field(N, 31).astype(np.float32)
+    dens.tofile(tmp_path / "dens.bin")
+    extra = "[GPU]\nprecision = FP64\nreferenceQuirks = %d\n" % quirks
+    r = run(make_ini(tmp_path, n=2, nv=2, case="GIVEN_BINNING", extra=extra))
+    assert r.returncode == 0, r.stderr
+    body = [list(map(float, l.split("\t"))) for l in (tmp_path / "test_ps.dat").read_text().splitlines()[1:]]
+    assert len(body) == N // 2
+    spec = O.forward_spectrum(dens.astype(np.float64))
+    PI = math.acos(-1.0)
+    ks = [2. * PI / L * (i + 1) for i in range(N // 2)]
+    for i, row in enumerate(body):
+        assert abs(row[0] - ks[i]) <= 1e-6 * ks[i]
+        if quirks and i != 1:
+            assert abs(row[2]) < 1e-6, row  # no closed polygon: Npolygons = 0
+            continue
+        lo = [ks[i], ks[1] if quirks else ks[i], 0.0]
+        v, lo2, hi2, _ = O.polyspectrum(spec, 2, [ks[i], ks[i], 0.0], lo, list(lo), 1.0, 2, L)
+        npg = O.num_polygons(N, 2, [ks[i], ks[i], 0.0], lo2, hi2, 1.0, 2, L)
+        np.testing.assert_allclose(row[1:], [v, npg], rtol=2e-6)

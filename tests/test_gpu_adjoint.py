@@ -193,3 +193,168 @@ def test_autograd_binding(gpu_lib):
     np.testing.assert_allclose(vals.detach().cpu().numpy(), ref_vals, rtol=1e-10)
     gg = field.grad.cpu().numpy()
     assert np.abs(gg - ref_grad).max() / np.abs(ref_grad).max() < 1e-10
+
+
+# ------------------------------------------------------------------ useOnlyPhase
+@pytest.mark.parametrize("case,pairs", [(0, [(5.0, 5.0)]), (2, [(4.0, 6.0)])])
+def test_adjoint_phase_only_vs_oracle(gpu_lib, case, pairs):
+    """Phase-only fields (p = z/|z|, utils_fftw.c:78-93): the FP64 adjoint
+    against the oracle's CPU gradient (which pulls R = sum_i m_i B(Q_i) back
+    through the normalisation), bar 1e-10 of max |grad|; power spectra are
+    constant (V N^-6) and have no gradient."""
+    import oracle as O
+    P = gpu_lib
+    N, L = 32, 100.0
+    field = lognormal_field(N, 505, sigma=0.8, alpha=1.5).astype(np.float64)
+    kf = 2 * PI / L
+    polys = []
+    for m1, m2 in pairs:
+        polys += triangles(m1 * kf, m2 * kf, 6, case)
+    polys += [{"n": 2, "k": [3 * kf] * 2, "kmin": [3 * kf] * 2, "kmax": [3 * kf] * 2}]
+    w = np.random.default_rng(4).standard_normal(len(polys))
+    ref_vals, ref_grad = O.polygon_gradient(field, polys, 1.0, case, L, w, phase=True)
+    with P.Context(N, L, P.PREC_FP64) as ctx:
+        ctx.load_field(field)
+        ctx.set_phase_only(1)
+        b = ctx.batch(polys, 1.0, case)
+        vals, _, _ = b.execute()
+        g = b.adjoint(w)
+        b.close()
+    np.testing.assert_allclose(vals, ref_vals, rtol=1e-10)
+    err = np.abs(g - ref_grad).max() / np.abs(ref_grad).max()
+    print("phase-only adjoint vs oracle:", err)
+    assert err < 1e-10
+
+
+def test_adjoint_phase_only_finite_differences(gpu_lib):
+    """Central differences (eps = 1e-3) of L = sum_t (B_t - B*_t)^2 through the
+    FP64 phase-only path at random voxels, bar 1e-4 relative."""
+    P = gpu_lib
+    N, L = 32, 100.0
+    kf = 2 * PI / L
+    polys = triangles(5.0 * kf, 5.0 * kf, 8, 0)
+    field = lognormal_field(N, 17, sigma=0.8, alpha=1.5).astype(np.float64)
+    other = lognormal_field(N, 18, sigma=0.8, alpha=1.5).astype(np.float64)
+    with P.Context(N, L, P.PREC_FP64) as ctx:
+        ctx.set_phase_only(1)
+        ctx.load_field(other)
+        bt = ctx.batch(polys, 1.0, 0)
+        target, _, _ = bt.execute()
+        bt.close()
+        loss, b = _loss_fn(P, ctx, polys, 0, target)
+        _, v = loss(field)
+        g = b.adjoint(2.0 * (v - target))
+        rng = np.random.default_rng(6)
+        errs = []
+        for _ in range(12):
+            x = tuple(rng.integers(0, N, 3))
+            fp, fm = field.copy(), field.copy()
+            fp[x] += 1e-3
+            fm[x] -= 1e-3
+            fd = (loss(fp)[0] - loss(fm)[0]) / 2e-3
+            errs.append(abs(fd - g[x]) / abs(g[x]))
+        b.close()
+    print("phase-only FD rel errors:", max(errs))
+    assert max(errs) < 1e-4
+
+
+# ------------------------------------------------------------------ 512^3 workload
+def _config4(N, L):
+    k = 0.4
+    return [{"n": 3, "k": [k, k, math.sqrt(2 * k * k - 2 * k * k * math.cos(PI * i / 16))],
+             "kmin": [0.0] * 3, "kmax": [0.0] * 3} for i in range(16)]
+
+
+@pytest.fixture(scope="module")
+def config4_fields():
+    N = 512
+    return (lognormal_field(N, 4000, sigma=0.8, alpha=1.5), lognormal_field(N, 4001, sigma=0.8, alpha=1.5))
+
+
+def test_config4_finite_differences_512(gpu_lib, config4_fields):
+    """L = sum_t (B_t - B*_t)^2 over 16 triangles (k1 = k2 = 0.4 h/Mpc, 16 theta
+    bins, DEFAULT binning) of a 512^3 log-normal field, L = 400 Mpc/h, B* from a
+    second seeded field: the FP64 adjoint against central differences
+    (eps = 1e-3) at 64 random voxels, each within 1e-4 of the gradient there.
+    The perturbed fields are uploaded from device memory."""
+    import torch
+    P = gpu_lib
+    N, L = 512, 400.0
+    field, other = config4_fields
+    polys = _config4(N, L)
+    with P.Context(N, L, P.PREC_FP64) as ctx:
+        ctx.load_field(other)
+        b = ctx.batch(polys, 1.0, 0)
+        target, _, _ = b.execute()
+        ctx.load_field(field)
+        v, _, _ = b.execute()
+        g = b.adjoint(2.0 * (v - target))
+        t = torch.from_numpy(field.astype(np.float64)).cuda()
+        rng = np.random.default_rng(44)
+        errs = []
+        for _ in range(64):
+            x = tuple(int(i) for i in rng.integers(0, N, 3))
+            x0 = float(t[x])
+            losses = []
+            for xv in (x0 + 1e-3, x0 - 1e-3):
+                t[x] = xv
+                ctx.load_field(t)
+                vv, _, _ = b.execute()
+                losses.append(float(((vv - target) ** 2).sum()))
+            t[x] = x0
+            fd = (losses[0] - losses[1]) / 2e-3
+            errs.append(abs(fd - g[x]) / abs(g[x]))
+        b.close()
+    errs = np.array(errs)
+    print(f"512^3 FD: max rel err {errs.max():.3e}, median {np.median(errs):.3e}")
+    assert errs.max() < 1e-4
+
+
+def test_config4_crop_vs_cpu_gradient(gpu_lib, config4_fields):
+    """The same setup on the 128^3 crop (L = 100 Mpc/h, same cell size): the
+    FP64 adjoint against the oracle's CPU FP64 gradient (one polygon at a
+    time, no combined mask), bar 1e-10 of max |grad|."""
+    import oracle as O
+    P = gpu_lib
+    N, L = 128, 100.0
+    field = np.ascontiguousarray(config4_fields[0][:N, :N, :N]).astype(np.float64)
+    other = np.ascontiguousarray(config4_fields[1][:N, :N, :N]).astype(np.float64)
+    polys = _config4(N, L)
+    with P.Context(N, L, P.PREC_FP64) as ctx:
+        ctx.load_field(other)
+        b = ctx.batch(polys, 1.0, 0)
+        target, _, _ = b.execute()
+        ctx.load_field(field)
+        v, _, _ = b.execute()
+        w = 2.0 * (v - target)
+        g = b.adjoint(w)
+        b.close()
+    ref_vals, ref = O.polygon_gradient(field, polys, 1.0, 0, L, w)
+    np.testing.assert_allclose(v, ref_vals, rtol=1e-10)
+    err = np.abs(g - ref).max() / np.abs(ref).max()
+    print("128^3 crop: adjoint vs CPU gradient", err)
+    assert err < 1e-10
+
+
+def test_config4_mixed_vs_fp64_512(gpu_lib, config4_fields):
+    """The mixed-precision adjoint (FP32 transforms of the float32 field) at
+    512^3 against the FP64 one, bar 1e-4 of max |grad|."""
+    P = gpu_lib
+    N, L = 512, 400.0
+    field, other = config4_fields
+    polys = _config4(N, L)
+    g = {}
+    for prec in (P.PREC_FP64, P.PREC_MIXED):
+        with P.Context(N, L, prec) as ctx:
+            ctx.load_field(other)
+            b = ctx.batch(polys, 1.0, 0)
+            target, _, _ = b.execute()
+            ctx.load_field(field)
+            v, _, _ = b.execute()
+            if prec == P.PREC_FP64:
+                w = 2.0 * (v - target)
+            g[prec] = b.adjoint(w)
+            b.close()
+    err = np.abs(g[P.PREC_MIXED] - g[P.PREC_FP64]).max() / np.abs(g[P.PREC_FP64]).max()
+    print("512^3 mixed adjoint vs FP64:", err)
+    assert err < 1e-4
