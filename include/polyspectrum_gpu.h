@@ -39,6 +39,7 @@ extern "C" {
 #define PS_ERR_NOMEM (-4)          /* device or host allocation failed */
 #define PS_ERR_STATE (-5)          /* no field loaded, or wrong slot */
 #define PS_ERR_NO_DEVICE (-6)      /* no usable GPU */
+#define PS_ERR_SCRATCH (-7)        /* work exceeds the static scratch buffer */
 
 /* ---- precision of a context ------------------------------------------ */
 #define PS_PREC_FP64 1  /* FP64 FFTs and products: the plain parity path */
@@ -257,6 +258,20 @@ int ps_batch_adjoint(ps_batch *b, int slot, const double *weights, void *grad, i
  * polyspectrum()/num_polygons() (polyspectrum.c:51-252). */
 int ps_direct_check(ps_ctx *ctx, ps_polygon *polys, int npoly, double kbinwidth, int kbinning_case, double *values,
                     double *npolygons, double *sums);
+
+/* ps_direct_check that allocates no device memory, for callers that cannot
+ * call hipMalloc (code under stream capture, tight serving loops, harnesses
+ * that forbid allocation).  No counterpart in the reference.  The cell lists,
+ * cell values, partial sums and DERIVED corner extremes live in one static
+ * device array of PS_DIRECT_SCRATCH_BYTES per GPU (held until the process
+ * ends); a polygon set whose two first shells need more returns
+ * PS_ERR_SCRATCH (about 1.4 million cells in both shells together).  Every
+ * call on a GPU shares that array: calls that overlap on one GPU (from
+ * several host threads) race.  Results equal ps_direct_check's bit for bit.
+ * Host-side work lists use ordinary host memory. */
+#define PS_DIRECT_SCRATCH_BYTES ((size_t)32 << 20)
+int ps_direct_check_static(ps_ctx *ctx, ps_polygon *polys, int npoly, double kbinwidth, int kbinning_case,
+                           double *values, double *npolygons, double *sums);
 
 /* Re-run the forward transform of a slot from its field in the upload
  * buffer (timing of the FFT passes alone); PS_ERR_STATE under the same

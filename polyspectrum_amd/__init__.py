@@ -144,6 +144,8 @@ def load_library():
     L.ps_batch_params.argtypes = [_vp, ctypes.POINTER(LaunchParams)]
     L.ps_direct_check.argtypes = [_vp, ctypes.POINTER(Polygon), ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                   _dp, _dp, _dp]
+    L.ps_direct_check_static.argtypes = [_vp, ctypes.POINTER(Polygon), ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                         _dp, _dp, _dp]
     L.ps_condition_numbers.argtypes = [_vp, ctypes.POINTER(Polygon), ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                        _dp]
     L.ps_multi_create.argtypes = [ctypes.POINTER(_vp), ctypes.c_int, ctypes.POINTER(_vp)]
@@ -299,18 +301,19 @@ class Context:
                                                 int(reps), ctypes.byref(lp), ctypes.byref(ms)))
         return _params_dict(lp), ms.value
 
-    def direct_check(self, polygons, kbinwidth=1.0, kbinning_case=0):
+    def direct_check(self, polygons, kbinwidth=1.0, kbinning_case=0, static=False):
         """The polygon values taken without transforms, over the closed
         lattice polygons of the shells (an independent check of a batch).
-        Returns (values, npolygons, sums[n,2]) as Batch.execute."""
+        Returns (values, npolygons, sums[n,2]) as Batch.execute.  static: the
+        entry point that allocates no device memory (ps_direct_check_static)."""
         arr = _polygon_array(polygons)
         n = len(polygons)
         v = np.zeros(n)
         npol = np.zeros(n)
         s = np.zeros(2 * n)
-        _check(load_library().ps_direct_check(self._h, arr, n, float(kbinwidth), int(kbinning_case),
-                                              v.ctypes.data_as(_dp), npol.ctypes.data_as(_dp),
-                                              s.ctypes.data_as(_dp)))
+        fn = load_library().ps_direct_check_static if static else load_library().ps_direct_check
+        _check(fn(self._h, arr, n, float(kbinwidth), int(kbinning_case), v.ctypes.data_as(_dp),
+                  npol.ctypes.data_as(_dp), s.ctypes.data_as(_dp)))
         return v, npol, s.reshape(n, 2)
 
     def condition_numbers(self, polygons, kbinwidth=1.0, kbinning_case=0):

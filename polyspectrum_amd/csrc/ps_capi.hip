@@ -18,6 +18,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <tuple>
@@ -94,6 +95,7 @@ extern "C" const char *ps_strerror(int s)
     case PS_ERR_NOMEM: return "out of memory";
     case PS_ERR_STATE: return "no field loaded in this slot";
     case PS_ERR_NO_DEVICE: return "no GPU available";
+    case PS_ERR_SCRATCH: return "the shells' cells exceed the static scratch buffer (PS_DIRECT_SCRATCH_BYTES)";
     }
     return "unknown status";
 }
@@ -276,15 +278,21 @@ static int shell_band(const MaskParams &p, int N)
 
 // DERIVED binning: kmin/kmax of a shell from its cells' cube corners
 // (construct_kfilter_with_determining_kbinwidth filter.c:78-114)
-static int derived_kminmax(ps_ctx *c, double k, double bw, double *kmin, double *kmax)
+constexpr int kCornerBlocks = 256;  // partial extremes of the corner reduction
+
+// corner: kCornerBlocks device partials, or nullptr for the context's own buffer
+static int derived_kminmax(ps_ctx *c, double k, double bw, double *kmin, double *kmax, double2 *corner = nullptr)
 {
     MaskParams mp = mp_shell(MASK_DETERMINING, k, bw, c->L);
     const int K = shell_band(mp, c->N);
-    const int nblocks = 256;
-    if (!c->dcorner) HIPCK(hipMalloc(&c->dcorner, sizeof(double2) * nblocks));
-    PSCK(run_corner_extremes(mp, c->N, K, bw, c->dcorner, nblocks, c->st));
-    std::vector<double2> h(nblocks);
-    HIPCK(hipMemcpyAsync(h.data(), c->dcorner, sizeof(double2) * nblocks, hipMemcpyDeviceToHost, c->st));
+    const int nblocks = kCornerBlocks;
+    if (!corner) {
+        if (!c->dcorner) HIPCK(hipMalloc(&c->dcorner, sizeof(double2) * nblocks));
+        corner = c->dcorner;
+    }
+    PSCK(run_corner_extremes(mp, c->N, K, bw, corner, nblocks, c->st));
+    double2 h[kCornerBlocks];
+    HIPCK(hipMemcpyAsync(h, corner, sizeof(double2) * nblocks, hipMemcpyDeviceToHost, c->st));
     HIPCK(hipStreamSynchronize(c->st));
     double lo = 1.e30 * 1.e30, hi = 0.0;
     for (auto &v : h) {
@@ -325,13 +333,13 @@ static void k3_min_and_max(const double *k, double *kmin, double *kmax)
 
 // the per-leg mask choice of polyspectrum.c:79-118; may update kmin/kmax
 static int leg_mask(ps_ctx *c, int i, int n, const double *k, double *kmin, double *kmax, double bw, int kcase,
-                    MaskParams *out)
+                    MaskParams *out, double2 *corner = nullptr)
 {
     const double L = c->L;
     switch (kcase) {
     case PS_BINNING_DERIVED:
         if (i < n - 1) {
-            PSCK(derived_kminmax(c, k[i], bw, &kmin[i], &kmax[i]));
+            PSCK(derived_kminmax(c, k[i], bw, &kmin[i], &kmax[i], corner));
             *out = mp_shell(MASK_DETERMINING, k[i], bw, L);
         } else {
             if (n == 3) {
@@ -2148,12 +2156,39 @@ struct DevTemps {
     ~DevTemps() { release(); }
 };
 
-extern "C" int ps_direct_check(ps_ctx *c, ps_polygon *polys, int npoly, double bw, int kcase, double *values,
-                               double *npolygons, double *sums)
+// Device buffers of one shell pairing of the transform-free check: taken
+// from a caller's scratch region (no allocation) or allocated per call.
+struct DirectBufs {
+    unsigned char *base = nullptr;  // scratch region, or nullptr: allocate
+    size_t size = 0;
+    size_t used = 0;
+    DevTemps tmp;
+    template <typename T> int take(T **q, size_t count)
+    {
+        if (!base) return tmp.alloc(q, count) == hipSuccess ? PS_OK : PS_ERR_NOMEM;
+        const size_t bytes = (sizeof(T) * std::max<size_t>(count, 1) + 255) & ~(size_t)255;
+        if (used + bytes > size) return PS_ERR_SCRATCH;
+        *q = reinterpret_cast<T *>(base + used);
+        used += bytes;
+        return PS_OK;
+    }
+    void release()
+    {
+        tmp.release();
+        used = 0;
+    }
+};
+
+// scratch: a device region of scratch_bytes for every buffer (first
+// kCornerBlocks double2 for DERIVED corner extremes), or nullptr to allocate
+static int direct_check_impl(ps_ctx *c, ps_polygon *polys, int npoly, double bw, int kcase, double *values,
+                             double *npolygons, double *sums, unsigned char *scratch, size_t scratch_bytes)
 {
     if (!c || !polys || npoly <= 0 || !values) return PS_ERR_INVALID;
     if (c->N > 1024) return PS_ERR_UNSUPPORTED_N;  // cells are packed 10 bits per coordinate
     HIPCK(hipSetDevice(c->device));
+    double2 *corner = scratch ? reinterpret_cast<double2 *>(scratch) : nullptr;
+    const size_t corner_bytes = scratch ? (sizeof(double2) * kCornerBlocks + 255) & ~(size_t)255 : 0;
     // polygons sharing their enumerated shells (bispectra: legs 1 and 2,
     // power spectra: leg 1) share one pass over the cells
     struct Pairing {
@@ -2169,7 +2204,7 @@ extern "C" int ps_direct_check(ps_ctx *c, ps_polygon *polys, int npoly, double b
         if (P.slot < 0 || P.slot >= c->nslots) return PS_ERR_INVALID;
         if (!c->loaded[P.slot]) return PS_ERR_STATE;
         MaskParams mp[3];
-        for (int i = 0; i < P.n; ++i) PSCK(leg_mask(c, i, P.n, P.k, P.kmin, P.kmax, bw, kcase, &mp[i]));
+        for (int i = 0; i < P.n; ++i) PSCK(leg_mask(c, i, P.n, P.k, P.kmin, P.kmax, bw, kcase, &mp[i], corner));
         size_t g = 0;
         while (g < pg.size() && !(pg[g].n == P.n && pg[g].slot == P.slot && same_mask(pg[g].m1, mp[0]) &&
                                   (P.n == 2 || same_mask(pg[g].m2, mp[1]))))
@@ -2188,7 +2223,9 @@ extern "C" int ps_direct_check(ps_ctx *c, ps_polygon *polys, int npoly, double b
     const int N = c->N;
     std::vector<double> S(npoly, 0.0), C(npoly, 0.0);
     std::vector<int> cells1, cells2;
-    DevTemps tmp;
+    DirectBufs buf;
+    buf.base = scratch ? scratch + corner_bytes : nullptr;
+    buf.size = scratch_bytes > corner_bytes ? scratch_bytes - corner_bytes : 0;
     for (const Pairing &q : pg) {
         shell_cells(q.m1, N, cells1);
         if (q.n == 3) shell_cells(q.m2, N, cells2);
@@ -2198,14 +2235,18 @@ extern "C" int ps_direct_check(ps_ctx *c, ps_polygon *polys, int npoly, double b
         const int nb = direct_sum_blocks(n1, n2);
         int *dc1 = nullptr, *dc2 = nullptr;
         double2 *dv1 = nullptr, *dv2 = nullptr, *dpart = nullptr;
-        HIPCK(tmp.alloc(&dc1, n1));
-        HIPCK(tmp.alloc(&dv1, n1));
-        HIPCK(tmp.alloc(&dpart, (size_t)nb * DIRECT_MEMBERS));
+        // the previous pairing's reads of these buffers are complete (its
+        // partial sums were copied back synchronously)
+        PSCK(buf.take(&dc1, n1));
+        PSCK(buf.take(&dv1, n1));
+        PSCK(buf.take(&dpart, (size_t)nb * DIRECT_MEMBERS));
+        if (n2 > 0) {
+            PSCK(buf.take(&dc2, n2));
+            PSCK(buf.take(&dv2, n2));
+        }
         HIPCK(hipMemcpyAsync(dc1, cells1.data(), sizeof(int) * n1, hipMemcpyHostToDevice, c->st));
         PSCK(run_direct_values(c->fp64, H, N, c->phase_only, dc1, n1, dv1, c->st));
         if (n2 > 0) {
-            HIPCK(tmp.alloc(&dc2, n2));
-            HIPCK(tmp.alloc(&dv2, n2));
             HIPCK(hipMemcpyAsync(dc2, cells2.data(), sizeof(int) * n2, hipMemcpyHostToDevice, c->st));
             PSCK(run_direct_values(c->fp64, H, N, c->phase_only, dc2, n2, dv2, c->st));
         }
@@ -2231,7 +2272,8 @@ extern "C" int ps_direct_check(ps_ctx *c, ps_polygon *polys, int npoly, double b
                 C[q.poly[m0 + j]] = (double)cnt;
             }
         }
-        tmp.release();
+        HIPCK(hipStreamSynchronize(c->st));  // the copies of the cell lists have been read
+        buf.release();
     }
     const double volume = c->L * c->L * c->L;
     const double cells = (double)c->cells();
@@ -2246,6 +2288,45 @@ extern "C" int ps_direct_check(ps_ctx *c, ps_polygon *polys, int npoly, double b
         }
     }
     return PS_OK;
+}
+
+extern "C" int ps_direct_check(ps_ctx *c, ps_polygon *polys, int npoly, double bw, int kcase, double *values,
+                               double *npolygons, double *sums)
+{
+    return direct_check_impl(c, polys, npoly, bw, kcase, values, npolygons, sums, nullptr, 0);
+}
+
+// ---------------------------------------------------------------- allocation-free check
+// The buffers of ps_direct_check_static: one file-scope device array per GPU
+// (the code object's copy on each device), held until the process ends.  Its
+// host-side name is not a device address; every use goes through the address
+// hipGetSymbolAddress returns on that device, kept per device.
+static __device__ __align__(256) unsigned char scratch[PS_DIRECT_SCRATCH_BYTES];
+constexpr int kMaxDevices = 64;
+static unsigned char *g_scratch_dev[kMaxDevices] = {};
+static std::mutex g_scratch_mu;
+
+static int scratch_on(int device, unsigned char **out)
+{
+    if (device < 0 || device >= kMaxDevices) return PS_ERR_INVALID;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[device]) {
+        void *p = nullptr;
+        HIPCK(hipSetDevice(device));
+        HIPCK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_dev[device] = static_cast<unsigned char *>(p);
+    }
+    *out = g_scratch_dev[device];
+    return PS_OK;
+}
+
+extern "C" int ps_direct_check_static(ps_ctx *c, ps_polygon *polys, int npoly, double bw, int kcase,
+                                      double *values, double *npolygons, double *sums)
+{
+    if (!c) return PS_ERR_INVALID;
+    unsigned char *p = nullptr;
+    PSCK(scratch_on(c->device, &p));
+    return direct_check_impl(c, polys, npoly, bw, kcase, values, npolygons, sums, p, PS_DIRECT_SCRATCH_BYTES);
 }
 
 // ---------------------------------------------------------------- drop-in symbols
