@@ -74,9 +74,10 @@ def cpu_model():
 def cpu_baseline(N, L):
     """Time the oracle (FP64 restatement of polyspectrum.c) on one core: one
     backward C2C FFT, one mask, one complex product and one complex sum of a
-    512^3 grid; per triangle the reference's polyspectrum() performs 6 FFTs,
-    3 masks, 9 products and 6 sums (polyspectrum.c:51-161, incl. the 4
-    diagnostic sums of :142)."""
+    512^3 grid.  Per triangle (B and Npolygons, as the GPU step returns them)
+    the reference runs polyspectrum(): 6 FFTs, 3 masks, 9 products and 6 sums
+    (polyspectrum.c:51-161, incl. the 4 diagnostic sums of :142), and
+    num_polygons(): 3 FFTs, 3 masks, 3 products and 1 sum (polyspectrum.c:163-252)."""
     import oracle as O
     lib = O.lib()
     dp = ctypes.POINTER(ctypes.c_double)
@@ -101,7 +102,7 @@ def cpu_baseline(N, L):
     t0 = time.perf_counter()
     lib.or_sum(n3, b.ctypes.data_as(dp), s.ctypes.data_as(dp))
     t_sum = time.perf_counter() - t0
-    per_tri = 6 * t_fft + 3 * t_mask + 9 * t_prod + 6 * t_sum
+    per_tri = 9 * t_fft + 6 * t_mask + 12 * t_prod + 7 * t_sum
     return {
         "value": 1.0 / per_tri,
         "unit": "triangles/s",
@@ -109,8 +110,8 @@ def cpu_baseline(N, L):
         "kind": "port",
         "sample": (f"oracle/ps_oracle.c on 1 core of '{cpu_model()}' (nproc={os.cpu_count()}): one {N}^3 FP64 C2C "
                    f"backward FFT {t_fft:.2f} s, mask {t_mask:.2f} s, complex product {t_prod:.2f} s, complex sum "
-                   f"{t_sum:.2f} s, composed as 6 FFT + 3 mask + 9 product + 6 sum = {per_tri:.1f} s per triangle "
-                   f"(polyspectrum.c:51-161)"),
+                   f"{t_sum:.2f} s, composed as 9 FFT + 6 mask + 12 product + 7 sum = {per_tri:.1f} s per triangle "
+                   f"(polyspectrum() + num_polygons(), polyspectrum.c:51-252)"),
     }
 
 
@@ -199,6 +200,64 @@ def adjoint_timing(P, N):
                         "gradient's allocation and device-to-host copy (1 GiB)",
             "device_vs_host_max_rel": dev_vs_host,
             "grad_rms": float(np.sqrt(np.mean(g ** 2)))}
+
+
+def kernel_sources_sha256():
+    """Hash of the kernel sources: a committed counter summary describes the
+    build whose sources it names."""
+    import glob
+    import hashlib
+    h = hashlib.sha256()
+    for f in sorted(glob.glob(os.path.join(ROOT, "polyspectrum_amd", "csrc", "*"))):
+        h.update(os.path.basename(f).encode())
+        h.update(open(f, "rb").read())
+    return h.hexdigest()
+
+
+def rocfft_comparison(P, N, L, field, ours):
+    """torch.fft (hipFFT -> rocFFT) on the same N^3 field, FP32 and FP64,
+    beside the project's forward R2C (ours[prec] = (ms, algorithmic bytes)):
+    rfftn (the forward R2C) and irfftn (C2R), HIP events on torch's stream,
+    the fastest of several calls after a warm-up (planning) call."""
+    import torch
+    print("[bench] rocFFT comparison", file=sys.stderr, flush=True)
+    out = {"library": "torch.fft -> hipFFT -> rocFFT " + str(getattr(torch.version, "hip", "")), "grid": N}
+    for name, dt in (("fp32", torch.float32), ("fp64", torch.float64)):
+        x = torch.from_numpy(field).to(device="cuda", dtype=dt)
+        res = {}
+        for op in ("rfftn", "irfftn"):
+            if op == "rfftn":
+                f = lambda: torch.fft.rfftn(x)
+            else:
+                X = torch.fft.rfftn(x)
+                f = lambda: torch.fft.irfftn(X, s=x.shape)
+            f()
+            torch.cuda.synchronize()
+            best = 1e30
+            for _ in range(5):
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record()
+                y = f()
+                e1.record()
+                e1.synchronize()
+                best = min(best, e0.elapsed_time(e1))
+                del y
+            res[op + "_ms"] = round(best, 4)
+        ms, by = ours[name]
+        # rocFFT's own model: its input is the field in this precision
+        s_in, cs = (4, 8) if name == "fp32" else (8, 16)
+        by_roc = s_in * float(N) ** 3 + 5.0 * cs * (N // 2 + 1) * float(N) ** 2
+        res["rocfft_bytes_model_GB"] = round(by_roc / 1e9, 4)
+        res["rfftn_GB/s"] = round(by_roc / res["rfftn_ms"] / 1e6, 1)
+        res["rfftn_frac"] = round(by_roc / res["rfftn_ms"] / 1e6 / HBM_PEAK_GBS, 4)
+        res["project_bytes_model_GB"] = round(by / 1e9, 4)  # float32 field in, both precisions
+        res["project_forward_ms"] = round(ms, 4)
+        res["project_forward_frac"] = round(by / ms / 1e6 / HBM_PEAK_GBS, 4)
+        res["project_speedup_vs_rocfft"] = round(res["rfftn_ms"] / ms, 3)
+        out[name] = res
+        del x
+        torch.cuda.empty_cache()
+    return out
 
 
 def scale_err(v, ref, group=32):
@@ -290,7 +349,9 @@ def main():
     if not args.no_autotune:
         print("[bench] autotuning the launch shape", file=sys.stderr, flush=True)
         tuned, tuned_ms = ctx.autotune(polys, 1.0, P.BINNING["GIVEN_BINNING"], reps=5)
+    t0 = time.perf_counter()
     batch = ctx.batch(polys, 1.0, P.BINNING["GIVEN_BINNING"])
+    plan_ms = 1e3 * (time.perf_counter() - t0)
     for _ in range(args.warmup):
         batch.execute()
     print(f"[bench] timing {args.steps} steps", file=sys.stderr, flush=True)
@@ -305,14 +366,21 @@ def main():
     used_params = batch.params
     stage_ms, stage_bytes, names = batch.profile(reps=3)
     dom = int(np.argmax(stage_ms))
-    traffic = None
+    traffic, traffic_note = None, "no counter summary for this workload"
     tfile = os.path.join(ROOT, "profiles", "bispectrum512_hbm_traffic.json" if args.method == "direct"
                          else "bispectrum512_fourier_hbm_traffic.json")
     if N == 512 and os.path.exists(tfile):
         # HBM bytes per step of the dominant pass (all its launches) from the
-        # committed FETCH_SIZE/WRITE_SIZE summary
-        t = json.load(open(tfile)).get("pass_traffic_GB", {}).get(names[dom])
-        traffic = None if t is None else round(t * 1e9)
+        # committed FETCH_SIZE/WRITE_SIZE summary, if it was taken of these
+        # kernel sources
+        tj = json.load(open(tfile))
+        if tj.get("kernel_sources_sha256") == kernel_sources_sha256():
+            t = tj.get("pass_traffic_GB", {}).get(names[dom])
+            traffic = None if t is None else round(t * 1e9)
+            traffic_note = ("bytes per step over the pass's launches (rocprofv3 FETCH_SIZE+WRITE_SIZE, copy-kernel "
+                            "scaled; " + os.path.relpath(tfile, ROOT) + ", taken of these kernel sources)")
+        else:
+            traffic_note = "committed counter summary was taken of other kernel sources: not used"
     fwd_ms, fwd_bytes = ctx.forward_profile(reps=3)
     stages = {names[i]: {"ms": round(float(stage_ms[i]), 4), "GB": round(float(stage_bytes[i]) / 1e9, 4),
                          "GB/s": round(float(stage_bytes[i] / stage_ms[i] / 1e6), 1) if stage_ms[i] > 0 else None}
@@ -324,6 +392,7 @@ def main():
     print("[bench] FP64 paths", file=sys.stderr, flush=True)
     c64 = P.Context(N, L, P.PREC_FP64)
     c64.load_field(field)
+    fwd64 = c64.forward_profile(reps=3)
     b64 = c64.batch(polys, 1.0, P.BINNING["GIVEN_BINNING"])
     vals64, npol64, _ = b64.execute()
     b64_ms = b64.last_ms
@@ -338,6 +407,8 @@ def main():
     for p in sample:
         plain.append(c64.polyspectrum(3, p["k"], p["kmin"], p["kmax"], 1.0, P.BINNING["GIVEN_BINNING"])[0])
     plain_s = (time.perf_counter() - t0) / len(sample)
+    # condition number of every triangle's sum (reference-shaped FP64 path)
+    kappa = c64.condition_numbers(polys, 1.0, P.BINNING["GIVEN_BINNING"])
     c64.close()
     idx = np.linspace(0, ntri - 1, args.plain_samples).astype(int)
     plain_vs_b64 = float(np.max(np.abs(np.array(plain) - vals64[idx]) / np.abs(vals64[idx])))
@@ -366,9 +437,7 @@ def main():
                      "achieved": round(float(stage_bytes[dom] / stage_ms[dom] / 1e6), 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(float(stage_bytes[dom] / stage_ms[dom] / 1e6 / HBM_PEAK_GBS), 4),
-                     "traffic": traffic, "traffic_unit": "bytes per step over the pass's launches (rocprofv3 "
-                                                         "FETCH_SIZE+WRITE_SIZE, copy-kernel scaled; "
-                                                         "profiles/bispectrum512_hbm_traffic.md)"},
+                     "traffic": traffic, "traffic_note": traffic_note},
         "step_roofline": {"model_GB": round(float(np.sum(stage_bytes)) / 1e9, 4),
                           "GB/s": round(float(np.sum(stage_bytes)) / (ms_per_step * 1e6), 1),
                           "frac": round(float(np.sum(stage_bytes)) / (ms_per_step * 1e6) / HBM_PEAK_GBS, 4)},
@@ -379,8 +448,21 @@ def main():
         # a step that also transforms the field (a new field per step)
         "with_forward_fft": {"ms_per_step": round(ms_per_step + fwd_ms, 3),
                              "triangles_per_s": round(ntri / ((ms_per_step + fwd_ms) / 1e3), 1)},
+        # planning (ps_batch_create: masks, grids, buffers, the FP64 polygon-count
+        # transforms of the bare masks, graph capture) and a cold evaluation:
+        # plan + forward transform of the field + one step
+        "plan": {"ms": round(plan_ms, 2),
+                 "cold_ms": round(plan_ms + fwd_ms + ms_per_step, 2),
+                 "includes": "leg masks and bands, alias-free grids, arena, FP64 I-part transforms "
+                             "(polygon counts), hipGraph capture and instantiation"},
         "accuracy": {"mixed_vs_fp64_rel_err_max": float(rel.max()), "median": float(np.median(rel)),
                      "fraction_le_1e-5": float(np.mean(rel <= 1e-5)),
+                     # per triangle (order: k1 = k2 = 0.1, 0.4, 0.7, 1.0, theta_i = pi i / 32):
+                     # relative error vs FP64 and the condition number
+                     # kappa = sum|prod D| / |sum prod D| of its sum
+                     "per_triangle_rel_err": [float("%.3e" % r) for r in rel],
+                     "per_triangle_kappa": [float("%.4g" % k) for k in kappa],
+                     "max_rel_err_over_u32_kappa": float(np.max(rel / (2.0 ** -24 * kappa))),
                      "max_err_over_group_scale": scale_err(vals32, vals64),
                      "npolygons_rel_err_max": float(np.max(np.abs(npol32 - npol64) / np.abs(npol64))),
                      "plain_fp64_vs_batched_fp64_max": plain_vs_b64,
@@ -396,6 +478,8 @@ def main():
         "launch_params": {"used": used_params, "autotuned": tuned is not None,
                           "autotune_best_ms": tuned_ms},
     }
+    if not args.skip_extras:
+        out["rocfft"] = rocfft_comparison(P, N, L, field, {"fp32": (fwd_ms, fwd_bytes), "fp64": fwd64})
     if not args.skip_extras and args.method == "direct":
         out["fourier_method"] = fourier_method(P, N, L, field, polys, args.steps, args.warmup, vals64,
                                                not args.no_autotune)
