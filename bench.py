@@ -346,8 +346,10 @@ def main():
     if args.method == "fourier":
         ctx.set_method(P.METHOD_FOURIER)
     tuned, tuned_ms = None, None
+    fwd_tuned = None
     if not args.no_autotune:
-        print("[bench] autotuning the launch shape", file=sys.stderr, flush=True)
+        print("[bench] autotuning the forward transform and the launch shape", file=sys.stderr, flush=True)
+        fwd_tuned = ctx.forward_autotune(reps=5)
         tuned, tuned_ms = ctx.autotune(polys, 1.0, P.BINNING["GIVEN_BINNING"], reps=5)
     t0 = time.perf_counter()
     batch = ctx.batch(polys, 1.0, P.BINNING["GIVEN_BINNING"])
@@ -392,6 +394,8 @@ def main():
     print("[bench] FP64 paths", file=sys.stderr, flush=True)
     c64 = P.Context(N, L, P.PREC_FP64)
     c64.load_field(field)
+    if not args.no_autotune:
+        c64.forward_autotune(reps=3)
     fwd64 = c64.forward_profile(reps=3)
     b64 = c64.batch(polys, 1.0, P.BINNING["GIVEN_BINNING"])
     vals64, npol64, _ = b64.execute()
@@ -442,7 +446,9 @@ def main():
                           "GB/s": round(float(np.sum(stage_bytes)) / (ms_per_step * 1e6), 1),
                           "frac": round(float(np.sum(stage_bytes)) / (ms_per_step * 1e6) / HBM_PEAK_GBS, 4)},
         "stages": stages,
-        "forward_fft": {"ms": round(fwd_ms, 4), "GB": round(fwd_bytes / 1e9, 4),
+        "forward_fft": {"variant": None if fwd_tuned is None else fwd_tuned[0],
+                        "variant_note": "radix plan (variant % 2) and tile width (variant // 2), autotuned",
+                        "ms": round(fwd_ms, 4), "GB": round(fwd_bytes / 1e9, 4),
                         "GB/s": round(fwd_bytes / fwd_ms / 1e6, 1),
                         "frac": round(fwd_bytes / fwd_ms / 1e6 / HBM_PEAK_GBS, 4)},
         # a step that also transforms the field (a new field per step)

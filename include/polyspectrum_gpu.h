@@ -273,6 +273,21 @@ int ps_direct_check(ps_ctx *ctx, ps_polygon *polys, int npoly, double kbinwidth,
 int ps_direct_check_static(ps_ctx *ctx, ps_polygon *polys, int npoly, double kbinwidth, int kbinning_case,
                            double *values, double *npolygons, double *sums);
 
+/* Forward-transform variants: radix plan (variant % 2: 0 = radix-3 stages,
+ * radix-8 stages, then a radix-4/2 stage; 1 = the radix-4/2 stage first and
+ * the radix-3 stages last) and tile width (variant < 2: the default lines per
+ * LDS tile, >= 2: half of them), where the grid has them
+ * (ps_forward_variant_count: 1 below N = 128).  ps_forward_autotune times
+ * every variant (the fastest of reps transforms of the slot's last host
+ * upload, into the scratch spectrum), keeps the fastest for this context,
+ * for contexts of the same N and precision created later in the process and
+ * in the file PS_TUNE_FILE names (if set).  All variants give the same
+ * transform up to rounding.  No counterpart in the reference (fft.c:30 plans
+ * with FFTW_ESTIMATE). */
+int ps_forward_variant_count(const ps_ctx *ctx);
+int ps_set_forward_variant(ps_ctx *ctx, int variant);
+int ps_forward_autotune(ps_ctx *ctx, int slot, int reps, int *best_variant, double *best_ms);
+
 /* Re-run the forward transform of a slot from its field in the upload
  * buffer (timing of the FFT passes alone); PS_ERR_STATE under the same
  * condition as ps_forward_profile. */

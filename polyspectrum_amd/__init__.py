@@ -146,6 +146,9 @@ def load_library():
                                   _dp, _dp, _dp]
     L.ps_direct_check_static.argtypes = [_vp, ctypes.POINTER(Polygon), ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                          _dp, _dp, _dp]
+    L.ps_forward_variant_count.argtypes = [_vp]
+    L.ps_set_forward_variant.argtypes = [_vp, ctypes.c_int]
+    L.ps_forward_autotune.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _ip, _dp]
     L.ps_condition_numbers.argtypes = [_vp, ctypes.POINTER(Polygon), ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                        _dp]
     L.ps_multi_create.argtypes = [ctypes.POINTER(_vp), ctypes.c_int, ctypes.POINTER(_vp)]
@@ -247,6 +250,21 @@ class Context:
         by = ctypes.c_double()
         _check(load_library().ps_forward_profile(self._h, slot, reps, ctypes.byref(ms), ctypes.byref(by)))
         return ms.value, by.value
+
+    def forward_autotune(self, reps=5, slot=0):
+        """Times every forward-transform variant (radix plan, tile width) on
+        the slot's last host upload; keeps and returns (variant, ms)."""
+        v = ctypes.c_int()
+        ms = ctypes.c_double()
+        _check(load_library().ps_forward_autotune(self._h, slot, reps, ctypes.byref(v), ctypes.byref(ms)))
+        return v.value, ms.value
+
+    def set_forward_variant(self, variant):
+        _check(load_library().ps_set_forward_variant(self._h, int(variant)))
+
+    @property
+    def forward_variants(self):
+        return load_library().ps_forward_variant_count(self._h)
 
     def forward_again(self, slot=0):
         _check(load_library().ps_forward_again(self._h, slot))

@@ -70,6 +70,7 @@ struct ps_ctx {
     void *staging = nullptr;             // field upload buffer (N^3 doubles)
     int staging_is_double = 0;
     int staging_slot = -1;               // slot whose last load went through the upload buffer (-1: none)
+    int fwd_variant = 0;                 // radix plan / tile width of the forward transform (ps_forward_autotune)
     void *fscratch = nullptr;            // half spectrum the forward profiler transforms into
     int phase_only = 0;
     int method = PS_METHOD_DIRECT;       // evaluation of batches created without explicit parameters
@@ -142,6 +143,9 @@ extern "C" int ps_ctx_destroy(ps_ctx *c)
     return PS_OK;
 }
 
+static void tune_load();
+static int forward_variant_tuned(const ps_ctx *c);
+
 extern "C" int ps_ctx_create(int nbins, double boxsize, int precision, int nslots, ps_ctx **out)
 {
     if (!out || nslots < 1 || !(boxsize > 0.0)) return PS_ERR_INVALID;
@@ -164,6 +168,7 @@ extern "C" int ps_ctx_create(int nbins, double boxsize, int precision, int nslot
         return PS_ERR_HIP;
     }
     c->cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+    c->fwd_variant = forward_variant_tuned(c);
     st = c->fp64 ? make_twiddles<double>(nbins, &c->tw) : make_twiddles<float>(nbins, &c->tw);
     if (st != PS_OK) {
         ps_ctx_destroy(c);
@@ -182,7 +187,7 @@ static int ensure_spec(ps_ctx *c, int slot)
 static int forward(ps_ctx *c, int slot, const void *dfield, int is_double)
 {
     PSCK(ensure_spec(c, slot));
-    PSCK(run_forward(c->fp64, c->N, dfield, is_double, c->spec[slot], c->tw, c->st));
+    PSCK(run_forward(c->fp64, c->N, dfield, is_double, c->spec[slot], c->tw, c->st, c->fwd_variant));
     return PS_OK;
 }
 
@@ -774,7 +779,15 @@ static int batch_record(ps_batch *b) { return batch_record_on(b, b->c->st); }
 // Best launch parameters found by ps_batch_autotune, per batch shape; kept for
 // the process and, when PS_TUNE_FILE names a file, appended to / read from it.
 static std::map<std::string, ps_launch_params> g_tuned;
+static std::map<std::string, int> g_fwd_tuned;  // forward-transform variant per (N, precision, CUs)
 static bool g_tune_loaded = false;
+
+static std::string fwd_key(const ps_ctx *c)
+{
+    char buf[64];
+    snprintf(buf, sizeof(buf), "fwd_N%d_fp%d_cu%d", c->N, c->fp64, c->cus);
+    return buf;
+}
 
 static std::string tune_key(const ps_ctx *c, const ps_polygon *polys, int npoly, double bw, int kcase)
 {
@@ -801,9 +814,62 @@ static void tune_load()
     char key[256];
     ps_launch_params lp;
     while (fscanf(fp, "%255s %d %d %d %d %d", key, &lp.grid_cap, &lp.chunk_tiles, &lp.g2_div, &lp.overlap,
-                  &lp.method) == 6)
-        g_tuned[key] = lp;
+                  &lp.method) == 6) {
+        if (strncmp(key, "fwd_", 4) == 0) g_fwd_tuned[key] = lp.grid_cap;  // a forward variant
+        else g_tuned[key] = lp;
+    }
     fclose(fp);
+}
+
+static int forward_variant_tuned(const ps_ctx *c)
+{
+    tune_load();
+    auto it = g_fwd_tuned.find(fwd_key(c));
+    const int v = (it == g_fwd_tuned.end()) ? 0 : it->second;
+    return (v >= 0 && v < forward_variants(c->N, c->fp64)) ? v : 0;
+}
+
+extern "C" int ps_set_forward_variant(ps_ctx *c, int variant)
+{
+    if (!c || variant < 0 || variant >= forward_variants(c->N, c->fp64)) return PS_ERR_INVALID;
+    c->fwd_variant = variant;
+    return PS_OK;
+}
+
+extern "C" int ps_forward_variant_count(const ps_ctx *c) { return c ? forward_variants(c->N, c->fp64) : 0; }
+
+extern "C" int ps_forward_autotune(ps_ctx *c, int slot, int reps, int *best_variant, double *best_ms)
+{
+    if (!c || reps < 1 || slot < 0 || slot >= c->nslots) return PS_ERR_INVALID;
+    const int nv = forward_variants(c->N, c->fp64);
+    const int keep = c->fwd_variant;
+    int best = keep;
+    double bms = 1e300;
+    for (int v = 0; v < nv; ++v) {
+        c->fwd_variant = v;
+        double ms = 0.0;
+        ps_forward_profile(c, slot, 1, &ms, nullptr);  // warm-up
+        const int s = ps_forward_profile(c, slot, reps, &ms, nullptr);
+        if (s != PS_OK) {
+            c->fwd_variant = keep;
+            return s;
+        }
+        if (ms < bms) {
+            bms = ms;
+            best = v;
+        }
+    }
+    c->fwd_variant = best;
+    g_fwd_tuned[fwd_key(c)] = best;
+    if (const char *f = getenv("PS_TUNE_FILE")) {
+        if (FILE *fp = fopen(f, "a")) {
+            fprintf(fp, "%s %d 0 0 0 0\n", fwd_key(c).c_str(), best);
+            fclose(fp);
+        }
+    }
+    if (best_variant) *best_variant = best;
+    if (best_ms) *best_ms = bms;
+    return PS_OK;
 }
 
 static ps_launch_params default_params(const ps_ctx *c)
@@ -1828,7 +1894,7 @@ extern "C" int ps_batch_adjoint(ps_batch *b, int slot, const double *weights, vo
         };
         auto project = [&](const double *f, void *H, const MaskParams *m, const double *w, int nm) -> int {
             // H = mask * FFT(f)
-            int r = run_forward(c->fp64, N, f, 1, H, c->tw, c->st);
+            int r = run_forward(c->fp64, N, f, 1, H, c->tw, c->st, c->fwd_variant);
             if (r != PS_OK) return r;
             return run_apply_masks(c->fp64, N, H, m, w, nm, c->st);
         };
@@ -2102,7 +2168,7 @@ extern "C" int ps_forward_profile(ps_ctx *c, int slot, int reps, double *ms, dou
     (void)hipEventRecord(e0, c->st);
     int st = PS_OK;
     for (int r = 0; r < reps && st == PS_OK; ++r)
-        st = run_forward(c->fp64, c->N, c->staging, c->staging_is_double, c->fscratch, c->tw, c->st);
+        st = run_forward(c->fp64, c->N, c->staging, c->staging_is_double, c->fscratch, c->tw, c->st, c->fwd_variant);
     (void)hipEventRecord(e1, c->st);
     (void)hipEventSynchronize(e1);
     float t = 0.f;

@@ -131,6 +131,32 @@ constexpr int stage_span(int N, int s)
     for (int i = 0; i < s; ++i) ns *= stage_radix(N, i);
     return ns;
 }
+// Alternative plans (chosen per context by ps_forward_autotune for the
+// forward transform): PLAN 0 is the order above; PLAN 1 runs the radix-4/2
+// stage first, then the radix-8 stages, then the radix-3 stages.  Both have
+// num_stages(N) stages.
+template <int PLAN> constexpr int stage_radix_p(int N, int s)
+{
+    if constexpr (PLAN == 0) {
+        return stage_radix(N, s);
+    } else {
+        const int n3 = pow3_count(N), a = log2_of(pow2_part(N));
+        const int nrem = (a % 3) ? 1 : 0;
+        if (s < nrem) return (a % 3 == 2) ? 4 : 2;
+        s -= nrem;
+        if (s < a / 3) return 8;
+        (void)n3;
+        return 3;
+    }
+}
+template <int PLAN> constexpr int stage_span_p(int N, int s)
+{
+    int ns = 1;
+    for (int i = 0; i < s; ++i) ns *= stage_radix_p<PLAN>(N, i);
+    return ns;
+}
+constexpr int kFwdPlans = 2;
+
 constexpr bool supported_n(int N)
 {
     return N >= 4 && pow2_part(N) >= 2 && (pow2_part(N) & (pow2_part(N) - 1)) == 0;
@@ -214,12 +240,12 @@ __device__ __forceinline__ void stage_twiddles(cplx<T> *stw, const cplx<T> *__re
     for (int i = tid; i < N; i += NTH) stw[i] = tw[i];
 }
 
-template <typename T, int N, int TLC, int S, int ST = 0>
+template <typename T, int N, int TLC, int S, int ST = 0, int PLAN = 0>
 __device__ __forceinline__ void fft_lds(cplx<T> *lds, const cplx<T> *__restrict__ tw, int tid)
 {
     if constexpr (ST < num_stages(N)) {
-        stage<T, N, TLC, S, stage_radix(N, ST), stage_span(N, ST)>(lds, tw, tid);
-        fft_lds<T, N, TLC, S, ST + 1>(lds, tw, tid);
+        stage<T, N, TLC, S, stage_radix_p<PLAN>(N, ST), stage_span_p<PLAN>(N, ST)>(lds, tw, tid);
+        fft_lds<T, N, TLC, S, ST + 1, PLAN>(lds, tw, tid);
     }
 }
 

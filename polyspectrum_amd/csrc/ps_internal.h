@@ -109,7 +109,10 @@ int g1_dmode_lines(int N, int fp64);            // kx lines of a D-only G1 tile
 int fused_tile_rows(int N, int fp64);
 int g_tile_lines(int N, int fp64);
 
-int run_forward(int fp64, int N, const void *field, int field_is_double, void *H, const void *tw, hipStream_t st);
+// variant: radix plan and tile width of the forward transform (0 .. forward_variants(N, fp64) - 1)
+int run_forward(int fp64, int N, const void *field, int field_is_double, void *H, const void *tw, hipStream_t st,
+                int variant = 0);
+int forward_variants(int N, int fp64);
 // G1 over tiles [g1a, g1b), G2 over tiles [g2a, g2b) of the leg table;
 // dmode: G1 tiles of 2 TLC D lines and no I lines (the leg table's tile
 // counts must match: D-part G2 tiles only)

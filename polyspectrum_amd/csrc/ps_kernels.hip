@@ -37,11 +37,11 @@ template <typename T> __device__ __forceinline__ T ld_real(const double *p, int6
 // ================================================================ forward
 // Z pass: R2C along z.  Two real rows share one complex FFT (z = a + i b);
 // A[k] = (Z[k] + conj Z[N-k]) / 2, B[k] = (Z[k] - conj Z[N-k]) / (2i).
-template <typename T, typename TI, int N>
+template <typename T, typename TI, int N, int TL = tile_lines<T, N>(), int PLAN = 0>
 __global__ void __launch_bounds__(NT) k_fwd_z(const TI *__restrict__ in, cplx<T> *__restrict__ H,
                                               const cplx<T> *__restrict__ tw)
 {
-    constexpr int TL = tile_lines<T, N>();  // real rows per tile
+    // TL: real rows per tile
     constexpr int TLC = TL / 2;
     __shared__ cplx<T> lds[N * TLC];
     __shared__ cplx<T> stw[N];
@@ -56,7 +56,7 @@ __global__ void __launch_bounds__(NT) k_fwd_z(const TI *__restrict__ in, cplx<T>
         p[row & 1] = v;
     });
     __syncthreads();
-    fft_lds<T, N, TLC, -1>(lds, stw, tid);
+    fft_lds<T, N, TLC, -1, 0, PLAN>(lds, stw, tid);
     constexpr int NH = N / 2 + 1;
     tile_loop<NH * TLC>(tid, [&](int idx) {
         const int c = idx % TLC, kz = idx / TLC;
@@ -71,10 +71,9 @@ __global__ void __launch_bounds__(NT) k_fwd_z(const TI *__restrict__ in, cplx<T>
 }
 
 // Y pass: C2C along the contiguous axis for nlines consecutive lines.
-template <typename T, int N, int S>
+template <typename T, int N, int S, int TLC = tile_lines<T, N>(), int PLAN = 0>
 __global__ void __launch_bounds__(NT) k_c2c_contig(cplx<T> *__restrict__ data, const cplx<T> *__restrict__ tw)
 {
-    constexpr int TLC = tile_lines<T, N>();
     __shared__ cplx<T> lds[N * TLC];
     __shared__ cplx<T> stw[N];
     const int tid = threadIdx.x;
@@ -82,15 +81,14 @@ __global__ void __launch_bounds__(NT) k_c2c_contig(cplx<T> *__restrict__ data, c
     stage_twiddles<T, N>(stw, tw, tid);
     tile_loop<TLC * N>(tid, [&](int idx) { lds[lidx<TLC>(idx % N, idx / N)] = base[idx]; });
     __syncthreads();
-    fft_lds<T, N, TLC, S>(lds, stw, tid);
+    fft_lds<T, N, TLC, S, 0, PLAN>(lds, stw, tid);
     tile_loop<TLC * N>(tid, [&](int idx) { base[idx] = lds[lidx<TLC>(idx % N, idx / N)]; });
 }
 
 // X pass: C2C along the stride-N axis; block = (ky tile, plane).
-template <typename T, int N, int S>
+template <typename T, int N, int S, int TLC = tile_lines<T, N>(), int PLAN = 0>
 __global__ void __launch_bounds__(NT) k_c2c_strided(cplx<T> *__restrict__ data, const cplx<T> *__restrict__ tw)
 {
-    constexpr int TLC = tile_lines<T, N>();
     __shared__ cplx<T> lds[N * TLC];
     __shared__ cplx<T> stw[N];
     const int tid = threadIdx.x;
@@ -101,7 +99,7 @@ __global__ void __launch_bounds__(NT) k_c2c_strided(cplx<T> *__restrict__ data, 
         lds[lidx<TLC>(e, l)] = base[(int64_t)e * N + l];
     });
     __syncthreads();
-    fft_lds<T, N, TLC, S>(lds, stw, tid);
+    fft_lds<T, N, TLC, S, 0, PLAN>(lds, stw, tid);
     tile_loop<TLC * N>(tid, [&](int idx) {
         const int l = idx % TLC, e = idx / TLC;
         base[(int64_t)e * N + l] = lds[lidx<TLC>(e, l)];
@@ -1009,21 +1007,46 @@ __global__ void __launch_bounds__(NT) k_shell_corner_extremes(MaskParams mp, int
 }
 
 // ================================================================ launchers
-template <typename T, int N>
-static int launch_forward_n(const void *field, int field_is_double, void *H, const void *tw, hipStream_t st)
+// Forward R2C with radix plan PLAN and tiles of TL lines (variant v of
+// run_forward: PLAN = v % 2, tiles of the default width (v < 2) or half of
+// it (v >= 2, where a half tile has at least 4 lines))
+template <typename T, int N, int TL, int PLAN>
+static int launch_forward_v(const void *field, int field_is_double, void *H, const void *tw, hipStream_t st)
 {
-    constexpr int TL = tile_lines<T, N>();
-    constexpr int TLC = tile_lines<T, N>();
     const cplx<T> *twp = reinterpret_cast<const cplx<T> *>(tw);
     cplx<T> *h = reinterpret_cast<cplx<T> *>(H);
     if (field_is_double)
-        hipLaunchKernelGGL((k_fwd_z<T, double, N>), dim3(N / TL, N), dim3(NT), 0, st, (const double *)field, h, twp);
+        hipLaunchKernelGGL((k_fwd_z<T, double, N, TL, PLAN>), dim3(N / TL, N), dim3(NT), 0, st, (const double *)field,
+                           h, twp);
     else
-        hipLaunchKernelGGL((k_fwd_z<T, float, N>), dim3(N / TL, N), dim3(NT), 0, st, (const float *)field, h, twp);
+        hipLaunchKernelGGL((k_fwd_z<T, float, N, TL, PLAN>), dim3(N / TL, N), dim3(NT), 0, st, (const float *)field,
+                           h, twp);
     const int64_t nlines = (int64_t)(N / 2 + 1) * N;  // (kz, kx) lines along ky
-    hipLaunchKernelGGL((k_c2c_contig<T, N, -1>), dim3((unsigned)(nlines / TLC)), dim3(NT), 0, st, h, twp);
-    hipLaunchKernelGGL((k_c2c_strided<T, N, -1>), dim3(N / TLC, N / 2 + 1), dim3(NT), 0, st, h, twp);
+    hipLaunchKernelGGL((k_c2c_contig<T, N, -1, TL, PLAN>), dim3((unsigned)(nlines / TL)), dim3(NT), 0, st, h, twp);
+    hipLaunchKernelGGL((k_c2c_strided<T, N, -1, TL, PLAN>), dim3(N / TL, N / 2 + 1), dim3(NT), 0, st, h, twp);
     return (int)hipGetLastError();
+}
+
+template <typename T, int N> constexpr bool fwd_half_tiles() { return N >= 128 && tile_lines<T, N>() / 2 >= 4; }
+
+template <typename T, int N>
+static int launch_forward_n(const void *field, int field_is_double, void *H, const void *tw, hipStream_t st,
+                            int variant)
+{
+    constexpr int TL = tile_lines<T, N>();
+    if constexpr (N >= 128) {
+        switch (variant) {
+        case 1: return launch_forward_v<T, N, TL, 1>(field, field_is_double, H, tw, st);
+        case 2:
+            if constexpr (fwd_half_tiles<T, N>()) return launch_forward_v<T, N, TL / 2, 0>(field, field_is_double, H, tw, st);
+            break;
+        case 3:
+            if constexpr (fwd_half_tiles<T, N>()) return launch_forward_v<T, N, TL / 2, 1>(field, field_is_double, H, tw, st);
+            break;
+        default: break;
+        }
+    }
+    return launch_forward_v<T, N, TL, 0>(field, field_is_double, H, tw, st);
 }
 
 template <typename T, int N>
@@ -1108,10 +1131,10 @@ static int launch_zc_fused_n(const GroupDev *groups, int ngroups, const LegDev *
 #define PS_FOR_EACH_N(X) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(144) X(192) X(216) X(256) X(384) X(512)
 
 template <typename T>
-static int dispatch_forward(int N, const void *field, int fd, void *H, const void *tw, hipStream_t st)
+static int dispatch_forward(int N, const void *field, int fd, void *H, const void *tw, hipStream_t st, int variant)
 {
     switch (N) {
-#define CASE(n) case n: return launch_forward_n<T, n>(field, fd, H, tw, st);
+#define CASE(n) case n: return launch_forward_n<T, n>(field, fd, H, tw, st, variant);
         PS_FOR_EACH_N(CASE)
 #undef CASE
     }
@@ -1201,10 +1224,23 @@ int fused_tile_rows(int N, int fp64)
 }
 int g_tile_lines(int N, int fp64) { return tile_rows(N, fp64) / 2; }
 
-int run_forward(int fp64, int N, const void *field, int field_is_double, void *H, const void *tw, hipStream_t st)
+int run_forward(int fp64, int N, const void *field, int field_is_double, void *H, const void *tw, hipStream_t st,
+                int variant)
 {
-    return fp64 ? dispatch_forward<double>(N, field, field_is_double, H, tw, st)
-                : dispatch_forward<float>(N, field, field_is_double, H, tw, st);
+    return fp64 ? dispatch_forward<double>(N, field, field_is_double, H, tw, st, variant)
+                : dispatch_forward<float>(N, field, field_is_double, H, tw, st, variant);
+}
+
+// variants of run_forward an N^3 context has: plan 0/1 with default tiles,
+// and with half tiles where they exist
+int forward_variants(int N, int fp64)
+{
+    switch (N) {
+#define CASE(n) case n: return (n < 128) ? 1 : ((fp64 ? fwd_half_tiles<double, n>() : fwd_half_tiles<float, n>()) ? 4 : 2);
+        PS_FOR_EACH_N(CASE)
+#undef CASE
+    }
+    return 1;
 }
 
 int run_inverse_g(int fp64, int N, const LegDev *legs, int nlegs, int64_t g1a, int64_t g1b, int64_t g2a, int64_t g2b,

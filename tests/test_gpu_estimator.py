@@ -233,3 +233,30 @@ def test_condition_numbers(gpu_lib):
         print("kappa", kp, ref)
         assert abs(kp - ref) <= 1e-9 * ref and kp >= 1.0
     assert abs(kap[3] - 1.0) < 1e-12  # a power spectrum sums squares
+
+
+@pytest.mark.parametrize("N", [128, 144, 192, 384, 512])
+@pytest.mark.parametrize("prec", [1, 0])
+def test_forward_variants_and_autotune(gpu_lib, N, prec):
+    """Every forward-transform variant (radix plan, tile width) gives
+    pocketfft's rfftn to the same bars; the autotuner keeps one of them and
+    the context transforms later uploads with it."""
+    P = gpu_lib
+    f = _field(N, 9 + N)
+    ref = _rfftn(f.astype(np.float64)).transpose(2, 0, 1)
+    scale = np.abs(ref).max()
+    tol = 1e-13 if prec == 1 else 2e-6
+    with P.Context(N, 100.0, prec) as ctx:
+        nv = ctx.forward_variants
+        assert nv == (4 if N >= 128 else 1) or (nv == 2 and N >= 128)
+        for v in range(nv):
+            ctx.set_forward_variant(v)
+            ctx.load_field(f)
+            h = ctx.spectrum()
+            err = max(float(np.abs(h[kz:kz + 16] - ref[kz:kz + 16]).max()) for kz in range(0, N // 2 + 1, 16)) / scale
+            print(N, prec, "variant", v, err)
+            assert err < tol
+        with pytest.raises(P.GPUError):
+            ctx.set_forward_variant(nv)
+        best, ms = ctx.forward_autotune(reps=2)
+        assert 0 <= best < nv and ms > 0
