@@ -73,7 +73,9 @@ def _polygon_array(polygons):
 
 
 def library_path():
-    return os.path.join(_HERE, _LIB_NAME)
+    """The in-tree build, or the file PS_LIBRARY names (another build of the
+    same sources, e.g. for timing two builds on one machine)."""
+    return os.environ.get("PS_LIBRARY") or os.path.join(_HERE, _LIB_NAME)
 
 
 def build():

@@ -22,6 +22,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <cstdlib>
+
 #include "ps_device.hpp"
 #include "ps_fft_core.hpp"
 #include "ps_internal.h"
@@ -136,7 +138,7 @@ k_inv_g1(const LegDev *__restrict__ legs, int nlegs, int64_t t0, int64_t t1, con
         const int j0 = (int)(local % nbt) * lpt;
         const int mz = lattice(kz, N);
         const cplx<T> *spec = reinterpret_cast<const cplx<T> *>(L.spec);
-        const T inv_scale = L.scale ? (T)(1.0 / *L.scale) : (T)1;
+        const T inv_scale = L.scale ? (T)(1.0 / *(const PS_GLOBAL double *)L.scale) : (T)1;
         auto gather = [&](int idx) {
             const int l = idx / N, ky = idx % N;
             const int j = j0 + l;
@@ -149,7 +151,7 @@ k_inv_g1(const LegDev *__restrict__ legs, int nlegs, int64_t t0, int64_t t1, con
                     if (spec) {
                         // the cell's index on the spectrum's grid (ns >= N)
                         const int ns = L.ns;
-                        d = spec[((int64_t)kz * ns + (mx < 0 ? mx + ns : mx)) * ns + (my < 0 ? my + ns : my)];
+                        d = gload<T>(spec, ((int64_t)kz * ns + (mx < 0 ? mx + ns : mx)) * ns + (my < 0 ? my + ns : my));
                         if (L.phase_only) {
                             // z / |z| (utils_fftw.c:78-93), applied to the cells a shell selects
                             const T a = hypot(d.x, d.y);
@@ -172,8 +174,8 @@ k_inv_g1(const LegDev *__restrict__ legs, int nlegs, int64_t t0, int64_t t1, con
             const int j = j0 + l;
             if (j < L.Bx) {
                 const int64_t o = ((int64_t)kz * L.Bx + j) * N + y;
-                if (gd) gd[o] = lds[lidx<TL2>(y, l)];
-                gi[o] = lds[lidx<TL2>(y, l + TLC)];
+                if (gd) gstore<T>(gd, o, lds[lidx<TL2>(y, l)]);
+                gstore<T>(gi, o, lds[lidx<TL2>(y, l + TLC)]);
             }
         };
         tile_loop_u<TLC * N, 4>(tid, put);
@@ -240,7 +242,7 @@ k_inv_g1d(const LegDev *__restrict__ legs, int nlegs, int64_t t0, int64_t t1, co
             if (L.ivlo >= 0 ? (m2 >= L.ivlo && m2 <= L.ivhi) : mask_eval(L.mask, mx, my, mz)) {
                 const cplx<T> *spec = reinterpret_cast<const cplx<T> *>(L.spec);
                 const int ns = L.ns;
-                d = spec[((int64_t)t.kz * ns + (mx < 0 ? mx + ns : mx)) * ns + (my < 0 ? my + ns : my)];
+                d = gload<T>(spec, ((int64_t)t.kz * ns + (mx < 0 ? mx + ns : mx)) * ns + (my < 0 ? my + ns : my));
                 if (L.phase_only) {
                     // z / |z| (utils_fftw.c:78-93), applied to the cells a shell selects
                     const T a = hypot(d.x, d.y);
@@ -260,7 +262,7 @@ k_inv_g1d(const LegDev *__restrict__ legs, int nlegs, int64_t t0, int64_t t1, co
         tile_loop_u<TOT, 4>(tid, [&](int idx) {
             const int l = idx / N, y = idx % N;
             const int j = t.j0 + l;
-            if (j < L.Bx) gd[((int64_t)t.kz * L.Bx + j) * N + y] = lds[lidx<LPD>(y, l)];
+            if (j < L.Bx) gstore<T>(gd, ((int64_t)t.kz * L.Bx + j) * N + y, lds[lidx<LPD>(y, l)]);
         });
     };
     if constexpr (PF) {
@@ -363,13 +365,14 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(band_pa
         const int l = idx % TLC, kx = idx / TLC;
         const int j = k_to_band(kx, t.Kx, t.Bx, N);
         const int mx = lattice(kx, N);
-        return (j >= 0 && t.g && mx * mx <= t.kx2) ? t.g[((int64_t)t.kz * t.Bx + j) * N + t.y0 + l] : mk<T>(0, 0);
+        return (j >= 0 && t.g && mx * mx <= t.kx2) ? gload<T>(t.g, ((int64_t)t.kz * t.Bx + j) * N + t.y0 + l)
+                                                   : mk<T>(0, 0);
     };
     auto put = [&](const Tile &t) {
         if (t.o) {
             tile_loop_u<TOT, 8>(tid, [&](int idx) {
                 const int l = idx % TLC, x = idx / TLC;
-                t.o[((int64_t)t.kz * N + x) * N + t.y0 + l] = lds[lidx<TLC>(x, l)];
+                gstore<T>(t.o, ((int64_t)t.kz * N + x) * N + t.y0 + l, lds[lidx<TLC>(x, l)]);
             });
         }
     };
@@ -432,8 +435,8 @@ __device__ __forceinline__ void load_zline_pair(cplx<T> *lds, const cplx<T> *__r
         cplx<T> X = mk<T>(0, 0), Y = mk<T>(0, 0);
         if (kz <= Kz) {
             const int64_t o = ((int64_t)kz * N + x) * N + y0 + l;
-            if (gd) X = gd[o];
-            if (gi) Y = gi[o];
+            if (gd) X = gload<T>(gd, o);
+            if (gi) Y = gload<T>(gi, o);
         }
         if (kz == 0 || kz == N / 2) {
             lds[lidx<TLC>(kz, l)] = mk<T>(X.x, Y.x);
@@ -617,13 +620,13 @@ __global__ void __launch_bounds__((ZcCfg<T, N, DM>::BT)) __attribute__((amdgpu_w
             const cplx<T> *gd = reinterpret_cast<const cplx<T> *>(L.g2d);
             if constexpr (DM) {
                 const int64_t o = ((int64_t)kz * N + x) * N + y0 + 2 * l;
-                X = gd[o];
-                Y = gd[o + 1];
+                X = gload<T>(gd, o);
+                Y = gload<T>(gd, o + 1);
             } else {
                 const cplx<T> *gi = reinterpret_cast<const cplx<T> *>(L.g2i);
                 const int64_t o = ((int64_t)kz * N + x) * N + y0 + l;
-                X = gd ? gd[o] : mk<T>(0, 0);
-                Y = gi ? gi[o] : mk<T>(0, 0);
+                X = gd ? gload<T>(gd, o) : mk<T>(0, 0);
+                Y = gi ? gload<T>(gi, o) : mk<T>(0, 0);
             }
         };
         auto fetch = [&](int64_t t, int q) {
@@ -777,6 +780,202 @@ __global__ void __launch_bounds__((ZcCfg<T, N, DM>::BT)) __attribute__((amdgpu_w
     }
 }
 
+// Zc, D parts only (FP32), with the planes staged as raw pairs.  Tiles,
+// work items, steps and partial sums are those of k_inv_zc_fused<T, N, true>
+// (the workgroup loads the next step's planes of the tile into registers
+// while its wavefronts transform the current step, then stores them into the
+// other staging buffer).  The staging buffers hold the raw pairs
+// (D(a), D(b)) of rows a = y0 + 2l, b = a + 1, one line l of SL per tile,
+// with consecutive kz adjacent; the Hermitian extension Z[k] = D(a)[k] +
+// i D(b)[k], Z[N-k] = conj D(a)[k] + i conj D(b)[k] (real parts only at k =
+// 0, N/2) is formed as the first butterfly stage reads them.  One 16-byte
+// store per pair, and a line pitch of 16/SL (mod 16) 16-byte words: the
+// stores of a quarter wavefront (16/SL planes of all SL lines) and the reads
+// of the first stage (consecutive kz of one line) fill distinct banks.
+typedef float f32x4 __attribute__((ext_vector_type(4)));  // a pair of FP32 complex values, in registers
+// where it measured faster than k_inv_zc_fused on the 512^3 sweep (192: 260 ->
+// 241 us, 144: 96 -> 91 us, 96: 62 -> 55 us); the compact form of the latter
+// keeps four wavefronts per SIMD at 64, 128 and 256 (128: 59 against 76 us)
+template <typename T, int N> constexpr bool zc_pair_ok()
+{
+    return sizeof(T) == 4 && zc_dmode_ok<T, N>() && !zc_pow2_mid<T, N>();
+}
+template <int NH, int SL> constexpr int zcp_pitch()
+{
+    const int r = (SL >= 16) ? 1 : 16 / SL;
+    int p = NH;
+    while (p % 16 != r % 16) ++p;
+    return p;
+}
+
+template <typename T, int N>
+__global__ void __launch_bounds__((ZcCfg<T, N, true>::BT)) __attribute__((amdgpu_waves_per_eu(zc_waves<T, N, true>())))
+k_zc_pairs(const GroupDev *__restrict__ groups, int ngroups, const LegDev *__restrict__ legs,
+           const int *__restrict__ suffix, int64_t i0, int64_t i1, double2 *__restrict__ partials, int max_chunks,
+           const cplx<T> *__restrict__ twg)
+{
+    using C = ZcCfg<T, N, true>;
+    constexpr int LW = C::LW, NWV = C::NWV, ROWS = C::ROWS, BT = C::BT, SL = C::SL;
+    constexpr int NH = N / 2 + 1;
+    constexpr int SP = zcp_pitch<NH, SL>();   // 16-byte words per staged line
+    constexpr int LP = wave_pitch<N>();
+    constexpr int MPT = (NH * SL + BT - 1) / BT;  // staged pairs per thread
+    using WO = WaveOut<T, N, LW>;
+    static_assert(sizeof(T) == 4, "FP32 pairs of complex values are 16 bytes");
+    __shared__ f32x4 stage[2][SL * SP];
+    __shared__ cplx<T> wbuf[NWV * LW * LP];
+    __shared__ double acc[NWV][MAX_SUFFIX][2];
+    const int tid = threadIdx.x, w = tid / 64, lane = tid & 63;
+    constexpr bool CMP = zc_compact<T, N>();
+    __shared__ cplx<T> stw[CMP ? N : 1];
+    WaveTw<T, N, LW> twr;
+    if constexpr (CMP) stage_twiddles<T, N, BT>(stw, twg, tid);
+    else twr.template load<+1>(twg, lane);
+    cplx<T> *wb = wbuf + w * LW * LP;
+    constexpr int tiles_per_x = N / ROWS;
+    const int64_t ntile = (int64_t)N * tiles_per_x;
+    for (int64_t item = i0 + blockIdx.x; item < i1; item += gridDim.x) {
+        const int gi = find_owner(groups, ngroups, item, &GroupDev::item_begin);
+        const GroupDev &G = groups[gi];
+        const int chunk = (int)(item - G.item_begin);
+        const int64_t t_begin = (int64_t)chunk * G.chunk_tiles;
+        const int64_t t_end = (t_begin + G.chunk_tiles < ntile) ? t_begin + G.chunk_tiles : ntile;
+        const int nleg = G.npre + G.nsuf;
+        const int64_t nsteps = (t_end - t_begin) * nleg;  // (tile, leg) steps of this item
+        for (int i = tid; i < NWV * MAX_SUFFIX * 2; i += BT) (&acc[0][0][0])[i] = 0.0;
+        auto leg_of = [&](int q) -> const LegDev & {
+            return legs[q < G.npre ? (q == 0 ? G.preA : G.preB) : suffix[G.suf_begin + q - G.npre]];
+        };
+        f32x4 pf[MPT];
+        // the tile's rows y0 .. y0 + 2 SL - 1 of plane kz at x, as SL pairs of
+        // complex values (16 bytes each; y0 is even)
+        auto fetch = [&](int64_t t, int q) {
+            const LegDev &L = leg_of(q);
+            const int x = (int)(t / tiles_per_x), y0 = (int)(t % tiles_per_x) * ROWS;
+            const int cnt = (L.Kz + 1) * SL;
+            const PS_GLOBAL f32x4 *gd = (const PS_GLOBAL f32x4 *)L.g2d;
+#pragma unroll
+            for (int m = 0; m < MPT; ++m) {
+                const int idx = tid + BT * m;
+                if (idx < cnt) pf[m] = gd[((int64_t)(idx / SL) * N + x) * (N / 2) + y0 / 2 + idx % SL];
+            }
+        };
+        auto store = [&](int q, int b) {
+            const int cnt = (leg_of(q).Kz + 1) * SL;
+#pragma unroll
+            for (int m = 0; m < MPT; ++m) {
+                const int idx = tid + BT * m;
+                if (idx < cnt) stage[b][(idx % SL) * SP + idx / SL] = pf[m];
+            }
+        };
+        fetch(t_begin, 0);
+        store(0, 0);
+        __syncthreads();
+        int q = 0;
+        int64_t tcur = t_begin;
+        T pd[WO::IT][WO::R], pi[WO::IT][WO::R];
+        for (int64_t s = 0; s < nsteps; ++s) {
+            const int b = (int)(s & 1);
+            const int qn = (q + 1 == nleg) ? 0 : q + 1;
+            const int64_t tn = (qn == 0) ? tcur + 1 : tcur;
+            if (s + 1 < nsteps) fetch(tn, qn);
+            if (q == 0) {
+#pragma unroll
+                for (int i = 0; i < WO::IT; ++i)
+#pragma unroll
+                    for (int r = 0; r < WO::R; ++r) { pd[i][r] = (T)1; pi[i][r] = (T)1; }
+            }
+            const int Kz = leg_of(q).Kz;
+            const f32x4 *sb = stage[b] + (w * LW) * SP;
+            auto src = [&](int l, int e) -> cplx<T> {
+                const bool lo = e <= N / 2;
+                const int k = lo ? e : N - e;
+                if (k > Kz) return mk<T>(0, 0);
+                const f32x4 v = sb[l * SP + k];
+                const bool edge_k = (k == 0) || (2 * k == N);
+                const T xi = edge_k ? (T)0 : v.y, yi = edge_k ? (T)0 : v.w;
+                return lo ? mk<T>(v.x - yi, xi + v.z) : mk<T>(v.x + yi, v.z - xi);
+            };
+            const bool edge = Kz < N / stage_radix(N, 0);
+            auto run = [&](auto &sink) {
+                if constexpr (CMP) {
+                    const LdsTw<T, N, LW> tw{stw};
+                    if (edge) wave_fft<T, N, LW, +1, 0, true, LdsTw<T, N, LW>>(wb, tw, lane, src, sink);
+                    else wave_fft<T, N, LW, +1, 0, false, LdsTw<T, N, LW>>(wb, tw, lane, src, sink);
+                } else {
+                    const RegTw<T, N, LW> tw{twr};
+                    if (edge) wave_fft<T, N, LW, +1, 0, true, RegTw<T, N, LW>>(wb, tw, lane, src, sink);
+                    else wave_fft<T, N, LW, +1, 0, false, RegTw<T, N, LW>>(wb, tw, lane, src, sink);
+                }
+            };
+            if (q < G.npre) {
+                auto sink1 = [&](int i, int r, int, int, cplx<T> v) {
+                    pd[i][r] *= v.x;
+                    pi[i][r] *= v.y;
+                };
+                auto sink2 = [&](int i, int r, int, int, cplx<T> v) {
+                    pd[i][r] *= v.x * v.x;
+                    pi[i][r] *= v.y * v.y;
+                };
+                if (G.pre_sq) run(sink2);
+                else run(sink1);
+            } else {
+                // both parts of a line are D of two rows: one sum
+                double sd = 0.0;
+                auto sink1 = [&](int i, int r, int, int, cplx<T> v) {
+                    sd += (double)pd[i][r] * (double)v.x;
+                    sd += (double)pi[i][r] * (double)v.y;
+                };
+                auto sink2 = [&](int i, int r, int, int, cplx<T> v) {
+                    const double dv = (double)v.x, iv = (double)v.y;
+                    sd += (double)pd[i][r] * (dv * dv);
+                    sd += (double)pi[i][r] * (iv * iv);
+                };
+                if (G.suf_sq) run(sink2);
+                else run(sink1);
+                sd = wave_sum_dpp(sd);
+                if (lane == 0) {
+                    const int o = q - G.npre;
+                    KSum a;
+                    a.s = acc[w][o][0];
+                    a.c = acc[w][o][1];
+                    a.add(sd);
+                    acc[w][o][0] = a.s;
+                    acc[w][o][1] = a.c;
+                }
+            }
+            if (s + 1 < nsteps) store(qn, b ^ 1);
+            __syncthreads();
+            q = qn;
+            tcur = tn;
+        }
+        for (int o = tid; o < G.nsuf; o += BT) {
+            KSum a;
+            a.init();
+            for (int ww = 0; ww < NWV; ++ww) {
+                a.add(acc[ww][o][0]);
+                a.add(acc[ww][o][1]);
+            }
+            // undo the legs' amplitude scales (D parts entered G1 divided by them)
+            double f = 1.0;
+            if (G.npre >= 1) {
+                const double sa = legs[G.preA].scale ? *legs[G.preA].scale : 1.0;
+                f *= G.pre_sq ? sa * sa : sa;
+            }
+            if (G.npre == 2 && legs[G.preB].scale) f *= *legs[G.preB].scale;
+            const LegDev &Ls = legs[suffix[G.suf_begin + o]];
+            const double sc = Ls.scale ? *Ls.scale : 1.0;
+            f *= G.suf_sq ? sc * sc : sc;
+            // the grid ratio (N_spectrum / N)^3 of a group run on a reduced grid
+            f *= G.fscale;
+            const int64_t po = (int64_t)(G.out_begin + o) * max_chunks + chunk;
+            partials[2 * po + 0] = make_double2(a.s * f, a.c * f);
+            partials[2 * po + 1] = make_double2(0.0, 0.0);
+        }
+        __syncthreads();
+    }
+}
+
 // Amplitude scale of each leg: the rms of |spectrum| over (a sample of about
 // 4k) cells of its shell.  Packing D + i I into one complex transform
 // in the last pass gives both parts the rounding noise of the larger one;
@@ -801,7 +1000,7 @@ __global__ void __launch_bounds__(NT) k_leg_scale(const LegDev *__restrict__ leg
             const int mx = lattice(band_to_k(jx, L.Kx, B, N), N), my = lattice(band_to_k(jy, L.Kx, B, N), N);
             if (mask_eval(L.mask, mx, my, lattice(kz, N))) {
                 const int ns = L.ns;
-                const cplx<T> d = spec[((int64_t)kz * ns + (mx < 0 ? mx + ns : mx)) * ns + (my < 0 ? my + ns : my)];
+                const cplx<T> d = gload<T>(spec, ((int64_t)kz * ns + (mx < 0 ? mx + ns : mx)) * ns + (my < 0 ? my + ns : my));
                 pw += (double)d.x * (double)d.x + (double)d.y * (double)d.y;
                 cnt += 1.0;
             }
@@ -1113,11 +1312,30 @@ static int launch_zc_fused_m(const GroupDev *groups, int ngroups, const LegDev *
     return (int)hipGetLastError();
 }
 
+// the raw-pair form of the D-only pass where it exists (PS_ZC_PAIRS=0: k_inv_zc_fused)
+static bool zc_pairs_enabled()
+{
+    static const bool on = [] {
+        const char *e = getenv("PS_ZC_PAIRS");
+        return !(e && e[0] == '0');
+    }();
+    return on;
+}
+
 template <typename T, int N>
 static int launch_zc_fused_n(const GroupDev *groups, int ngroups, const LegDev *legs, const int *suffix,
                              int64_t i0, int64_t i1, double2 *partials, int max_chunks, int grid_cap, int kz_cap,
                              int dmode, const void *tw, hipStream_t st)
 {
+    if constexpr (zc_pair_ok<T, N>()) {
+        if (dmode && zc_pairs_enabled()) {
+            if (i1 <= i0) return 0;
+            const unsigned g = (unsigned)((i1 - i0 < grid_cap) ? i1 - i0 : grid_cap);
+            hipLaunchKernelGGL((k_zc_pairs<T, N>), dim3(g), dim3(ZcCfg<T, N, true>::BT), 0, st, groups, ngroups, legs,
+                               suffix, i0, i1, partials, max_chunks, reinterpret_cast<const cplx<T> *>(tw));
+            return (int)hipGetLastError();
+        }
+    }
     if (dmode) {
         if constexpr (zc_dmode_ok<T, N>())
             return launch_zc_fused_m<T, N, true>(groups, ngroups, legs, suffix, i0, i1, partials, max_chunks,

@@ -80,7 +80,7 @@ __global__ void __launch_bounds__((ZfCfg<T, N>::BT)) k_spec_zf(const SpecGroupDe
             for (int idx = tid; idx < (Kz + 1) * ROWS; idx += BT) {
                 const int row = idx % ROWS, kz = idx / ROWS;
                 const int64_t o = ((int64_t)kz * N + x) * N + y0 + row;
-                const cplx<T> X = gd ? gd[o] : mk<T>(0, 0), Y = gI[o];
+                const cplx<T> X = gd ? gload<T>(gd, o) : mk<T>(0, 0), Y = gload<T>(gI, o);
                 if (kz == 0 || kz == N / 2) {
                     sD[kz * SP + row] = mk<T>(X.x, Y.x);
                 } else {
@@ -184,13 +184,13 @@ __global__ void __launch_bounds__((ZfCfg<T, N>::BT)) k_spec_zf(const SpecGroupDe
             if constexpr (PAIRED) {
                 const cplx<T> *ld = dyn + (row & ~1) * LP, *li = dyn + (row | 1) * LP;
                 const cplx<T> ad = ld[kn], cd = ld[km], ai = li[kn], ci = li[km];
-                od[o] = (row & 1) ? im_part(ad, cd) : re_part(ad, cd);
-                oi[o] = (row & 1) ? im_part(ai, ci) : re_part(ai, ci);
+                gstore<T>(od, o, (row & 1) ? im_part(ad, cd) : re_part(ad, cd));
+                gstore<T>(oi, o, (row & 1) ? im_part(ai, ci) : re_part(ai, ci));
             } else {
                 const cplx<T> *line = dyn + row * LP;
                 const cplx<T> a = line[kn], c = line[km];
-                od[o] = re_part(a, c);
-                oi[o] = im_part(a, c);
+                gstore<T>(od, o, re_part(a, c));
+                gstore<T>(oi, o, im_part(a, c));
             }
         }
     }
@@ -222,7 +222,7 @@ __global__ void __launch_bounds__(NT) k_spec_fx(const SpecGroupDev *__restrict__
         cplx<T> *out = reinterpret_cast<cplx<T> *>(part ? G.bi : G.bd);
         tile_loop<TLC * N>(tid, [&](int idx) {
             const int l = idx % TLC, xx = idx / TLC;
-            lds[lidx<TLC>(xx, l)] = in[((int64_t)kz * N + xx) * N + y0 + l];
+            lds[lidx<TLC>(xx, l)] = gload<T>(in, ((int64_t)kz * N + xx) * N + y0 + l);
         });
         __syncthreads();
         fft_lds<T, N, TLC, -1>(lds, stw, tid);
@@ -230,7 +230,7 @@ __global__ void __launch_bounds__(NT) k_spec_fx(const SpecGroupDev *__restrict__
         for (int idx = tid; idx < TLC * Bq; idx += NT) {
             const int l = idx % TLC, jx = idx / TLC;
             const int kx = band_to_k(jx, G.Kq, Bq, N);
-            out[((int64_t)kz * Bq + jx) * N + y0 + l] = lds[lidx<TLC>(kx, l)];
+            gstore<T>(out, ((int64_t)kz * Bq + jx) * N + y0 + l, lds[lidx<TLC>(kx, l)]);
         }
         __syncthreads();
     }
@@ -306,7 +306,7 @@ __global__ void __launch_bounds__((DotCfg<T, N>::BT)) k_spec_dot(const SpecGroup
                     const int64_t line = wl0 + l;
                     if (line >= l1) return mk<T>(0, 0);
                     const cplx<T> *p = reinterpret_cast<const cplx<T> *>((line & 1) ? G.bi : G.bd) + (line >> 1) * N;
-                    const cplx<T> v = p[e];
+                    const cplx<T> v = gload<T>(p, e);
                     return mk<T>(v.x, -v.y);
                 };
                 auto sink = [&](int i, int r, int, int, cplx<T> v) { qo[i][r] = mk<T>(v.x, -v.y); };
@@ -329,7 +329,7 @@ __global__ void __launch_bounds__((DotCfg<T, N>::BT)) k_spec_dot(const SpecGroup
                 const double wz = (kz == 0 || 2 * kz == N) ? 1.0 : 2.0;
                 cplx<T> V = mk<T>(0, 0);
                 if (part == 0) {
-                    V = spec[((int64_t)kz * ns + (mx < 0 ? mx + ns : mx)) * ns + (my < 0 ? my + ns : my)];
+                    V = gload<T>(spec, ((int64_t)kz * ns + (mx < 0 ? mx + ns : mx)) * ns + (my < 0 ? my + ns : my));
                     if (G.phase_only) {
                         // z / |z| (utils_fftw.c:78-93) on the cells a shell selects
                         const T a = hypot(V.x, V.y);
