@@ -204,14 +204,9 @@ def adjoint_timing(P, N):
 
 def kernel_sources_sha256():
     """Hash of the kernel sources: a committed counter summary describes the
-    build whose sources it names."""
-    import glob
-    import hashlib
-    h = hashlib.sha256()
-    for f in sorted(glob.glob(os.path.join(ROOT, "polyspectrum_amd", "csrc", "*"))):
-        h.update(os.path.basename(f).encode())
-        h.update(open(f, "rb").read())
-    return h.hexdigest()
+    build whose sources it names (polyspectrum_amd.profile_summary writes it)."""
+    from polyspectrum_amd.profile_summary import kernel_sources_sha256 as h
+    return h()
 
 
 def rocfft_comparison(P, N, L, field, ours):
