@@ -45,6 +45,7 @@ sys.path.insert(0, ROOT)
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+U32 = 2.0 ** -24  # FP32 unit roundoff
 METRIC = "bispectrum triangles/sec at 512³ (mixed vs FP64 plain, ≤1e-5 rel err); HBM GB/s"
 
 
@@ -266,8 +267,9 @@ def scale_err(v, ref, group=32):
     return out
 
 
-def fourier_method(P, N, L, field, polys, steps, warmup, vals64, autotune):
-    """The same sweep by the Fourier-space method (Context.set_method)."""
+def fourier_method(P, N, L, field, polys, steps, warmup, vals64, autotune, kappa):
+    """The same sweep by the Fourier-space method (Context.set_method); its
+    errors also against the condition-number bar 2 u32 kappa."""
     print("[bench] Fourier-space method", file=sys.stderr, flush=True)
     ctx = P.Context(N, L, P.PREC_MIXED)
     ctx.load_field(field)
@@ -291,6 +293,9 @@ def fourier_method(P, N, L, field, polys, steps, warmup, vals64, autotune):
                        for i in range(len(names))},
             "accuracy_vs_fp64_direct": {"rel_err_max": float(rel.max()), "median": float(np.median(rel)),
                                         "fraction_le_1e-5": float(np.mean(rel <= 1e-5)),
+                                        "worst_triangle": int(np.argmax(rel)),
+                                        "worst_triangle_kappa": float("%.4g" % kappa[int(np.argmax(rel))]),
+                                        "max_rel_err_over_u32_kappa": float(np.max(rel / (U32 * kappa))),
                                         "max_err_over_group_scale": scale_err(v, vals64)},
             "launch_params": params}
 
@@ -463,7 +468,7 @@ def main():
                      # kappa = sum|prod D| / |sum prod D| of its sum
                      "per_triangle_rel_err": [float("%.3e" % r) for r in rel],
                      "per_triangle_kappa": [float("%.4g" % k) for k in kappa],
-                     "max_rel_err_over_u32_kappa": float(np.max(rel / (2.0 ** -24 * kappa))),
+                     "max_rel_err_over_u32_kappa": float(np.max(rel / (U32 * kappa))),
                      "max_err_over_group_scale": scale_err(vals32, vals64),
                      "npolygons_rel_err_max": float(np.max(np.abs(npol32 - npol64) / np.abs(npol64))),
                      "plain_fp64_vs_batched_fp64_max": plain_vs_b64,
@@ -483,7 +488,7 @@ def main():
         out["rocfft"] = rocfft_comparison(P, N, L, field, {"fp32": (fwd_ms, fwd_bytes), "fp64": fwd64})
     if not args.skip_extras and args.method == "direct":
         out["fourier_method"] = fourier_method(P, N, L, field, polys, args.steps, args.warmup, vals64,
-                                               not args.no_autotune)
+                                               not args.no_autotune, kappa)
     if not args.skip_extras:
         out["realisations"] = realisation_batch(P)
         out["adjoint"] = adjoint_timing(P, N)
