@@ -34,6 +34,16 @@ template <typename T> __device__ __forceinline__ void gstore(cplx<T> *p, int64_t
     ((PS_GLOBAL typename gvec<T>::type *)p)[i] = t;
 }
 
+// The same tables read through the constant address space: uniform loads
+// from it are scalar (s_load, counted on lgkmcnt), so reading a table entry
+// never waits for the vector memory instructions in flight.  Only for tables
+// that no kernel writes.
+#define PS_CONST __attribute__((address_space(4)))
+template <typename X> __device__ __forceinline__ const PS_CONST X *const_tab(const X *p)
+{
+    return (const PS_CONST X *)p;
+}
+
 // band index j in [0, B) <-> grid index k of the band |m| <= K (B = min(2K+1, N))
 __device__ __forceinline__ int band_to_k(int j, int K, int B, int N) { return (B == N) ? j : (j <= K ? j : j + N - B); }
 __device__ __forceinline__ int k_to_band(int k, int K, int B, int N)

@@ -976,6 +976,264 @@ k_zc_pairs(const GroupDev *__restrict__ groups, int ngroups, const LegDev *__res
     }
 }
 
+// Zc, D parts only (FP32), with the planes copied into LDS by the memory
+// system (global_load_lds_dwordx4: no registers hold data in flight) and
+// three staging buffers, so that each step's planes are requested two steps
+// before they are transformed.  Tiles, work items, steps, transforms, sinks
+// and partial sums are those of k_zc_pairs.
+//
+// Step s of a work item runs as: wait for this wavefront's copies of step s
+// (s_waitcnt vmcnt with the number of copy instructions it has issued since),
+// workgroup barrier (every wavefront's copies of step s are in LDS, and every
+// wavefront has finished reading the buffer of step s - 1), issue the copies
+// of step s + 2 into that buffer, transform step s.  The copies are issued in
+// inline assembly, out of the compiler's sight: its own wait insertion would
+// otherwise wait for every copy in flight before each read of the staging
+// LDS.  The kernel waits for them itself, with the count of copy
+// instructions each wavefront issued; any other vector memory instruction
+// the compiler places in between only makes the wait longer.
+//
+// One copy instruction moves 64 lanes x 16 bytes to consecutive LDS slots
+// (M0 + 16 lane), so the LDS order is fixed by which global pair each lane
+// reads.  Slot s of a step holds plane kz = s / SL of staged line l = ((s mod
+// SL) - kz) mod SL: the SL slots of a plane are one 16 SL-byte segment of
+// global memory (whole 128-byte lines), and the first butterfly stage, which
+// reads consecutive kz of one line, finds them in distinct 16-byte bank
+// groups of every 256 bytes.
+template <typename T, int N> constexpr bool zc_dma_ok()
+{
+    return zc_pair_ok<T, N>() && (ZcCfg<T, N, true>::SL % 8 == 0) && N >= 96;
+}
+
+// copy 16 bytes per active lane from g to LDS byte offset lds_base + 16 lane
+// (lds_base uniform over the wavefront); M0 is restored afterwards
+__device__ __forceinline__ void lds_copy16(const PS_GLOBAL void *g, unsigned lds_base)
+{
+    unsigned saved;
+    asm volatile("s_mov_b32 %0, m0\n\t"
+                 "s_mov_b32 m0, %2\n\t"
+                 "s_nop 0\n\t"
+                 "global_load_lds_dwordx4 %1, off\n\t"
+                 "s_mov_b32 m0, %0"
+                 : "=&s"(saved)
+                 : "v"(g), "s"(lds_base)
+                 : "memory");
+}
+
+// wait until at most n vector memory instructions of this wavefront are
+// outstanding (n uniform; counts above 7 wait for 7)
+__device__ __forceinline__ void wait_vm_at_most(int n)
+{
+    switch (n) {
+    case 0: asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); break;
+    case 1: asm volatile("s_waitcnt vmcnt(1)" ::: "memory"); break;
+    case 2: asm volatile("s_waitcnt vmcnt(2)" ::: "memory"); break;
+    case 3: asm volatile("s_waitcnt vmcnt(3)" ::: "memory"); break;
+    case 4: asm volatile("s_waitcnt vmcnt(4)" ::: "memory"); break;
+    case 5: asm volatile("s_waitcnt vmcnt(5)" ::: "memory"); break;
+    case 6: asm volatile("s_waitcnt vmcnt(6)" ::: "memory"); break;
+    default: asm volatile("s_waitcnt vmcnt(7)" ::: "memory"); break;
+    }
+}
+
+template <typename T, int N>
+__global__ void __launch_bounds__((ZcCfg<T, N, true>::BT)) __attribute__((amdgpu_waves_per_eu(zc_waves<T, N, true>())))
+k_zc_dma(const GroupDev *__restrict__ groups, int ngroups, const LegDev *__restrict__ legs,
+         const int *__restrict__ suffix, int64_t i0, int64_t i1, double2 *__restrict__ partials, int max_chunks,
+         const cplx<T> *__restrict__ twg)
+{
+    using C = ZcCfg<T, N, true>;
+    constexpr int LW = C::LW, NWV = C::NWV, ROWS = C::ROWS, BT = C::BT, SL = C::SL;
+    constexpr int NH = N / 2 + 1;
+    constexpr int LP = wave_pitch<N>();
+    constexpr int SLOTS = NH * SL;                    // 16-byte slots of a staging buffer
+    constexpr int MPT = (SLOTS + BT - 1) / BT;        // copy instructions per wavefront and step, at most
+    constexpr int NBUF = 3;
+    using WO = WaveOut<T, N, LW>;
+    static_assert(sizeof(T) == 4, "FP32 pairs of complex values are 16 bytes");
+    static_assert(SL % 8 == 0 && (SL & (SL - 1)) == 0, "slot rotation: whole 128-byte segments per plane, SL a power of two");
+    static_assert(MPT <= 7, "waits count at most 7 copy instructions (one step's)");
+    __shared__ f32x4 stage[NBUF][SLOTS];
+    __shared__ cplx<T> wbuf[NWV * LW * LP];
+    __shared__ double acc[NWV][MAX_SUFFIX][2];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid / 64);
+    WaveTw<T, N, LW> twr;
+    twr.template load<+1>(twg, lane);
+    // the twiddles have arrived before the first copy is issued (an empty
+    // use of each register makes the compiler wait for its load here): no
+    // waits for them are left inside the step loop, where they would also
+    // wait for copies in flight
+#pragma unroll
+    for (int i = 0; i < WaveTw<T, N, LW>::total(); ++i) asm volatile("" ::"v"(twr.w[i].x), "v"(twr.w[i].y));
+    cplx<T> *wb = wbuf + w * LW * LP;
+    constexpr int tiles_per_x = N / ROWS;
+    const int64_t ntile = (int64_t)N * tiles_per_x;
+    const unsigned stage_lds = (unsigned)(uintptr_t)(__attribute__((address_space(3))) void *)&stage[0][0];
+    // the group, leg and suffix tables through scalar loads (a vector load's
+    // wait would also wait for the copies in flight)
+    const PS_CONST GroupDev *cgroups = const_tab(groups);
+    const PS_CONST LegDev *clegs = const_tab(legs);
+    const PS_CONST int *csuffix = const_tab(suffix);
+    for (int64_t item = i0 + blockIdx.x; item < i1; item += gridDim.x) {
+        int gi = 0;
+        {
+            int hi = ngroups - 1;
+            while (gi < hi) {
+                const int mid = (gi + hi + 1) >> 1;
+                if (cgroups[mid].item_begin <= item) gi = mid;
+                else hi = mid - 1;
+            }
+        }
+        const PS_CONST GroupDev &G = cgroups[gi];
+        const int chunk = (int)(item - G.item_begin);
+        const int64_t t_begin = (int64_t)chunk * G.chunk_tiles;
+        const int64_t t_end = (t_begin + G.chunk_tiles < ntile) ? t_begin + G.chunk_tiles : ntile;
+        const int nleg = G.npre + G.nsuf;
+        const int64_t nsteps = (t_end - t_begin) * nleg;  // (tile, leg) steps of this item
+        for (int i = tid; i < NWV * MAX_SUFFIX * 2; i += BT) (&acc[0][0][0])[i] = 0.0;
+        auto leg_of = [&](int q) -> const PS_CONST LegDev & {
+            return clegs[q < G.npre ? (q == 0 ? G.preA : G.preB) : csuffix[G.suf_begin + q - G.npre]];
+        };
+        // issue the copies of step (t, q) into buffer b; returns their count
+        auto issue = [&](int64_t t, int q, int b) -> int {
+            const PS_CONST LegDev &L = leg_of(q);
+            const int x = (int)(t / tiles_per_x), y0 = (int)(t % tiles_per_x) * ROWS;
+            const int cnt = (L.Kz + 1) * SL;
+            const PS_GLOBAL f32x4 *gd = (const PS_GLOBAL f32x4 *)L.g2d;
+            int n = 0;
+#pragma unroll
+            for (int m = 0; m < MPT; ++m) {
+                const int base = BT * m + 64 * w;  // uniform over the wavefront
+                if (base < cnt) {
+                    const int s = base + lane;
+                    if (s < cnt) {
+                        const int kz = s / SL;
+                        const int l = ((s % SL) - kz % SL + SL) % SL;
+                        lds_copy16(gd + ((int64_t)kz * N + x) * (N / 2) + y0 / 2 + l,
+                                   stage_lds + 16u * (unsigned)(b * SLOTS + base));
+                    }
+                    ++n;
+                }
+            }
+            return n;
+        };
+        // steps s, s + 1, s + 2 as (tile, leg) and the copy counts of s + 1, s + 2
+        int q0 = 0;
+        int64_t t0s = t_begin;
+        auto next_of = [&](int64_t t, int q, int64_t &tn, int &qn) {
+            qn = (q + 1 == nleg) ? 0 : q + 1;
+            tn = (qn == 0) ? t + 1 : t;
+        };
+        int64_t t1s, t2s;
+        int q1, q2;
+        next_of(t0s, q0, t1s, q1);
+        next_of(t1s, q1, t2s, q2);
+        int n1 = 0, n2 = 0;
+        issue(t0s, q0, 0);
+        if (1 < nsteps) n1 = issue(t1s, q1, 1);
+        T pd[WO::IT][WO::R], pi[WO::IT][WO::R];
+        for (int64_t s = 0; s < nsteps; ++s) {
+            const int b = (int)(s % NBUF);
+            // this wavefront's copies of step s are in LDS once at most the
+            // n1 instructions issued after them are outstanding
+            wait_vm_at_most(n1);
+            __syncthreads();
+            if (s + 2 < nsteps) n2 = issue(t2s, q2, (int)((s + 2) % NBUF));
+            else n2 = 0;
+            const int q = q0;
+            if (q == 0) {
+#pragma unroll
+                for (int i = 0; i < WO::IT; ++i)
+#pragma unroll
+                    for (int r = 0; r < WO::R; ++r) { pd[i][r] = (T)1; pi[i][r] = (T)1; }
+            }
+            const int Kz = leg_of(q).Kz;
+            const f32x4 *sb = stage[b];
+            auto src = [&](int l, int e) -> cplx<T> {
+                const bool lo = e <= N / 2;
+                const int k = lo ? e : N - e;
+                if (k > Kz) return mk<T>(0, 0);
+                const int line = w * LW + l;
+                const f32x4 v = sb[k * SL + ((line + k) & (SL - 1))];
+                const bool edge_k = (k == 0) || (2 * k == N);
+                const T xi = edge_k ? (T)0 : v.y, yi = edge_k ? (T)0 : v.w;
+                return lo ? mk<T>(v.x - yi, xi + v.z) : mk<T>(v.x + yi, v.z - xi);
+            };
+            const bool edge = Kz < N / stage_radix(N, 0);
+            auto run = [&](auto &sink) {
+                const RegTw<T, N, LW> tw{twr};
+                if (edge) wave_fft<T, N, LW, +1, 0, true, RegTw<T, N, LW>>(wb, tw, lane, src, sink);
+                else wave_fft<T, N, LW, +1, 0, false, RegTw<T, N, LW>>(wb, tw, lane, src, sink);
+            };
+            if (q < G.npre) {
+                auto sink1 = [&](int i, int r, int, int, cplx<T> v) {
+                    pd[i][r] *= v.x;
+                    pi[i][r] *= v.y;
+                };
+                auto sink2 = [&](int i, int r, int, int, cplx<T> v) {
+                    pd[i][r] *= v.x * v.x;
+                    pi[i][r] *= v.y * v.y;
+                };
+                if (G.pre_sq) run(sink2);
+                else run(sink1);
+            } else {
+                double sd = 0.0;
+                auto sink1 = [&](int i, int r, int, int, cplx<T> v) {
+                    sd += (double)pd[i][r] * (double)v.x;
+                    sd += (double)pi[i][r] * (double)v.y;
+                };
+                auto sink2 = [&](int i, int r, int, int, cplx<T> v) {
+                    const double dv = (double)v.x, iv = (double)v.y;
+                    sd += (double)pd[i][r] * (dv * dv);
+                    sd += (double)pi[i][r] * (iv * iv);
+                };
+                if (G.suf_sq) run(sink2);
+                else run(sink1);
+                sd = wave_sum_dpp(sd);
+                if (lane == 0) {
+                    const int o = q - G.npre;
+                    KSum a;
+                    a.s = acc[w][o][0];
+                    a.c = acc[w][o][1];
+                    a.add(sd);
+                    acc[w][o][0] = a.s;
+                    acc[w][o][1] = a.c;
+                }
+            }
+            t0s = t1s; q0 = q1;
+            t1s = t2s; q1 = q2;
+            next_of(t1s, q1, t2s, q2);
+            n1 = n2;
+        }
+        __syncthreads();
+        for (int o = tid; o < G.nsuf; o += BT) {
+            KSum a;
+            a.init();
+            for (int ww = 0; ww < NWV; ++ww) {
+                a.add(acc[ww][o][0]);
+                a.add(acc[ww][o][1]);
+            }
+            // undo the legs' amplitude scales (D parts entered G1 divided by them)
+            double f = 1.0;
+            if (G.npre >= 1) {
+                const double sa = legs[G.preA].scale ? *legs[G.preA].scale : 1.0;
+                f *= G.pre_sq ? sa * sa : sa;
+            }
+            if (G.npre == 2 && legs[G.preB].scale) f *= *legs[G.preB].scale;
+            const LegDev &Ls = legs[suffix[G.suf_begin + o]];
+            const double sc = Ls.scale ? *Ls.scale : 1.0;
+            f *= G.suf_sq ? sc * sc : sc;
+            // the grid ratio (N_spectrum / N)^3 of a group run on a reduced grid
+            f *= G.fscale;
+            const int64_t po = (int64_t)(G.out_begin + o) * max_chunks + chunk;
+            partials[2 * po + 0] = make_double2(a.s * f, a.c * f);
+            partials[2 * po + 1] = make_double2(0.0, 0.0);
+        }
+        __syncthreads();
+    }
+}
+
 // Amplitude scale of each leg: the rms of |spectrum| over (a sample of about
 // 4k) cells of its shell.  Packing D + i I into one complex transform
 // in the last pass gives both parts the rounding noise of the larger one;
@@ -1322,11 +1580,30 @@ static bool zc_pairs_enabled()
     return on;
 }
 
+// the copy-pipelined form of the raw-pair pass where it exists (PS_ZC_DMA=0: k_zc_pairs)
+static bool zc_dma_enabled()
+{
+    static const bool on = [] {
+        const char *e = getenv("PS_ZC_DMA");
+        return !(e && e[0] == '0');
+    }();
+    return on;
+}
+
 template <typename T, int N>
 static int launch_zc_fused_n(const GroupDev *groups, int ngroups, const LegDev *legs, const int *suffix,
                              int64_t i0, int64_t i1, double2 *partials, int max_chunks, int grid_cap, int kz_cap,
                              int dmode, const void *tw, hipStream_t st)
 {
+    if constexpr (zc_dma_ok<T, N>()) {
+        if (dmode && zc_pairs_enabled() && zc_dma_enabled()) {
+            if (i1 <= i0) return 0;
+            const unsigned g = (unsigned)((i1 - i0 < grid_cap) ? i1 - i0 : grid_cap);
+            hipLaunchKernelGGL((k_zc_dma<T, N>), dim3(g), dim3(ZcCfg<T, N, true>::BT), 0, st, groups, ngroups, legs,
+                               suffix, i0, i1, partials, max_chunks, reinterpret_cast<const cplx<T> *>(tw));
+            return (int)hipGetLastError();
+        }
+    }
     if constexpr (zc_pair_ok<T, N>()) {
         if (dmode && zc_pairs_enabled()) {
             if (i1 <= i0) return 0;
