@@ -582,7 +582,7 @@ __global__ void __launch_bounds__((ZcCfg<T, N, DM>::BT)) __attribute__((amdgpu_w
 {
     using C = ZcCfg<T, N, DM>;
     constexpr int LW = C::LW, NWV = C::NWV, ROWS = C::ROWS, SL = C::SL, SP = C::SP, MPT = C::MPT, BT = C::BT;
-    constexpr int LP = wave_pitch<N>();
+    constexpr int LP = wave_fft_pitch<T, N>();
     using WO = WaveOut<T, N, LW>;
     extern __shared__ unsigned char dyn_lds[];
     __shared__ cplx<T> wbuf[NWV * LW * LP];
@@ -818,7 +818,7 @@ k_zc_pairs(const GroupDev *__restrict__ groups, int ngroups, const LegDev *__res
     constexpr int LW = C::LW, NWV = C::NWV, ROWS = C::ROWS, BT = C::BT, SL = C::SL;
     constexpr int NH = N / 2 + 1;
     constexpr int SP = zcp_pitch<NH, SL>();   // 16-byte words per staged line
-    constexpr int LP = wave_pitch<N>();
+    constexpr int LP = wave_fft_pitch<T, N>();
     constexpr int MPT = (NH * SL + BT - 1) / BT;  // staged pairs per thread
     using WO = WaveOut<T, N, LW>;
     static_assert(sizeof(T) == 4, "FP32 pairs of complex values are 16 bytes");
@@ -1045,7 +1045,7 @@ k_zc_dma(const GroupDev *__restrict__ groups, int ngroups, const LegDev *__restr
     using C = ZcCfg<T, N, true>;
     constexpr int LW = C::LW, NWV = C::NWV, ROWS = C::ROWS, BT = C::BT, SL = C::SL;
     constexpr int NH = N / 2 + 1;
-    constexpr int LP = wave_pitch<N>();
+    constexpr int LP = wave_fft_pitch<T, N>();
     constexpr int SLOTS = NH * SL;                    // 16-byte slots of a staging buffer
     constexpr int MPT = (SLOTS + BT - 1) / BT;        // copy instructions per wavefront and step, at most
     constexpr int NBUF = 3;

@@ -38,7 +38,8 @@ template <typename T, int N> struct ZfCfg {
     static constexpr int BT = 64 * NWV;
     static constexpr int ROWS = NWV * LW;  // y rows of a tile (at one x)
     static constexpr int SP = ROWS + 1;    // staging pitch
-    static constexpr int LP = wave_pitch<N>();
+    static constexpr int LP = wave_pitch<N>();   // the tile's Q lines
+    static constexpr int WP = wave_fft_pitch<T, N>();  // wave_fft's lines
 };
 
 // Tile = (group, x, ROWS adjacent y rows).  Dynamic LDS holds the staged
@@ -50,17 +51,17 @@ __global__ void __launch_bounds__((ZfCfg<T, N>::BT)) k_spec_zf(const SpecGroupDe
                                                                const cplx<T> *__restrict__ twg, int kz_cap)
 {
     using C = ZfCfg<T, N>;
-    constexpr int LW = C::LW, NWV = C::NWV, ROWS = C::ROWS, SP = C::SP, BT = C::BT, LP = C::LP;
+    constexpr int LW = C::LW, NWV = C::NWV, ROWS = C::ROWS, SP = C::SP, BT = C::BT, LP = C::LP, WP = C::WP;
     using WO = WaveOut<T, N, LW>;
     extern __shared__ unsigned char dyn_lds[];
-    __shared__ cplx<T> wbuf[NWV * LW * LP];
+    __shared__ cplx<T> wbuf[NWV * LW * WP];
     cplx<T> *dyn = reinterpret_cast<cplx<T> *>(dyn_lds);
     cplx<T> *sD = dyn, *sI = dyn + kz_cap * SP;
     const int tid = threadIdx.x, w = tid / 64, lane = tid & 63;
     WaveTw<T, N, LW> twr;
     twr.template load<+1>(twg, lane);
     const RegTw<T, N, LW> tw{twr};
-    cplx<T> *wb = wbuf + w * LW * LP;
+    cplx<T> *wb = wbuf + w * LW * WP;
     cplx<T> *qb = dyn + w * LW * LP;  // this wavefront's Q lines
     constexpr int tiles_per_x = N / ROWS;
     for (int64_t tile = t0 + blockIdx.x; tile < t1; tile += gridDim.x) {
@@ -266,7 +267,7 @@ __global__ void __launch_bounds__((DotCfg<T, N>::BT)) k_spec_dot(const SpecGroup
     using C = DotCfg<T, N>;
     constexpr int LW = C::LW, NWV = C::NWV, BT = C::BT, LINES = C::LINES;
     static_assert(MAX_SUFFIX <= 32, "lanes (t, part) of a wavefront hold the radial sums");
-    constexpr int LP = wave_pitch<N>();
+    constexpr int LP = wave_fft_pitch<T, N>();
     using WO = WaveOut<T, N, LW>;
     constexpr int IT = WO::IT, R = WO::R, NB = WO::NB;
     __shared__ cplx<T> wbuf[NWV * LW * LP];

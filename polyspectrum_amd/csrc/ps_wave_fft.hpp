@@ -46,9 +46,64 @@ template <typename T, int N> constexpr int wave_lines()
 
 template <int N> constexpr int wave_pitch() { return N + N / 8 + 1; }
 __device__ __forceinline__ int wpad(int e) { return e + (e >> 3); }
-// wpad(e + s) = wpad(e) + s*9/8 when s is a multiple of 8, and = wpad(e) + s
-// when e is a multiple of 8 and s < 8: the stages below use that to turn the
-// per-r addresses into constant offsets of the LDS instructions
+
+// LDS layout of a wavefront's lines inside wave_fft: element e of line l at
+// l * LP + pad(e), pad(e) = e + (e / Q) * A.  Chosen per precision and length
+// with a model of the stages' accesses (32 banks of 4 bytes; a 64-lane access
+// of 8-byte values served 16 lanes at a time): for FP32 the layouts below
+// take the modelled bank-conflict cycles of the exchanges from 1.5-2x the
+// conflict-free count to 1-1.4x at every line count (e.g. 192: 2.0 -> 1.25);
+// FP64 keeps e + e/8 and pitch N + N/8 + 1.  Each one keeps the per-r
+// offsets of every stage's reads and writes constant (checked below), so
+// they fold into the LDS instructions' offsets.  pitch >= pad(N - 1) + 1.
+struct WLayP {
+    int Q, A, LP;
+};
+template <typename T, int N> constexpr WLayP wlay_params()
+{
+    if constexpr (sizeof(T) == 8) return WLayP{8, 1, N + N / 8 + 1};
+    switch (N) {
+    case 16: return WLayP{4, 1, 20};
+    case 24: return WLayP{6, 1, 35};
+    case 32: return WLayP{4, 2, 49};
+    case 48: return WLayP{48, 1, 54};
+    case 64: return WLayP{8, 1, 72};
+    case 96: return WLayP{48, 1, 108};
+    case 128: return WLayP{16, 1, 136};
+    case 144: return WLayP{144, 1, 146};
+    case 192: return WLayP{48, 2, 200};
+    case 216: return WLayP{216, 1, 216};
+    case 256: return WLayP{16, 1, 272};
+    case 384: return WLayP{48, 2, 400};
+    case 512: return WLayP{16, 1, 544};
+    default: return WLayP{8, 1, N + N / 8 + 1};
+    }
+}
+template <typename T, int N> struct WLay {
+    static constexpr int Q = wlay_params<T, N>().Q, A = wlay_params<T, N>().A, LP = wlay_params<T, N>().LP;
+    __host__ __device__ static constexpr int pad(int e) { return e + (int)((unsigned)e / (unsigned)Q) * A; }
+    static_assert(pad(N - 1) < LP, "the pitch holds a whole line");
+    // pad(j + r NB) = pad(j) + pad(r NB) for every j < NB (reads of a stage)
+    static constexpr bool read_const(int NB, int R)
+    {
+        for (int j = 0; j < NB; ++j)
+            for (int r = 0; r < R; ++r)
+                if (pad(j + r * NB) - pad(j) != pad(r * NB)) return false;
+        return true;
+    }
+    // pad(d + r NS) = pad(d) + pad(r NS) for the write bases d of a stage
+    static constexpr bool write_const(int NB, int NS, int R)
+    {
+        for (int j = 0; j < NB; ++j) {
+            const int d = (j / NS) * NS * R + j % NS;
+            for (int r = 0; r < R; ++r)
+                if (pad(d + r * NS) - pad(d) != pad(r * NS)) return false;
+        }
+        return true;
+    }
+};
+// wavefront buffer of wave_fft: LW lines of this pitch
+template <typename T, int N> constexpr int wave_fft_pitch() { return WLay<T, N>::LP; }
 
 
 template <typename T, int N, int LW, int S> struct WaveStage {
@@ -140,7 +195,8 @@ __device__ __forceinline__ void wave_fft(cplx<T> *__restrict__ wbuf, const Tw &t
 {
     using St = WaveStage<T, N, LW, S>;
     constexpr int R = St::R, NS = St::NS, NB = St::NB, TOT = St::TOT, IT = St::IT;
-    constexpr int LP = wave_pitch<N>();
+    using Lay = WLay<T, N>;
+    constexpr int LP = Lay::LP;
     cplx<T> v[IT][R];
 #pragma unroll
     for (int i = 0; i < IT; ++i) {
@@ -152,10 +208,10 @@ __device__ __forceinline__ void wave_fft(cplx<T> *__restrict__ wbuf, const Tw &t
                 if constexpr (S == 0) {
                     if (EDGE && r != 0 && r != R - 1) v[i][r] = mk<T>(0, 0);
                     else v[i][r] = src(l, j + r * NB);
-                } else if constexpr (NB % 8 == 0) {
-                    v[i][r] = wbuf[l * LP + wpad(j) + r * (NB + NB / 8)];
+                } else if constexpr (Lay::read_const(NB, R)) {
+                    v[i][r] = wbuf[l * LP + Lay::pad(j) + Lay::pad(r * NB)];
                 } else {
-                    v[i][r] = wbuf[l * LP + wpad(j + r * NB)];
+                    v[i][r] = wbuf[l * LP + Lay::pad(j + r * NB)];
                 }
             }
             if constexpr (NS > 1) {
@@ -183,17 +239,13 @@ __device__ __forceinline__ void wave_fft(cplx<T> *__restrict__ wbuf, const Tw &t
             if (TOT % 64 == 0 || b < TOT) {
                 const int j = b % NB, l = b / NB;
                 const int d = (j / NS) * NS * R + j % NS;
-                if constexpr (NS % 8 == 0) {
-                    const int base = l * LP + wpad(d);
+                if constexpr (Lay::write_const(NB, NS, R)) {
+                    const int base = l * LP + Lay::pad(d);
 #pragma unroll
-                    for (int r = 0; r < R; ++r) wbuf[base + r * (NS + NS / 8)] = v[i][r];
-                } else if constexpr (NS == 1 && R == 8) {
-                    const int base = l * LP + wpad(d);  // d = 8 j
-#pragma unroll
-                    for (int r = 0; r < R; ++r) wbuf[base + r] = v[i][r];
+                    for (int r = 0; r < R; ++r) wbuf[base + Lay::pad(r * NS)] = v[i][r];
                 } else {
 #pragma unroll
-                    for (int r = 0; r < R; ++r) wbuf[l * LP + wpad(d + r * NS)] = v[i][r];
+                    for (int r = 0; r < R; ++r) wbuf[l * LP + Lay::pad(d + r * NS)] = v[i][r];
                 }
             }
         }

@@ -43,7 +43,7 @@ PASSES = {
     "direct": [
         ("G1 masked gather + ky inverse", r"k_inv_g1d?<"),
         ("G2 kx inverse (band -> all x)", r"k_inv_g2<"),
-        ("Zc fused C2R + product-reduce", r"k_inv_zc_fused<|k_zc_pairs<"),
+        ("Zc fused C2R + product-reduce", r"k_inv_zc_fused<|k_zc_pairs<|k_zc_dma<"),
         ("final reduce", r"k_reduce_partials"),
     ],
     "fourier": [
