@@ -116,6 +116,13 @@ def cpu_baseline(N, L):
     }
 
 
+def fwd64_refine_ms(P, N, L, field):
+    """Forward transform time of the FP64 spectrum a refining context keeps."""
+    with P.Context(N, L, P.PREC_FP64) as c:
+        c.load_field(field)
+        return c.forward_profile(reps=3)[0]
+
+
 def realisation_batch(P, nper=16, sizes=(128, 192, 256)):
     """Uneven batch of realisations: nper seeded log-normal fields (sigma 0.8) at
     each size, boxes of 1 Mpc/h cells; per field the power spectrum in 32 bins
@@ -313,6 +320,9 @@ def main():
                     help="method of the measured step (direct: the reference's; fourier: opt-in, same sums)")
     ap.add_argument("--skip-extras", action="store_true",
                     help="skip the realisation-batch and adjoint measurements")
+    ap.add_argument("--refine-tol", type=float, default=1e-5,
+                    help="mixed precision: triangles whose error estimate exceeds this are evaluated again in FP64 "
+                         "inside the step (0: off)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy 5 times (scale for the FETCH/WRITE counters)")
     args = ap.parse_args()
@@ -342,6 +352,9 @@ def main():
 
     # ---- mixed-precision batched path (the measured value)
     ctx = P.Context(N, L, P.PREC_MIXED)
+    refine = args.refine_tol > 0 and args.method == "direct"
+    if refine:
+        ctx.set_refinement(args.refine_tol)  # before the load: the context keeps an FP64 spectrum too
     ctx.load_field(field)
     if args.method == "fourier":
         ctx.set_method(P.METHOD_FOURIER)
@@ -363,6 +376,21 @@ def main():
     elapsed = time.perf_counter() - t0
     ms_per_step = 1e3 * elapsed / args.steps
     value = ntri * args.steps / elapsed
+    refined, err_est = batch.refinement()
+    unrefined = None
+    if refine:
+        # the same batch without refinement (and without the magnitude sums)
+        ctx.set_refinement(0.0)
+        for _ in range(args.warmup):
+            batch.execute()
+        nu = max(1, args.steps // 2)
+        t0 = time.perf_counter()
+        for _ in range(nu):
+            vals_u, _, _ = batch.execute()
+        eu = time.perf_counter() - t0
+        unrefined = {"ms_per_step": round(1e3 * eu / nu, 3), "triangles_per_s": round(ntri * nu / eu, 2),
+                     "steps": nu}
+        ctx.set_refinement(args.refine_tol)
 
     # ---- per-pass HIP-event times (dominant pass) and the forward FFT
     used_params = batch.params
@@ -384,6 +412,8 @@ def main():
         else:
             traffic_note = "committed counter summary was taken of other kernel sources: not used"
     fwd_ms, fwd_bytes = ctx.forward_profile(reps=3)
+    # a new field in a refining context also needs its FP64 spectrum
+    fwd_refine_ms = fwd64_refine_ms(P, N, L, field) if refine else 0.0
     stages = {names[i]: {"ms": round(float(stage_ms[i]), 4), "GB": round(float(stage_bytes[i]) / 1e9, 4),
                          "GB/s": round(float(stage_bytes[i] / stage_ms[i] / 1e6), 1) if stage_ms[i] > 0 else None}
               for i in range(len(names))}
@@ -417,6 +447,10 @@ def main():
     idx = np.linspace(0, ntri - 1, args.plain_samples).astype(int)
     plain_vs_b64 = float(np.max(np.abs(np.array(plain) - vals64[idx]) / np.abs(vals64[idx])))
     rel = np.abs(vals32 - vals64) / np.abs(vals64)
+    if unrefined is not None:
+        rel_u = np.abs(vals_u - vals64) / np.abs(vals64)
+        unrefined.update({"rel_err_max": float(rel_u.max()), "fraction_le_1e-5": float(np.mean(rel_u <= 1e-5)),
+                          "note": "the same step without refinement: every triangle from the FP32 path"})
 
     out = {
         "metric": METRIC,
@@ -429,7 +463,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 FFT/products, f64 compensated sums",
+        "dtype": "f32 FFT/products, f64 compensated sums; triangles over the error tolerance re-evaluated in f64",
         "data": "synthetic (seeded log-normal field generated on the host)",
         "config": {"workload": "equilateral-pair bispectrum sweep: 512^3 log-normal DENS field, L=300 Mpc/h, "
                                "k1=k2 in {0.1,0.4,0.7,1.0} h/Mpc x 32 theta, GIVEN_BINNING dcos=0.05, kbinwidth=1",
@@ -452,17 +486,25 @@ def main():
                         "GB/s": round(fwd_bytes / fwd_ms / 1e6, 1),
                         "frac": round(fwd_bytes / fwd_ms / 1e6 / HBM_PEAK_GBS, 4)},
         # a step that also transforms the field (a new field per step)
-        "with_forward_fft": {"ms_per_step": round(ms_per_step + fwd_ms, 3),
-                             "triangles_per_s": round(ntri / ((ms_per_step + fwd_ms) / 1e3), 1)},
+        "with_forward_fft": {"ms_per_step": round(ms_per_step + fwd_ms + fwd_refine_ms, 3),
+                             "triangles_per_s": round(ntri / ((ms_per_step + fwd_ms + fwd_refine_ms) / 1e3), 1),
+                             "includes": "the FP32 forward transform" + (" and the FP64 one the refinement reads "
+                                                                       "(%.3f ms)" % fwd_refine_ms if refine else "")},
         # planning (ps_batch_create: masks, grids, buffers, the FP64 polygon-count
         # transforms of the bare masks, graph capture) and a cold evaluation:
         # plan + forward transform of the field + one step
         "plan": {"ms": round(plan_ms, 2),
-                 "cold_ms": round(plan_ms + fwd_ms + ms_per_step, 2),
+                 "cold_ms": round(plan_ms + fwd_ms + fwd_refine_ms + ms_per_step, 2),
                  "includes": "leg masks and bands, alias-free grids, arena, FP64 I-part transforms "
                              "(polygon counts), hipGraph capture and instantiation"},
         "accuracy": {"mixed_vs_fp64_rel_err_max": float(rel.max()), "median": float(np.median(rel)),
                      "fraction_le_1e-5": float(np.mean(rel <= 1e-5)),
+                     # triangles whose estimated mixed-precision error u32 (16 + 32 kappa / sqrt(#polygons))
+                     # exceeded refine_tol: evaluated again by the FP64 batched path within the step
+                     "refinement": {"tol": args.refine_tol if refine else 0.0,
+                                    "refined_triangles": np.flatnonzero(refined).tolist(),
+                                    "max_rel_err_over_estimate_unrefined":
+                                        float(np.max(rel[~refined] / err_est[~refined])) if refine else None},
                      # per triangle (order: k1 = k2 = 0.1, 0.4, 0.7, 1.0, theta_i = pi i / 32):
                      # relative error vs FP64 and the condition number
                      # kappa = sum|prod D| / |sum prod D| of its sum
@@ -481,6 +523,7 @@ def main():
         "fp64_plain": {"triangles_per_s": round(1.0 / plain_s, 3), "sampled_triangles": len(sample),
                        "speedup_mixed_batched": round(value * plain_s, 1)},
         "fp64_batched": {"triangles_per_s": round(ntri / (b64_ms / 1e3), 1)},
+        "unrefined_step": unrefined,
         "launch_params": {"used": used_params, "autotuned": tuned is not None,
                           "autotune_best_ms": tuned_ms},
     }

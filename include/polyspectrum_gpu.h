@@ -119,6 +119,20 @@ int ps_set_phase_only(ps_ctx *ctx, int phase_only);
  * way, polyspectrum.c:51-161); the choice changes the order of the sums only. */
 int ps_set_method(ps_ctx *ctx, int method);
 
+/* Mixed-precision contexts: refine ill-conditioned values in FP64.  With tol
+ * > 0 the context keeps a second, FP64 spectrum of every field loaded
+ * afterwards (and of the field still in its upload buffer), the batches'
+ * product passes also sum |prod D| (kappa = sum |prod D| / |sum prod D|),
+ * and ps_batch_execute evaluates every polygon whose error estimate
+ * u32 (16 + 32 kappa / sqrt(#polygons)) (u32 = 2^-24) exceeds tol again
+ * with the FP64 batched path (one FP64 batch per set of flagged polygons,
+ * planned on first use, the last 8 sets kept) and returns those values.  tol = 0 switches it off.  PS_ERR_INVALID for FP64
+ * contexts with tol > 0.
+ * Replaces: nothing in the reference (FP64 throughout); the FP32 data of the
+ * mixed path carry each term of a sum to about u32, which near a zero of the
+ * polyspectrum (large kappa) limits the relative accuracy. */
+int ps_set_refinement(ps_ctx *ctx, double tol);
+
 /* One configuration, shaped like the reference: per leg a mask, an inverse
  * FFT of the masked spectrum and of the mask, products into N^3
  * accumulators, then the sums.  result = polyspectrum value, sums (optional,
@@ -186,6 +200,12 @@ int ps_batch_execute(ps_batch *b, double *values, double *npolygons, double *sum
 /* Values of the batch's last execution (by ps_batch_execute or as part of a
  * ps_multi), without launching anything. */
 int ps_batch_results(ps_batch *b, double *values, double *npolygons, double *sums);
+/* Refinement of the last ps_batch_execute: per polygon (optional arrays of
+ * npoly) whether its value came from the FP64 path and the error estimate of
+ * its mixed-precision value (0 without refinement).  Returns the number of
+ * refined polygons (or a negative PS_ERR_*).
+ * Replaces: nothing in the reference. */
+int ps_batch_refinement(const ps_batch *b, int *refined, double *error_estimates);
 
 /* Several batches -- contexts of different grid sizes, each with any number
  * of field slots (realisations) -- captured into one hipGraph on the first

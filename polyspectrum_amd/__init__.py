@@ -115,6 +115,8 @@ def load_library():
     L.ps_load_field.argtypes = [_vp, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int]
     L.ps_set_phase_only.argtypes = [_vp, ctypes.c_int]
     L.ps_set_method.argtypes = [_vp, ctypes.c_int]
+    L.ps_set_refinement.argtypes = [_vp, ctypes.c_double]
+    L.ps_batch_refinement.argtypes = [_vp, _ip, _dp]
     L.ps_polyspectrum.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, ctypes.c_double,
                                   ctypes.c_int, _dp, _dp]
     L.ps_num_polygons.argtypes = [_vp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_double, ctypes.c_int, _dp]
@@ -284,6 +286,13 @@ class Context:
     def method(self):
         return getattr(self, "_method", METHOD_DIRECT)
 
+    def set_refinement(self, tol):
+        """Mixed precision: evaluate polygons whose estimated relative error
+        u32 (16 + 32 kappa / sqrt(#polygons)) exceeds tol again in FP64
+        (tol = 0: off; inf: estimates only).  Fields loaded afterwards (and
+        the one still in the upload buffer) also get an FP64 spectrum."""
+        _check(load_library().ps_set_refinement(self._h, float(tol)))
+
     def polyspectrum(self, n, k, kmin=None, kmax=None, kbinwidth=1.0, kbinning_case=0, slot=0):
         """polyspectrum.c:51-161 on the GPU.  Returns (value, kmin, kmax,
         (sum prod D, sum prod I)); kmin/kmax as updated for DERIVED binning."""
@@ -378,6 +387,15 @@ class Batch:
         _check(load_library().ps_batch_execute(self._h, v.ctypes.data_as(_dp), npol.ctypes.data_as(_dp),
                                                s.ctypes.data_as(_dp)))
         return v, npol, s.reshape(self.n, 2)
+
+    def refinement(self):
+        """(refined flags, error estimates) of the last execute()."""
+        f = np.zeros(self.n, dtype=np.int32)
+        e = np.zeros(self.n)
+        r = load_library().ps_batch_refinement(self._h, f.ctypes.data_as(_ip), e.ctypes.data_as(_dp))
+        if r < 0:
+            _check(r)
+        return f.astype(bool), e
 
     def adjoint(self, weights, slot=0, out=None):
         """dL/d field for L with dL/dvalues = weights (after execute()).  out:

@@ -81,6 +81,10 @@ struct ps_ctx {
     double *dsum = nullptr;              // 2 doubles
     LegDev *dleg = nullptr;
     double2 *dcorner = nullptr;          // corner-extreme partials
+    // mixed precision: polygons whose error estimate exceeds refine_tol are
+    // evaluated again from r64, an FP64 context holding the same fields
+    double refine_tol = 0.0;
+    ps_ctx *r64 = nullptr;
     size_t csize() const { return fp64 ? 16 : 8; }
     int64_t half_elems() const { return (int64_t)(N / 2 + 1) * N * N; }
     int64_t cells() const { return (int64_t)N * N * N; }
@@ -127,6 +131,7 @@ template <typename T> static int make_twiddles(int N, void **out)
 extern "C" int ps_ctx_destroy(ps_ctx *c)
 {
     if (!c) return PS_ERR_INVALID;
+    if (c->r64) ps_ctx_destroy(c->r64);
     (void)hipSetDevice(c->device);
     if (c->st) (void)hipStreamSynchronize(c->st);
     for (void *p : c->spec) if (p) (void)hipFree(p);
@@ -209,6 +214,8 @@ extern "C" int ps_load_field(ps_ctx *c, int slot, const void *field, int field_i
     HIPCK(hipStreamSynchronize(c->st));
     c->loaded[slot] = 1;
     if (!on_device) c->staging_slot = slot;
+    // the FP64 spectrum of the same field for refinement (src is on the device)
+    if (c->r64) PSCK(ps_load_field(c->r64, slot, src, field_is_double, 1));
     return PS_OK;
 }
 
@@ -217,7 +224,12 @@ extern "C" int ps_forward_again(ps_ctx *c, int slot)
     if (!c || slot < 0 || slot >= c->nslots) return PS_ERR_INVALID;
     if (!c->staging || !c->loaded[slot] || c->staging_slot != slot) return PS_ERR_STATE;
     HIPCK(hipSetDevice(c->device));
-    return forward(c, slot, c->staging, c->staging_is_double);
+    PSCK(forward(c, slot, c->staging, c->staging_is_double));
+    if (c->r64) {
+        HIPCK(hipStreamSynchronize(c->st));
+        PSCK(ps_load_field(c->r64, slot, c->staging, c->staging_is_double, 1));
+    }
+    return PS_OK;
 }
 
 extern "C" int ps_get_spectrum(ps_ctx *c, int slot, void *out)
@@ -241,6 +253,23 @@ extern "C" int ps_set_phase_only(ps_ctx *c, int phase_only)
 {
     if (!c) return PS_ERR_INVALID;
     c->phase_only = (phase_only == 1) ? 1 : 0;  // run.c:48: only the value 1 switches it on
+    if (c->r64) c->r64->phase_only = c->phase_only;
+    return PS_OK;
+}
+
+extern "C" int ps_set_refinement(ps_ctx *c, double tol)
+{
+    if (!c || !(tol >= 0.0)) return PS_ERR_INVALID;
+    if (c->fp64) return (tol > 0.0) ? PS_ERR_INVALID : PS_OK;  // FP64 contexts need none
+    HIPCK(hipSetDevice(c->device));
+    if (tol > 0.0 && !c->r64) {
+        PSCK(ps_ctx_create(c->N, c->L, PS_PREC_FP64, c->nslots, &c->r64));
+        c->r64->phase_only = c->phase_only;
+        // the field still in the upload buffer; other slots once they are loaded again
+        if (c->staging && c->staging_slot >= 0 && c->loaded[c->staging_slot])
+            PSCK(ps_load_field(c->r64, c->staging_slot, c->staging, c->staging_is_double, 1));
+    }
+    c->refine_tol = tol;
     return PS_OK;
 }
 
@@ -543,7 +572,8 @@ struct ps_batch {
     int *dcounts = nullptr;
     double *dscale = nullptr;            // per wave-leg amplitude scales
     double2 *dpartials = nullptr;
-    double *dsum = nullptr;              // 2*nout
+    double *dsum = nullptr;              // 3*nout: D sums, I sums, |prod D| sums
+    double *hsum = nullptr;              // pinned host copy of dsum
     std::vector<Wave> waves;
     MaskParams *dmasks = nullptr;        // suffix masks in suffix order (Fourier method)
     int2 *divl = nullptr;                // their m^2 intervals and counts
@@ -561,6 +591,17 @@ struct ps_batch {
     std::vector<int> leg_slot;           // field slot of every leg
     std::vector<double> last_sumI;       // sum prod I of every output (last execute)
     int have_sums = 0;
+    // mixed-precision refinement
+    double *dabs = nullptr;              // sum |prod D| of every output: dsum + 2 nout
+    int abs_rec = 0;                     // the captured sequence takes those sums
+    std::vector<ps_polygon> polys;       // the polygons as planned
+    double bw = 0.0;
+    int kcase = 0;
+    std::map<std::vector<int>, ps_batch *> ref;  // FP64 batches of refined polygon sets (planned on first use)
+    std::vector<std::vector<int>> ref_order;      // their keys, oldest first (at most kRefineSets kept)
+    std::vector<int> last_flagged;                // the set the last results refined
+    std::vector<int> refined;            // per polygon: evaluated again in FP64 by the last results
+    std::vector<double> err_est;         // per polygon: error estimate of the mixed value (last results)
     int nstages = 4;
     double stage_bytes[PS_MAX_STAGES] = {};
 };
@@ -577,6 +618,8 @@ static const char *const kStageFourier[] = {"G1 masked gather + ky inverse (pref
 extern "C" int ps_batch_destroy(ps_batch *b)
 {
     if (!b) return PS_ERR_INVALID;
+    for (auto &kv : b->ref)
+        if (kv.second) ps_batch_destroy(kv.second);
     if (b->c && b->c->st) (void)hipStreamSynchronize(b->c->st);
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
@@ -590,6 +633,7 @@ extern "C" int ps_batch_destroy(ps_batch *b)
         for (void *p : wb) if (p) (void)hipFree(p);
     }
     void *bufs[] = {b->dcounts, b->dpartials, b->dsum, b->dscale, b->dmasks, b->divl, b->arena};
+    if (b->hsum) (void)hipHostFree(b->hsum);
     for (void *p : bufs) if (p) (void)hipFree(p);
     for (void *p : b->adj) if (p) (void)hipFree(p);
     delete b;
@@ -611,8 +655,9 @@ static int record_wave(ps_batch *b, Wave &w, hipStream_t ws)
     if (b->method == PS_METHOD_DIRECT && w.dmode) {
         PSCK(run_inverse_g(c->fp64, M, w.dlegs_d, nl, 0, w.g1_tiles_d, 0, w.g2_tiles_d, b->grid_cap, b->g2_div, 1,
                            w.tw, ws));
+        // with the magnitude sums of the products when the batch is refined
         PSCK(run_zc_fused(c->fp64, M, w.dgroups, (int)w.groups.size(), w.dlegs_d, w.dsuffix, 0, w.items,
-                          b->dpartials, b->max_chunks, b->grid_cap, w.kz_cap, 1, w.tw, ws));
+                          b->dpartials, b->max_chunks, b->grid_cap, w.kz_cap, b->abs_rec ? 3 : 1, w.tw, ws));
         return PS_OK;
     }
     if (nl > 0) {
@@ -768,12 +813,48 @@ static int batch_record_on(ps_batch *b, hipStream_t s)
     } else {
         for (Wave &w : b->waves) PSCK(record_wave(b, w, s));
     }
-    // direct method: the I sums were taken when the batch was planned
-    double *outI = (b->method == PS_METHOD_DIRECT) ? nullptr : b->dsum + b->nout;
+    // direct method: the I sums were taken when the batch was planned (the
+    // second partial of each output is sum |prod D| when refined, else unused)
+    double *outI = (b->method == PS_METHOD_DIRECT) ? (b->abs_rec ? b->dabs : nullptr) : b->dsum + b->nout;
     PSCK(run_reduce_partials(b->dpartials, b->max_chunks, b->dcounts, 0, b->nout, b->dsum, outI, s));
     return PS_OK;
 }
 static int batch_record(ps_batch *b) { return batch_record_on(b, b->c->st); }
+
+// (re)capture the launch sequence into the batch's graph (PS_NO_GRAPH: none,
+// executions record the sequence each time)
+static int batch_capture(ps_batch *b)
+{
+    ps_ctx *c = b->c;
+    if (b->exec) HIPCK(hipGraphExecDestroy(b->exec));
+    if (b->graph) HIPCK(hipGraphDestroy(b->graph));
+    b->exec = nullptr;
+    b->graph = nullptr;
+    if (getenv("PS_NO_GRAPH")) return PS_OK;
+    HIPCK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+    const int st = batch_record(b);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->st, &g);
+    if (st != PS_OK || e != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return st != PS_OK ? st : PS_ERR_HIP;
+    }
+    b->graph = g;
+    HIPCK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    return PS_OK;
+}
+
+// refinement needs the magnitude sums: switch them on (or off) with the
+// context's setting, re-capturing the sequence
+static int batch_sync_refinement(ps_batch *b)
+{
+    ps_ctx *c = b->c;
+    const int want = (c->refine_tol > 0.0 && !c->fp64 && b->method == PS_METHOD_DIRECT) ? 1 : 0;
+    if (want == b->abs_rec) return PS_OK;
+    if (want) b->dabs = b->dsum + 2 * (size_t)b->nout;
+    b->abs_rec = want;
+    return batch_capture(b);
+}
 
 // ---------------------------------------------------------------- launch-shape tuning
 // Best launch parameters found by ps_batch_autotune, per batch shape; kept for
@@ -1616,7 +1697,8 @@ extern "C" int ps_batch_create_ex(ps_ctx *c, ps_polygon *polys, int npoly, doubl
     } while (0)
     BCK(hipMalloc(&b->dcounts, sizeof(int) * counts.size()));
     BCK(hipMalloc(&b->dpartials, sizeof(double2) * 2 * (size_t)b->nout * b->max_chunks));
-    BCK(hipMalloc(&b->dsum, sizeof(double) * 2 * (size_t)b->nout));
+    BCK(hipMalloc(&b->dsum, sizeof(double) * 3 * (size_t)b->nout));  // D sums, I sums, |prod D| sums
+    BCK(hipHostMalloc(&b->hsum, sizeof(double) * 3 * (size_t)b->nout, hipHostMallocDefault));
     BCK(hipMemcpy(b->dcounts, counts.data(), sizeof(int) * counts.size(), hipMemcpyHostToDevice));
     BCK(hipEventCreate(&b->ev0));
     BCK(hipEventCreate(&b->ev1));
@@ -1624,20 +1706,20 @@ extern "C" int ps_batch_create_ex(ps_ctx *c, ps_polygon *polys, int npoly, doubl
         st = batch_i_sums(b);
         if (st != PS_OK) goto fail;
     }
-    // 5. capture the launch sequence once
-    if (!getenv("PS_NO_GRAPH")) {
-        BCK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
-        st = batch_record(b);
-        hipGraph_t g = nullptr;
-        hipError_t e = hipStreamEndCapture(c->st, &g);
-        if (st != PS_OK || e != hipSuccess) {
-            if (g) (void)hipGraphDestroy(g);
-            if (st == PS_OK) st = PS_ERR_HIP;
-            goto fail;
-        }
-        b->graph = g;
-        BCK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    // 5. the polygons as planned (DERIVED ranges filled in), for refinement
+    b->polys.assign(polys, polys + npoly);
+    b->bw = bw;
+    b->kcase = kcase;
+    b->refined.assign(npoly, 0);
+    b->err_est.assign(npoly, 0.0);
+    // 6. capture the launch sequence once (with the magnitude sums when the
+    // context refines mixed-precision values)
+    if (c->refine_tol > 0.0 && !c->fp64 && b->method == PS_METHOD_DIRECT) {
+        b->dabs = b->dsum + 2 * (size_t)b->nout;
+        b->abs_rec = 1;
     }
+    st = batch_capture(b);
+    if (st != PS_OK) goto fail;
 #undef BCK
     *out = b;
     return PS_OK;
@@ -1647,16 +1729,37 @@ fail:
 }
 
 // device sums of the last execution -> normalised values (polyspectrum.c:155, :247)
-static int batch_results(ps_batch *b, double *values, double *npolygons, double *sums)
+// Error estimate of a mixed-precision value: the FP32 data (spectrum,
+// filtered fields) carry each term of S = sum prod D to about u32 = 2^-24 of
+// its size; over the #polygons closed lattice polygons of the sum these
+// errors add up like a random walk, and a floor of a few u32 remains (the
+// rounding of the normalised terms themselves), so
+//   |dS| / |S| <= u32 (C0 + C kappa / sqrt(#polygons)),
+//   kappa = sum |prod D| / |sum prod D|.
+// C0 = 16, C = 32 bound the measured errors of every polygon of the 512^3
+// sweep with a factor 1.4 to spare.
+static constexpr double kRefineC0 = 16.0, kRefineC = 32.0;
+static constexpr size_t kRefineSets = 8;  // FP64 batches kept per batch (refined polygon sets)
+
+// spec: the FP64 batch of the last refined set, launched (on its own
+// context's stream) beside this execution; its values are used when the set
+// is the same again, else the new set is evaluated after the results
+static int batch_results(ps_batch *b, double *values, double *npolygons, double *sums, int refine,
+                         ps_batch *spec = nullptr)
 {
     ps_ctx *c = b->c;
-    std::vector<double> h(2 * (size_t)b->nout);
-    HIPCK(hipMemcpyAsync(h.data(), b->dsum, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->st));
+    // one copy of the D and I sums (and the magnitude sums when refining)
+    const int with_abs = (refine && b->abs_rec) ? 1 : 0;
+    const double *h = b->hsum;
+    HIPCK(hipMemcpyAsync(b->hsum, b->dsum, sizeof(double) * (2 + with_abs) * (size_t)b->nout,
+                         hipMemcpyDeviceToHost, c->st));
     HIPCK(hipStreamSynchronize(c->st));
-    b->last_sumI.assign(h.begin() + b->nout, h.end());
+    b->last_sumI.assign(h + b->nout, h + 2 * b->nout);
     b->have_sums = 1;
+    const double *ha = with_abs ? h + 2 * b->nout : nullptr;
     const double volume = c->L * c->L * c->L;
     const double cells = (double)((int64_t)c->N * c->N * c->N);
+    std::vector<int> flagged;
     for (int p = 0; p < b->npoly; ++p) {
         const int o = b->poly_out[p], n = b->poly_n[p];
         const double sD = h[o], sI = h[b->nout + o];
@@ -1666,6 +1769,53 @@ static int batch_results(ps_batch *b, double *values, double *npolygons, double 
             sums[2 * p] = sD;
             sums[2 * p + 1] = sI;
         }
+        b->refined[p] = 0;
+        b->err_est[p] = 0.0;
+        if (!ha) continue;
+        const double npoly = std::max(1.0, sI / cells);
+        const double kappa = ha[o] / fabs(sD);
+        b->err_est[p] = (sD != 0.0) ? 0x1p-24 * (kRefineC0 + kRefineC * kappa / sqrt(npoly)) : HUGE_VAL;
+        if (!(b->err_est[p] <= c->refine_tol)) flagged.push_back(p);
+    }
+    if (spec && flagged != b->last_flagged) HIPCK(hipStreamSynchronize(spec->c->st));  // not used
+    b->last_flagged = flagged;
+    if (flagged.empty()) return PS_OK;
+    // the flagged polygons again, together, with the FP64 batched path of the
+    // companion context (one batch per flagged set, planned on first use)
+    for (int p : flagged) {
+        const int sl = b->polys[p].slot;
+        if (sl < 0 || sl >= c->nslots || !c->r64 || !c->r64->loaded[sl]) return PS_ERR_STATE;
+    }
+    ps_batch *rb = nullptr;
+    auto it = b->ref.find(flagged);
+    if (it != b->ref.end()) {
+        rb = it->second;
+    } else {
+        std::vector<ps_polygon> q;
+        for (int p : flagged) q.push_back(b->polys[p]);
+        PSCK(ps_batch_create(c->r64, q.data(), (int)q.size(), b->bw, b->kcase, &rb));
+        if (b->ref_order.size() >= kRefineSets) {
+            ps_batch_destroy(b->ref[b->ref_order.front()]);
+            b->ref.erase(b->ref_order.front());
+            b->ref_order.erase(b->ref_order.begin());
+        }
+        b->ref[flagged] = rb;
+        b->ref_order.push_back(flagged);
+    }
+    const size_t nf = flagged.size();
+    std::vector<double> v(nf), np(nf), s2(2 * nf);
+    if (rb == spec) PSCK(batch_results(rb, v.data(), np.data(), s2.data(), 0));  // ran beside this execution
+    else PSCK(ps_batch_execute(rb, v.data(), np.data(), s2.data()));
+    HIPCK(hipSetDevice(c->device));
+    for (size_t i = 0; i < nf; ++i) {
+        const int p = flagged[i];
+        values[p] = v[i];
+        if (npolygons) npolygons[p] = np[i];
+        if (sums) {
+            sums[2 * p] = s2[2 * i];
+            sums[2 * p + 1] = s2[2 * i + 1];
+        }
+        b->refined[p] = 1;
     }
     return PS_OK;
 }
@@ -1675,6 +1825,18 @@ extern "C" int ps_batch_execute(ps_batch *b, double *values, double *npolygons, 
     if (!b || !values) return PS_ERR_INVALID;
     ps_ctx *c = b->c;
     HIPCK(hipSetDevice(c->device));
+    PSCK(batch_sync_refinement(b));
+    // refinement: the FP64 batch of the set the last execution refined runs
+    // beside this one, on the FP64 context's stream (recomputed every time;
+    // used if the set comes out the same)
+    ps_batch *spec = nullptr;
+    if (b->abs_rec && c->r64 && !b->last_flagged.empty()) {
+        auto it = b->ref.find(b->last_flagged);
+        if (it != b->ref.end() && it->second->exec) {
+            spec = it->second;
+            HIPCK(hipGraphLaunch(spec->exec, spec->c->st));
+        }
+    }
     HIPCK(hipEventRecord(b->ev0, c->st));
     if (b->exec) HIPCK(hipGraphLaunch(b->exec, c->st));
     else PSCK(batch_record(b));
@@ -1683,14 +1845,28 @@ extern "C" int ps_batch_execute(ps_batch *b, double *values, double *npolygons, 
     float ms = 0.f;
     HIPCK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
     b->last_ms = ms;
-    return batch_results(b, values, npolygons, sums);
+    return batch_results(b, values, npolygons, sums, 1, spec);
 }
 
 extern "C" int ps_batch_results(ps_batch *b, double *values, double *npolygons, double *sums)
 {
     if (!b || !values) return PS_ERR_INVALID;
     HIPCK(hipSetDevice(b->c->device));
-    return batch_results(b, values, npolygons, sums);
+    // after ps_multi_execute the magnitude sums exist only if the batch took
+    // them when the multi-batch was captured: no refinement then
+    return batch_results(b, values, npolygons, sums, 0);
+}
+
+extern "C" int ps_batch_refinement(const ps_batch *b, int *refined, double *error_estimates)
+{
+    if (!b) return PS_ERR_INVALID;
+    int n = 0;
+    for (int p = 0; p < b->npoly; ++p) {
+        if (refined) refined[p] = b->refined[p];
+        if (error_estimates) error_estimates[p] = b->err_est[p];
+        n += b->refined[p];
+    }
+    return n;
 }
 
 // ---------------------------------------------------------------- multi-batch

@@ -1,7 +1,7 @@
 // ps_fft_core.hpp -- device building blocks of the gfx950 FFT engine.
 //
 // A workgroup transforms TLC complex lines of length N held in LDS.  Element
-// e of line l lives at lidx<TLC>(e, l) = e*TLC + (l ^ (e & (TLC-1))): lines
+// e of line l lives at lidx<TLC, T>(e, l) = e*TLC + (l ^ (e & (TLC-1))): lines
 // are interleaved (line index fastest) and XOR-swizzled, so both the strided
 // global tiles (TLC adjacent columns) and the Stockham butterflies hit
 // distinct LDS banks without padding.
@@ -39,7 +39,21 @@ template <typename T, int S> __device__ __forceinline__ cplx<T> mul_si(cplx<T> a
     return (S > 0) ? mk<T>(-a.y, a.x) : mk<T>(a.y, -a.x);
 }
 
-template <int TLC> __device__ __forceinline__ int lidx(int e, int l) { return e * TLC + (l ^ (e & (TLC - 1))); }
+// the XOR key takes e >> SH: with SH = log2(128 / (TLC * element bytes)),
+// the elements e, e + 1, ... of one line fill distinct bank groups of every
+// 128 bytes (one pass of a 32-bank LDS), which the tiles' global loads and
+// stores (e fastest over a workgroup's threads) need; the stages' accesses
+// (l fastest) stay conflict-free for any key
+template <int TLC, typename T> constexpr int lidx_shift()
+{
+    int sh = 0;
+    while ((TLC * (int)sizeof(cplx<T>) << sh) < 128) ++sh;
+    return sh;
+}
+template <int TLC, typename T> __device__ __forceinline__ int lidx(int e, int l)
+{
+    return e * TLC + (l ^ ((e >> lidx_shift<TLC, T>()) & (TLC - 1)));
+}
 
 // ---------------------------------------------------------------- butterflies
 template <typename T, int S> __device__ __forceinline__ void dft2(cplx<T> *v)
@@ -185,7 +199,7 @@ __device__ __forceinline__ void stage(cplx<T> *lds, const cplx<T> *__restrict__ 
         if (TOT % NT == 0 || b < TOT) {
             const int l = b % TLC, j = b / TLC;
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[it][r] = lds[lidx<TLC>(j + r * NB, l)];
+            for (int r = 0; r < R; ++r) v[it][r] = lds[lidx<TLC, T>(j + r * NB, l)];
             if constexpr (NS > 1) {
                 const int k = j % NS;
 #pragma unroll
@@ -203,7 +217,7 @@ __device__ __forceinline__ void stage(cplx<T> *lds, const cplx<T> *__restrict__ 
             const int k = j % NS;
             const int d = (j / NS) * NS * R + k;
 #pragma unroll
-            for (int r = 0; r < R; ++r) lds[lidx<TLC>(d + r * NS, l)] = v[it][r];
+            for (int r = 0; r < R; ++r) lds[lidx<TLC, T>(d + r * NS, l)] = v[it][r];
         }
     }
     __syncthreads();

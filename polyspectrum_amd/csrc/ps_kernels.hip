@@ -23,6 +23,7 @@
 #include <stdint.h>
 
 #include <cstdlib>
+#include <type_traits>
 
 #include "ps_device.hpp"
 #include "ps_fft_core.hpp"
@@ -54,7 +55,7 @@ __global__ void __launch_bounds__(NT) k_fwd_z(const TI *__restrict__ in, cplx<T>
     tile_loop<TL * N>(tid, [&](int idx) {
         const int row = idx / N, z = idx % N;
         T v = ld_real<T>(in, base + idx);
-        T *p = reinterpret_cast<T *>(&lds[lidx<TLC>(z, row >> 1)]);
+        T *p = reinterpret_cast<T *>(&lds[lidx<TLC, T>(z, row >> 1)]);
         p[row & 1] = v;
     });
     __syncthreads();
@@ -62,8 +63,8 @@ __global__ void __launch_bounds__(NT) k_fwd_z(const TI *__restrict__ in, cplx<T>
     constexpr int NH = N / 2 + 1;
     tile_loop<NH * TLC>(tid, [&](int idx) {
         const int c = idx % TLC, kz = idx / TLC;
-        const cplx<T> z1 = lds[lidx<TLC>(kz, c)];
-        const cplx<T> z2 = lds[lidx<TLC>((N - kz) % N, c)];
+        const cplx<T> z1 = lds[lidx<TLC, T>(kz, c)];
+        const cplx<T> z2 = lds[lidx<TLC, T>((N - kz) % N, c)];
         const cplx<T> A = mk<T>((z1.x + z2.x) * (T)0.5, (z1.y - z2.y) * (T)0.5);
         const cplx<T> B = mk<T>((z1.y + z2.y) * (T)0.5, (z2.x - z1.x) * (T)0.5);
         const int64_t o = ((int64_t)kz * N + x) * N + y0 + 2 * c;
@@ -81,10 +82,10 @@ __global__ void __launch_bounds__(NT) k_c2c_contig(cplx<T> *__restrict__ data, c
     const int tid = threadIdx.x;
     cplx<T> *base = data + (int64_t)blockIdx.x * TLC * N;
     stage_twiddles<T, N>(stw, tw, tid);
-    tile_loop<TLC * N>(tid, [&](int idx) { lds[lidx<TLC>(idx % N, idx / N)] = base[idx]; });
+    tile_loop<TLC * N>(tid, [&](int idx) { lds[lidx<TLC, T>(idx % N, idx / N)] = base[idx]; });
     __syncthreads();
     fft_lds<T, N, TLC, S, 0, PLAN>(lds, stw, tid);
-    tile_loop<TLC * N>(tid, [&](int idx) { base[idx] = lds[lidx<TLC>(idx % N, idx / N)]; });
+    tile_loop<TLC * N>(tid, [&](int idx) { base[idx] = lds[lidx<TLC, T>(idx % N, idx / N)]; });
 }
 
 // X pass: C2C along the stride-N axis; block = (ky tile, plane).
@@ -98,13 +99,13 @@ __global__ void __launch_bounds__(NT) k_c2c_strided(cplx<T> *__restrict__ data, 
     stage_twiddles<T, N>(stw, tw, tid);
     tile_loop<TLC * N>(tid, [&](int idx) {
         const int l = idx % TLC, e = idx / TLC;
-        lds[lidx<TLC>(e, l)] = base[(int64_t)e * N + l];
+        lds[lidx<TLC, T>(e, l)] = base[(int64_t)e * N + l];
     });
     __syncthreads();
     fft_lds<T, N, TLC, S, 0, PLAN>(lds, stw, tid);
     tile_loop<TLC * N>(tid, [&](int idx) {
         const int l = idx % TLC, e = idx / TLC;
-        base[(int64_t)e * N + l] = lds[lidx<TLC>(e, l)];
+        base[(int64_t)e * N + l] = lds[lidx<TLC, T>(e, l)];
     });
 }
 
@@ -161,8 +162,8 @@ k_inv_g1(const LegDev *__restrict__ legs, int nlegs, int64_t t0, int64_t t1, con
                     }
                 }
             }
-            lds[lidx<TL2>(ky, l)] = d;
-            lds[lidx<TL2>(ky, l + TLC)] = u;
+            lds[lidx<TL2, T>(ky, l)] = d;
+            lds[lidx<TL2, T>(ky, l + TLC)] = u;
         };
         tile_loop_u<TLC * N, 4>(tid, gather);
         __syncthreads();
@@ -174,8 +175,8 @@ k_inv_g1(const LegDev *__restrict__ legs, int nlegs, int64_t t0, int64_t t1, con
             const int j = j0 + l;
             if (j < L.Bx) {
                 const int64_t o = ((int64_t)kz * L.Bx + j) * N + y;
-                if (gd) gstore<T>(gd, o, lds[lidx<TL2>(y, l)]);
-                gstore<T>(gi, o, lds[lidx<TL2>(y, l + TLC)]);
+                if (gd) gstore<T>(gd, o, lds[lidx<TL2, T>(y, l)]);
+                gstore<T>(gi, o, lds[lidx<TL2, T>(y, l + TLC)]);
             }
         };
         tile_loop_u<TLC * N, 4>(tid, put);
@@ -262,7 +263,7 @@ k_inv_g1d(const LegDev *__restrict__ legs, int nlegs, int64_t t0, int64_t t1, co
         tile_loop_u<TOT, 4>(tid, [&](int idx) {
             const int l = idx / N, y = idx % N;
             const int j = t.j0 + l;
-            if (j < L.Bx) gstore<T>(gd, ((int64_t)t.kz * L.Bx + j) * N + y, lds[lidx<LPD>(y, l)]);
+            if (j < L.Bx) gstore<T>(gd, ((int64_t)t.kz * L.Bx + j) * N + y, lds[lidx<LPD, T>(y, l)]);
         });
     };
     if constexpr (PF) {
@@ -279,7 +280,7 @@ k_inv_g1d(const LegDev *__restrict__ legs, int nlegs, int64_t t0, int64_t t1, co
 #pragma unroll
             for (int m = 0; m < NL; ++m) {
                 const int idx = tid + m * NT;
-                if (TOT % NT == 0 || idx < TOT) lds[lidx<LPD>(idx % N, idx / N)] = r[m];
+                if (TOT % NT == 0 || idx < TOT) lds[lidx<LPD, T>(idx % N, idx / N)] = r[m];
             }
             __syncthreads();
             const int64_t nt = tile + gridDim.x;
@@ -305,7 +306,7 @@ k_inv_g1d(const LegDev *__restrict__ legs, int nlegs, int64_t t0, int64_t t1, co
         for (int64_t tile = t0 + blockIdx.x; tile < t1; tile += gridDim.x) {
             const Tile cur = decode(tile);
             if (cur.empty) continue;  // uniform over the workgroup
-            tile_loop_u<TOT, 4>(tid, [&](int idx) { lds[lidx<LPD>(idx % N, idx / N)] = gather(cur, idx); });
+            tile_loop_u<TOT, 4>(tid, [&](int idx) { lds[lidx<LPD, T>(idx % N, idx / N)] = gather(cur, idx); });
             __syncthreads();
             fft_lds<T, N, LPD, +1>(lds, stw, tid);
             put(cur);
@@ -372,7 +373,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(band_pa
         if (t.o) {
             tile_loop_u<TOT, 8>(tid, [&](int idx) {
                 const int l = idx % TLC, x = idx / TLC;
-                gstore<T>(t.o, ((int64_t)t.kz * N + x) * N + t.y0 + l, lds[lidx<TLC>(x, l)]);
+                gstore<T>(t.o, ((int64_t)t.kz * N + x) * N + t.y0 + l, lds[lidx<TLC, T>(x, l)]);
             });
         }
     };
@@ -390,7 +391,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(band_pa
 #pragma unroll
             for (int m = 0; m < NL; ++m) {
                 const int idx = tid + m * NT;
-                if (TOT % NT == 0 || idx < TOT) lds[lidx<TLC>(idx / TLC, idx % TLC)] = r[m];
+                if (TOT % NT == 0 || idx < TOT) lds[lidx<TLC, T>(idx / TLC, idx % TLC)] = r[m];
             }
             __syncthreads();
             const int64_t nt = tile + gridDim.x;
@@ -413,7 +414,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(band_pa
     } else {
         for (int64_t tile = t0 + blockIdx.x; tile < t1; tile += gridDim.x) {
             const Tile cur = decode(tile);
-            tile_loop_u<TOT, 8>(tid, [&](int idx) { lds[lidx<TLC>(idx / TLC, idx % TLC)] = band(cur, idx); });
+            tile_loop_u<TOT, 8>(tid, [&](int idx) { lds[lidx<TLC, T>(idx / TLC, idx % TLC)] = band(cur, idx); });
             __syncthreads();
             fft_lds<T, N, TLC, +1>(lds, stw, tid);
             put(cur);
@@ -439,10 +440,10 @@ __device__ __forceinline__ void load_zline_pair(cplx<T> *lds, const cplx<T> *__r
             if (gi) Y = gload<T>(gi, o);
         }
         if (kz == 0 || kz == N / 2) {
-            lds[lidx<TLC>(kz, l)] = mk<T>(X.x, Y.x);
+            lds[lidx<TLC, T>(kz, l)] = mk<T>(X.x, Y.x);
         } else {
-            lds[lidx<TLC>(kz, l)] = mk<T>(X.x - Y.y, X.y + Y.x);
-            lds[lidx<TLC>(N - kz, l)] = mk<T>(X.x + Y.y, Y.x - X.y);
+            lds[lidx<TLC, T>(kz, l)] = mk<T>(X.x - Y.y, X.y + Y.x);
+            lds[lidx<TLC, T>(N - kz, l)] = mk<T>(X.x + Y.y, Y.x - X.y);
         }
     }
 }
@@ -474,7 +475,7 @@ __global__ void __launch_bounds__(NT) k_inv_zc_plain(const cplx<T> *__restrict__
     si.init();
     for (int idx = tid; idx < TLC * N; idx += NT) {
         const int l = idx / N, z = idx % N;
-        const cplx<T> v = lds[lidx<TLC>(z, l)];
+        const cplx<T> v = lds[lidx<TLC, T>(z, l)];
         const int64_t o = base + idx;
         if (mode == 0) {
             if (accD) accD[o] = (double)v.x;
@@ -492,7 +493,7 @@ __global__ void __launch_bounds__(NT) k_inv_zc_plain(const cplx<T> *__restrict__
         // numerator of the sum's condition number); the I part is not summed
         for (int idx = tid; idx < TLC * N; idx += NT) {
             const int l = idx / N, z = idx % N;
-            const double p = accD[base + idx] * (double)lds[lidx<TLC>(z, l)].x;
+            const double p = accD[base + idx] * (double)lds[lidx<TLC, T>(z, l)].x;
             sd.add(p);
             si.add(fabs(p));
         }
@@ -542,6 +543,11 @@ __global__ void __launch_bounds__(NT) k_inv_zc_plain(const cplx<T> *__restrict__
 // (6 rows of 144 = 3 paired lines held 212 VGPRs, two waves per SIMD)
 template <typename T, int N> constexpr int zc_wave_rows() { return (sizeof(T) == 4 && N == 144) ? 4 : wave_lines<T, N>(); }
 
+// D-only passes with magnitude sums (ABS): sum |prod D| is taken on the x
+// planes x = 0 mod kAbsStride only and scaled by kAbsStride (an estimate of
+// the condition number's numerator; every supported grid is a multiple of 4)
+constexpr int kAbsStride = 4;
+
 template <typename T, int N, bool DM = false> struct ZcCfg {
     static constexpr int LW0 = zc_wave_rows<T, N>();  // y rows per wavefront
     static constexpr int LW = DM ? LW0 / 2 : LW0;   // transformed lines per wavefront
@@ -573,7 +579,9 @@ template <typename T, int N, bool DM> constexpr int zc_waves()
     return (DM && sizeof(T) == 4 && zc_pow2_mid<T, N>()) ? 4 : 1;
 }
 
-template <typename T, int N, bool DM>
+// ABS (D-only mode): partial slot 1 carries sum |prod D| per output, the
+// numerator of the sum's condition number (an estimate of its rounding error)
+template <typename T, int N, bool DM, bool ABS = false>
 __global__ void __launch_bounds__((ZcCfg<T, N, DM>::BT)) __attribute__((amdgpu_waves_per_eu(zc_waves<T, N, DM>()))) k_inv_zc_fused(const GroupDev *__restrict__ groups,
                                                          int ngroups, const LegDev *__restrict__ legs,
                                                          const int *__restrict__ suffix, int64_t i0, int64_t i1,
@@ -721,23 +729,47 @@ __global__ void __launch_bounds__((ZcCfg<T, N, DM>::BT)) __attribute__((amdgpu_w
                 if (G.pre_sq) run(sink2);
                 else run(sink1);
             } else {
-                // DM: both parts of a line are D of two rows, one sum
+                // DM: both parts of a line are D of two rows, one sum (and
+                // with ABS the sum of the magnitudes of its terms in si)
                 double sd = 0.0, si = 0.0;
-                auto sink1 = [&](int i, int r, int, int, cplx<T> v) {
-                    sd += (double)pd[i][r] * (double)v.x;
-                    if constexpr (DM) sd += (double)pi[i][r] * (double)v.y;
-                    else si += (double)pi[i][r] * (double)v.y;
+                auto sink1 = [&](auto with_abs) {
+                    return [&](int i, int r, int, int, cplx<T> v) {
+                        sd += (double)pd[i][r] * (double)v.x;
+                        if constexpr (DM) sd += (double)pi[i][r] * (double)v.y;
+                        else si += (double)pi[i][r] * (double)v.y;
+                        if constexpr (DM && decltype(with_abs)::value) {
+                            si += fabs((double)pd[i][r]) * fabs((double)v.x);
+                            si += fabs((double)pi[i][r]) * fabs((double)v.y);
+                        }
+                    };
                 };
-                auto sink2 = [&](int i, int r, int, int, cplx<T> v) {
-                    const double dv = (double)v.x, iv = (double)v.y;
-                    sd += (double)pd[i][r] * (dv * dv);
-                    if constexpr (DM) sd += (double)pi[i][r] * (iv * iv);
-                    else si += (double)pi[i][r] * (iv * iv);
+                auto sink2 = [&](auto with_abs) {
+                    return [&](int i, int r, int, int, cplx<T> v) {
+                        const double dv = (double)v.x, iv = (double)v.y;
+                        sd += (double)pd[i][r] * (dv * dv);
+                        if constexpr (DM) sd += (double)pi[i][r] * (iv * iv);
+                        else si += (double)pi[i][r] * (iv * iv);
+                        if constexpr (DM && decltype(with_abs)::value) {
+                            si += fabs((double)pd[i][r]) * (dv * dv);
+                            si += fabs((double)pi[i][r]) * (iv * iv);
+                        }
+                    };
                 };
-                if (G.suf_sq) run(sink2);
-                else run(sink1);
+                // the magnitude sums on every kAbsStride-th x plane (an estimate)
+                const bool ab = DM && ABS && ((int)(tcur / tiles_per_x) % kAbsStride == 0);
+                if (ab) {
+                    auto s1 = sink1(std::true_type{});
+                    auto s2 = sink2(std::true_type{});
+                    if (G.suf_sq) run(s2);
+                    else run(s1);
+                } else {
+                    auto s1 = sink1(std::false_type{});
+                    auto s2 = sink2(std::false_type{});
+                    if (G.suf_sq) run(s2);
+                    else run(s1);
+                }
                 sd = wave_sum_dpp(sd);
-                if constexpr (!DM) si = wave_sum_dpp(si);
+                if (!DM || ab) si = wave_sum_dpp(si);
                 if (lane == 0) {
                     const int o = q - G.npre;
                     KSum a; a.s = acc[w][o][0]; a.c = acc[w][o][1]; a.add(sd);
@@ -774,7 +806,9 @@ __global__ void __launch_bounds__((ZcCfg<T, N, DM>::BT)) __attribute__((amdgpu_w
             f *= g;
             const int64_t po = (int64_t)(G.out_begin + o) * max_chunks + chunk;
             partials[2 * po + 0] = make_double2(a.s * f, a.c * f);
-            partials[2 * po + 1] = make_double2(c.s * g, c.c * g);
+            // I parts: the grid ratio only; |prod D| (ABS): the D part's scale
+            const double fi = (DM && ABS) ? f * kAbsStride : g;
+            partials[2 * po + 1] = make_double2(c.s * fi, c.c * fi);
         }
         __syncthreads();
     }
@@ -808,7 +842,7 @@ template <int NH, int SL> constexpr int zcp_pitch()
     return p;
 }
 
-template <typename T, int N>
+template <typename T, int N, bool ABS = false>
 __global__ void __launch_bounds__((ZcCfg<T, N, true>::BT)) __attribute__((amdgpu_waves_per_eu(zc_waves<T, N, true>())))
 k_zc_pairs(const GroupDev *__restrict__ groups, int ngroups, const LegDev *__restrict__ legs,
            const int *__restrict__ suffix, int64_t i0, int64_t i1, double2 *__restrict__ partials, int max_chunks,
@@ -825,6 +859,7 @@ k_zc_pairs(const GroupDev *__restrict__ groups, int ngroups, const LegDev *__res
     __shared__ f32x4 stage[2][SL * SP];
     __shared__ cplx<T> wbuf[NWV * LW * LP];
     __shared__ double acc[NWV][MAX_SUFFIX][2];
+    __shared__ float acca[ABS ? NWV : 1][ABS ? MAX_SUFFIX : 1];  // sum |prod D| (an estimate: FP32, no compensation)
     const int tid = threadIdx.x, w = tid / 64, lane = tid & 63;
     constexpr bool CMP = zc_compact<T, N>();
     __shared__ cplx<T> stw[CMP ? N : 1];
@@ -843,6 +878,8 @@ k_zc_pairs(const GroupDev *__restrict__ groups, int ngroups, const LegDev *__res
         const int nleg = G.npre + G.nsuf;
         const int64_t nsteps = (t_end - t_begin) * nleg;  // (tile, leg) steps of this item
         for (int i = tid; i < NWV * MAX_SUFFIX * 2; i += BT) (&acc[0][0][0])[i] = 0.0;
+        if constexpr (ABS)
+            for (int i = tid; i < NWV * MAX_SUFFIX; i += BT) (&acca[0][0])[i] = 0.f;
         auto leg_of = [&](int q) -> const LegDev & {
             return legs[q < G.npre ? (q == 0 ? G.preA : G.preB) : suffix[G.suf_begin + q - G.npre]];
         };
@@ -921,21 +958,46 @@ k_zc_pairs(const GroupDev *__restrict__ groups, int ngroups, const LegDev *__res
                 else run(sink1);
             } else {
                 // both parts of a line are D of two rows: one sum
-                double sd = 0.0;
-                auto sink1 = [&](int i, int r, int, int, cplx<T> v) {
-                    sd += (double)pd[i][r] * (double)v.x;
-                    sd += (double)pi[i][r] * (double)v.y;
+                double sd = 0.0, sa = 0.0;
+                auto sink1 = [&](auto with_abs) {
+                    return [&](int i, int r, int, int, cplx<T> v) {
+                        sd += (double)pd[i][r] * (double)v.x;
+                        sd += (double)pi[i][r] * (double)v.y;
+                        if constexpr (decltype(with_abs)::value) {
+                            sa += fabs((double)pd[i][r]) * fabs((double)v.x);
+                            sa += fabs((double)pi[i][r]) * fabs((double)v.y);
+                        }
+                    };
                 };
-                auto sink2 = [&](int i, int r, int, int, cplx<T> v) {
-                    const double dv = (double)v.x, iv = (double)v.y;
-                    sd += (double)pd[i][r] * (dv * dv);
-                    sd += (double)pi[i][r] * (iv * iv);
+                auto sink2 = [&](auto with_abs) {
+                    return [&](int i, int r, int, int, cplx<T> v) {
+                        const double dv = (double)v.x, iv = (double)v.y;
+                        sd += (double)pd[i][r] * (dv * dv);
+                        sd += (double)pi[i][r] * (iv * iv);
+                        if constexpr (decltype(with_abs)::value) {
+                            sa += fabs((double)pd[i][r]) * (dv * dv);
+                            sa += fabs((double)pi[i][r]) * (iv * iv);
+                        }
+                    };
                 };
-                if (G.suf_sq) run(sink2);
-                else run(sink1);
+                // the magnitude sums on every kAbsStride-th x plane (an estimate)
+                const bool ab = ABS && ((int)(tcur / tiles_per_x) % kAbsStride == 0);
+                if (ab) {
+                    auto s1 = sink1(std::true_type{});
+                    auto s2 = sink2(std::true_type{});
+                    if (G.suf_sq) run(s2);
+                    else run(s1);
+                } else {
+                    auto s1 = sink1(std::false_type{});
+                    auto s2 = sink2(std::false_type{});
+                    if (G.suf_sq) run(s2);
+                    else run(s1);
+                }
                 sd = wave_sum_dpp(sd);
+                if (ab) sa = wave_sum_dpp(sa);
                 if (lane == 0) {
                     const int o = q - G.npre;
+                    if constexpr (ABS) acca[w][o] += (float)sa;
                     KSum a;
                     a.s = acc[w][o][0];
                     a.c = acc[w][o][1];
@@ -970,7 +1032,10 @@ k_zc_pairs(const GroupDev *__restrict__ groups, int ngroups, const LegDev *__res
             f *= G.fscale;
             const int64_t po = (int64_t)(G.out_begin + o) * max_chunks + chunk;
             partials[2 * po + 0] = make_double2(a.s * f, a.c * f);
-            partials[2 * po + 1] = make_double2(0.0, 0.0);
+            double sabs = 0.0;
+            if constexpr (ABS)
+                for (int ww = 0; ww < NWV; ++ww) sabs += (double)acca[ww][o];
+            partials[2 * po + 1] = make_double2(sabs * f * kAbsStride, 0.0);
         }
         __syncthreads();
     }
@@ -1036,7 +1101,7 @@ __device__ __forceinline__ void wait_vm_at_most(int n)
     }
 }
 
-template <typename T, int N>
+template <typename T, int N, bool ABS = false>
 __global__ void __launch_bounds__((ZcCfg<T, N, true>::BT)) __attribute__((amdgpu_waves_per_eu(zc_waves<T, N, true>())))
 k_zc_dma(const GroupDev *__restrict__ groups, int ngroups, const LegDev *__restrict__ legs,
          const int *__restrict__ suffix, int64_t i0, int64_t i1, double2 *__restrict__ partials, int max_chunks,
@@ -1056,6 +1121,7 @@ k_zc_dma(const GroupDev *__restrict__ groups, int ngroups, const LegDev *__restr
     __shared__ f32x4 stage[NBUF][SLOTS];
     __shared__ cplx<T> wbuf[NWV * LW * LP];
     __shared__ double acc[NWV][MAX_SUFFIX][2];
+    __shared__ float acca[ABS ? NWV : 1][ABS ? MAX_SUFFIX : 1];  // sum |prod D| (an estimate: FP32, no compensation)
     const int tid = threadIdx.x, lane = tid & 63;
     const int w = __builtin_amdgcn_readfirstlane(tid / 64);
     WaveTw<T, N, LW> twr;
@@ -1092,6 +1158,8 @@ k_zc_dma(const GroupDev *__restrict__ groups, int ngroups, const LegDev *__restr
         const int nleg = G.npre + G.nsuf;
         const int64_t nsteps = (t_end - t_begin) * nleg;  // (tile, leg) steps of this item
         for (int i = tid; i < NWV * MAX_SUFFIX * 2; i += BT) (&acc[0][0][0])[i] = 0.0;
+        if constexpr (ABS)
+            for (int i = tid; i < NWV * MAX_SUFFIX; i += BT) (&acca[0][0])[i] = 0.f;
         auto leg_of = [&](int q) -> const PS_CONST LegDev & {
             return clegs[q < G.npre ? (q == 0 ? G.preA : G.preB) : csuffix[G.suf_begin + q - G.npre]];
         };
@@ -1178,21 +1246,46 @@ k_zc_dma(const GroupDev *__restrict__ groups, int ngroups, const LegDev *__restr
                 if (G.pre_sq) run(sink2);
                 else run(sink1);
             } else {
-                double sd = 0.0;
-                auto sink1 = [&](int i, int r, int, int, cplx<T> v) {
-                    sd += (double)pd[i][r] * (double)v.x;
-                    sd += (double)pi[i][r] * (double)v.y;
+                double sd = 0.0, sa = 0.0;
+                auto sink1 = [&](auto with_abs) {
+                    return [&](int i, int r, int, int, cplx<T> v) {
+                        sd += (double)pd[i][r] * (double)v.x;
+                        sd += (double)pi[i][r] * (double)v.y;
+                        if constexpr (decltype(with_abs)::value) {
+                            sa += fabs((double)pd[i][r]) * fabs((double)v.x);
+                            sa += fabs((double)pi[i][r]) * fabs((double)v.y);
+                        }
+                    };
                 };
-                auto sink2 = [&](int i, int r, int, int, cplx<T> v) {
-                    const double dv = (double)v.x, iv = (double)v.y;
-                    sd += (double)pd[i][r] * (dv * dv);
-                    sd += (double)pi[i][r] * (iv * iv);
+                auto sink2 = [&](auto with_abs) {
+                    return [&](int i, int r, int, int, cplx<T> v) {
+                        const double dv = (double)v.x, iv = (double)v.y;
+                        sd += (double)pd[i][r] * (dv * dv);
+                        sd += (double)pi[i][r] * (iv * iv);
+                        if constexpr (decltype(with_abs)::value) {
+                            sa += fabs((double)pd[i][r]) * (dv * dv);
+                            sa += fabs((double)pi[i][r]) * (iv * iv);
+                        }
+                    };
                 };
-                if (G.suf_sq) run(sink2);
-                else run(sink1);
+                // the magnitude sums on every kAbsStride-th x plane (an estimate)
+                const bool ab = ABS && ((int)(t0s / tiles_per_x) % kAbsStride == 0);
+                if (ab) {
+                    auto s1 = sink1(std::true_type{});
+                    auto s2 = sink2(std::true_type{});
+                    if (G.suf_sq) run(s2);
+                    else run(s1);
+                } else {
+                    auto s1 = sink1(std::false_type{});
+                    auto s2 = sink2(std::false_type{});
+                    if (G.suf_sq) run(s2);
+                    else run(s1);
+                }
                 sd = wave_sum_dpp(sd);
+                if (ab) sa = wave_sum_dpp(sa);
                 if (lane == 0) {
                     const int o = q - G.npre;
+                    if constexpr (ABS) acca[w][o] += (float)sa;
                     KSum a;
                     a.s = acc[w][o][0];
                     a.c = acc[w][o][1];
@@ -1228,7 +1321,10 @@ k_zc_dma(const GroupDev *__restrict__ groups, int ngroups, const LegDev *__restr
             f *= G.fscale;
             const int64_t po = (int64_t)(G.out_begin + o) * max_chunks + chunk;
             partials[2 * po + 0] = make_double2(a.s * f, a.c * f);
-            partials[2 * po + 1] = make_double2(0.0, 0.0);
+            double sabs = 0.0;
+            if constexpr (ABS)
+                for (int ww = 0; ww < NWV; ++ww) sabs += (double)acca[ww][o];
+            partials[2 * po + 1] = make_double2(sabs * f * kAbsStride, 0.0);
         }
         __syncthreads();
     }
@@ -1366,10 +1462,10 @@ __global__ void __launch_bounds__(NT) k_inv_z_real(const cplx<T> *__restrict__ H
         const int64_t o = ((int64_t)kz * N + x) * N + y0 + 2 * c;
         const cplx<T> X = H[o], Y = H[o + 1];
         if (kz == 0 || kz == N / 2) {
-            lds[lidx<TLC>(kz, c)] = mk<T>(X.x, Y.x);
+            lds[lidx<TLC, T>(kz, c)] = mk<T>(X.x, Y.x);
         } else {
-            lds[lidx<TLC>(kz, c)] = mk<T>(X.x - Y.y, X.y + Y.x);
-            lds[lidx<TLC>(N - kz, c)] = mk<T>(X.x + Y.y, Y.x - X.y);
+            lds[lidx<TLC, T>(kz, c)] = mk<T>(X.x - Y.y, X.y + Y.x);
+            lds[lidx<TLC, T>(N - kz, c)] = mk<T>(X.x + Y.y, Y.x - X.y);
         }
     }
     __syncthreads();
@@ -1377,7 +1473,7 @@ __global__ void __launch_bounds__(NT) k_inv_z_real(const cplx<T> *__restrict__ H
     const int64_t base = ((int64_t)x * N + y0) * N;
     for (int idx = tid; idx < TL * N; idx += NT) {
         const int row = idx / N, z = idx % N;
-        const cplx<T> v = lds[lidx<TLC>(z, row >> 1)];
+        const cplx<T> v = lds[lidx<TLC, T>(z, row >> 1)];
         const double r = (double)((row & 1) ? v.y : v.x);
         out[base + idx] = accumulate ? out[base + idx] + r : r;
     }
@@ -1539,7 +1635,7 @@ static int launch_zc_plain_n(const void *gd, const void *gi, int Kz, double *acc
     return (int)hipGetLastError();
 }
 
-template <typename T, int N, bool DM>
+template <typename T, int N, bool DM, bool ABS = false>
 static int launch_zc_fused_m(const GroupDev *groups, int ngroups, const LegDev *legs, const int *suffix,
                              int64_t i0, int64_t i1, double2 *partials, int max_chunks, int grid_cap, int kz_cap,
                              const void *tw, hipStream_t st)
@@ -1549,7 +1645,7 @@ static int launch_zc_fused_m(const GroupDev *groups, int ngroups, const LegDev *
     // two staging buffers (the next step's LDS stores need no extra barrier)
     // when they fit next to the static LDS, except in the compact form
     hipFuncAttributes fa;
-    hipError_t e = hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(&k_inv_zc_fused<T, N, DM>));
+    hipError_t e = hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(&k_inv_zc_fused<T, N, DM, ABS>));
     if (e != hipSuccess) return (int)e;
     int dev = 0, lds_max = 0;
     e = hipGetDevice(&dev);
@@ -1560,12 +1656,12 @@ static int launch_zc_fused_m(const GroupDev *groups, int ngroups, const LegDev *
     if (fa.sharedSizeBytes + 2 * per_buf > (size_t)lds_max) nbuf = 1;
     if (fa.sharedSizeBytes + nbuf * per_buf > (size_t)lds_max) return PS_ERR_UNSUPPORTED_N;
     const size_t dyn = nbuf * per_buf;
-    e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_inv_zc_fused<T, N, DM>),
+    e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_inv_zc_fused<T, N, DM, ABS>),
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
     if (e != hipSuccess) return (int)e;
     if (i1 <= i0) return 0;
     const unsigned g = (unsigned)((i1 - i0 < grid_cap) ? i1 - i0 : grid_cap);
-    hipLaunchKernelGGL((k_inv_zc_fused<T, N, DM>), dim3(g), dim3(C::BT), dyn, st, groups, ngroups, legs, suffix, i0,
+    hipLaunchKernelGGL((k_inv_zc_fused<T, N, DM, ABS>), dim3(g), dim3(C::BT), dyn, st, groups, ngroups, legs, suffix, i0,
                        i1, partials, max_chunks, reinterpret_cast<const cplx<T> *>(tw), kz_cap, nbuf);
     return (int)hipGetLastError();
 }
@@ -1590,34 +1686,48 @@ static bool zc_dma_enabled()
     return on;
 }
 
+// dmode bit 0: D parts only; bit 1 (FP32 D-only): partial slot 1 of every
+// output carries sum |prod D| (the numerator of its condition number)
+template <typename T, int N, bool ABS>
+static int launch_zc_dmode(const GroupDev *groups, int ngroups, const LegDev *legs, const int *suffix, int64_t i0,
+                           int64_t i1, double2 *partials, int max_chunks, int grid_cap, int kz_cap, const void *tw,
+                           hipStream_t st)
+{
+    const unsigned g = (unsigned)((i1 - i0 < grid_cap) ? i1 - i0 : grid_cap);
+    if constexpr (zc_dma_ok<T, N>()) {
+        if (zc_pairs_enabled() && zc_dma_enabled()) {
+            if (i1 <= i0) return 0;
+            hipLaunchKernelGGL((k_zc_dma<T, N, ABS>), dim3(g), dim3(ZcCfg<T, N, true>::BT), 0, st, groups, ngroups,
+                               legs, suffix, i0, i1, partials, max_chunks, reinterpret_cast<const cplx<T> *>(tw));
+            return (int)hipGetLastError();
+        }
+    }
+    if constexpr (zc_pair_ok<T, N>()) {
+        if (zc_pairs_enabled()) {
+            if (i1 <= i0) return 0;
+            hipLaunchKernelGGL((k_zc_pairs<T, N, ABS>), dim3(g), dim3(ZcCfg<T, N, true>::BT), 0, st, groups, ngroups,
+                               legs, suffix, i0, i1, partials, max_chunks, reinterpret_cast<const cplx<T> *>(tw));
+            return (int)hipGetLastError();
+        }
+    }
+    if constexpr (zc_dmode_ok<T, N>())
+        return launch_zc_fused_m<T, N, true, ABS>(groups, ngroups, legs, suffix, i0, i1, partials, max_chunks,
+                                                  grid_cap, kz_cap, tw, st);
+    return PS_ERR_INVALID;
+}
+
 template <typename T, int N>
 static int launch_zc_fused_n(const GroupDev *groups, int ngroups, const LegDev *legs, const int *suffix,
                              int64_t i0, int64_t i1, double2 *partials, int max_chunks, int grid_cap, int kz_cap,
                              int dmode, const void *tw, hipStream_t st)
 {
-    if constexpr (zc_dma_ok<T, N>()) {
-        if (dmode && zc_pairs_enabled() && zc_dma_enabled()) {
-            if (i1 <= i0) return 0;
-            const unsigned g = (unsigned)((i1 - i0 < grid_cap) ? i1 - i0 : grid_cap);
-            hipLaunchKernelGGL((k_zc_dma<T, N>), dim3(g), dim3(ZcCfg<T, N, true>::BT), 0, st, groups, ngroups, legs,
-                               suffix, i0, i1, partials, max_chunks, reinterpret_cast<const cplx<T> *>(tw));
-            return (int)hipGetLastError();
-        }
-    }
-    if constexpr (zc_pair_ok<T, N>()) {
-        if (dmode && zc_pairs_enabled()) {
-            if (i1 <= i0) return 0;
-            const unsigned g = (unsigned)((i1 - i0 < grid_cap) ? i1 - i0 : grid_cap);
-            hipLaunchKernelGGL((k_zc_pairs<T, N>), dim3(g), dim3(ZcCfg<T, N, true>::BT), 0, st, groups, ngroups, legs,
-                               suffix, i0, i1, partials, max_chunks, reinterpret_cast<const cplx<T> *>(tw));
-            return (int)hipGetLastError();
-        }
-    }
-    if (dmode) {
-        if constexpr (zc_dmode_ok<T, N>())
-            return launch_zc_fused_m<T, N, true>(groups, ngroups, legs, suffix, i0, i1, partials, max_chunks,
-                                                 grid_cap, kz_cap, tw, st);
-        return PS_ERR_INVALID;
+    if (dmode & 1) {
+        // the magnitude sums exist in the FP32 kernels only (mixed-precision refinement)
+        if (sizeof(T) == 4 && (dmode & 2))
+            return launch_zc_dmode<T, N, sizeof(T) == 4>(groups, ngroups, legs, suffix, i0, i1, partials,
+                                                           max_chunks, grid_cap, kz_cap, tw, st);
+        return launch_zc_dmode<T, N, false>(groups, ngroups, legs, suffix, i0, i1, partials, max_chunks, grid_cap,
+                                            kz_cap, tw, st);
     }
     return launch_zc_fused_m<T, N, false>(groups, ngroups, legs, suffix, i0, i1, partials, max_chunks, grid_cap,
                                           kz_cap, tw, st);

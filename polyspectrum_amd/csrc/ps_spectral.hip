@@ -223,7 +223,7 @@ __global__ void __launch_bounds__(NT) k_spec_fx(const SpecGroupDev *__restrict__
         cplx<T> *out = reinterpret_cast<cplx<T> *>(part ? G.bi : G.bd);
         tile_loop<TLC * N>(tid, [&](int idx) {
             const int l = idx % TLC, xx = idx / TLC;
-            lds[lidx<TLC>(xx, l)] = gload<T>(in, ((int64_t)kz * N + xx) * N + y0 + l);
+            lds[lidx<TLC, T>(xx, l)] = gload<T>(in, ((int64_t)kz * N + xx) * N + y0 + l);
         });
         __syncthreads();
         fft_lds<T, N, TLC, -1>(lds, stw, tid);
@@ -231,7 +231,7 @@ __global__ void __launch_bounds__(NT) k_spec_fx(const SpecGroupDev *__restrict__
         for (int idx = tid; idx < TLC * Bq; idx += NT) {
             const int l = idx % TLC, jx = idx / TLC;
             const int kx = band_to_k(jx, G.Kq, Bq, N);
-            gstore<T>(out, ((int64_t)kz * Bq + jx) * N + y0 + l, lds[lidx<TLC>(kx, l)]);
+            gstore<T>(out, ((int64_t)kz * Bq + jx) * N + y0 + l, lds[lidx<TLC, T>(kx, l)]);
         }
         __syncthreads();
     }
