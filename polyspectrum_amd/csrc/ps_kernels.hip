@@ -1101,8 +1101,11 @@ __device__ __forceinline__ void wait_vm_at_most(int n)
     }
 }
 
-template <typename T, int N, bool ABS = false>
-__global__ void __launch_bounds__((ZcCfg<T, N, true>::BT)) __attribute__((amdgpu_waves_per_eu(zc_waves<T, N, true>())))
+// NBUF staging buffers: copies are requested NBUF - 1 steps ahead (3: two
+// steps; 2: one step, with the registers capped for four waves per SIMD)
+template <typename T, int N, bool ABS = false, int NBUF = 3>
+__global__ void __launch_bounds__((ZcCfg<T, N, true>::BT))
+__attribute__((amdgpu_waves_per_eu(NBUF == 2 ? 4 : zc_waves<T, N, true>())))
 k_zc_dma(const GroupDev *__restrict__ groups, int ngroups, const LegDev *__restrict__ legs,
          const int *__restrict__ suffix, int64_t i0, int64_t i1, double2 *__restrict__ partials, int max_chunks,
          const cplx<T> *__restrict__ twg)
@@ -1113,7 +1116,7 @@ k_zc_dma(const GroupDev *__restrict__ groups, int ngroups, const LegDev *__restr
     constexpr int LP = wave_fft_pitch<T, N>();
     constexpr int SLOTS = NH * SL;                    // 16-byte slots of a staging buffer
     constexpr int MPT = (SLOTS + BT - 1) / BT;        // copy instructions per wavefront and step, at most
-    constexpr int NBUF = 3;
+    static_assert(NBUF == 2 || NBUF == 3, "one or two steps ahead");
     using WO = WaveOut<T, N, LW>;
     static_assert(sizeof(T) == 4, "FP32 pairs of complex values are 16 bytes");
     static_assert(SL % 8 == 0 && (SL & (SL - 1)) == 0, "slot rotation: whole 128-byte segments per plane, SL a power of two");
@@ -1199,7 +1202,7 @@ k_zc_dma(const GroupDev *__restrict__ groups, int ngroups, const LegDev *__restr
         next_of(t1s, q1, t2s, q2);
         int n1 = 0, n2 = 0;
         issue(t0s, q0, 0);
-        if (1 < nsteps) n1 = issue(t1s, q1, 1);
+        if (NBUF == 3 && 1 < nsteps) n1 = issue(t1s, q1, 1);
         T pd[WO::IT][WO::R], pi[WO::IT][WO::R];
         for (int64_t s = 0; s < nsteps; ++s) {
             const int b = (int)(s % NBUF);
@@ -1207,8 +1210,13 @@ k_zc_dma(const GroupDev *__restrict__ groups, int ngroups, const LegDev *__restr
             // n1 instructions issued after them are outstanding
             wait_vm_at_most(n1);
             __syncthreads();
-            if (s + 2 < nsteps) n2 = issue(t2s, q2, (int)((s + 2) % NBUF));
-            else n2 = 0;
+            if constexpr (NBUF == 3) {
+                if (s + 2 < nsteps) n2 = issue(t2s, q2, (int)((s + 2) % NBUF));
+                else n2 = 0;
+            } else {
+                if (s + 1 < nsteps) issue(t1s, q1, (int)((s + 1) % NBUF));
+                n2 = 0;  // nothing is issued between a step's copies and its wait
+            }
             const int q = q0;
             if (q == 0) {
 #pragma unroll
@@ -1686,6 +1694,20 @@ static bool zc_dma_enabled()
     return on;
 }
 
+// staging buffers of the copy-pipelined pass: two (one step ahead, four
+// workgroups per CU) where three would leave LDS for only three workgroups
+// (96, 192: measured Zc 0.404 -> 0.392 ms on the 512^3 sweep); three at 144,
+// where four workgroups fit either way.  PS_ZC_NBUF=2 or 3 sets every grid.
+template <int N> static int zc_dma_nbuf()
+{
+    static const int n = [] {
+        const char *e = getenv("PS_ZC_NBUF");
+        if (e && (e[0] == '2' || e[0] == '3')) return e[0] - '0';
+        return (N == 144) ? 3 : 2;
+    }();
+    return n;
+}
+
 // dmode bit 0: D parts only; bit 1 (FP32 D-only): partial slot 1 of every
 // output carries sum |prod D| (the numerator of its condition number)
 template <typename T, int N, bool ABS>
@@ -1697,8 +1719,14 @@ static int launch_zc_dmode(const GroupDev *groups, int ngroups, const LegDev *le
     if constexpr (zc_dma_ok<T, N>()) {
         if (zc_pairs_enabled() && zc_dma_enabled()) {
             if (i1 <= i0) return 0;
-            hipLaunchKernelGGL((k_zc_dma<T, N, ABS>), dim3(g), dim3(ZcCfg<T, N, true>::BT), 0, st, groups, ngroups,
-                               legs, suffix, i0, i1, partials, max_chunks, reinterpret_cast<const cplx<T> *>(tw));
+            if (zc_dma_nbuf<N>() == 2)
+                hipLaunchKernelGGL((k_zc_dma<T, N, ABS, 2>), dim3(g), dim3(ZcCfg<T, N, true>::BT), 0, st, groups,
+                                   ngroups, legs, suffix, i0, i1, partials, max_chunks,
+                                   reinterpret_cast<const cplx<T> *>(tw));
+            else
+                hipLaunchKernelGGL((k_zc_dma<T, N, ABS, 3>), dim3(g), dim3(ZcCfg<T, N, true>::BT), 0, st, groups,
+                                   ngroups, legs, suffix, i0, i1, partials, max_chunks,
+                                   reinterpret_cast<const cplx<T> *>(tw));
             return (int)hipGetLastError();
         }
     }
