@@ -50,8 +50,12 @@ template <int TLC, typename T> constexpr int lidx_shift()
     while ((TLC * (int)sizeof(cplx<T>) << sh) < 128) ++sh;
     return sh;
 }
-template <int TLC, typename T> __device__ __forceinline__ int lidx(int e, int l)
+template <int TLC, typename T, bool SWZ = true> __device__ __forceinline__ int lidx(int e, int l)
 {
+    // SWZ = false: plain interleaving e*TLC + l, for tiles whose every
+    // access has the line index fastest (then any key is conflict-free) and
+    // that are filled by LDS copies of 16-byte pairs of adjacent lines
+    if constexpr (!SWZ) return e * TLC + l;
     return e * TLC + (l ^ ((e >> lidx_shift<TLC, T>()) & (TLC - 1)));
 }
 
@@ -186,7 +190,7 @@ template <typename T, int S> __device__ __forceinline__ cplx<T> twiddle(const cp
 
 // One Stockham stage over TLC lines in LDS.  Starts by reading LDS (callers
 // synchronise before the first stage) and ends with a barrier.
-template <typename T, int N, int TLC, int S, int R, int NS>
+template <typename T, int N, int TLC, int S, int R, int NS, bool SWZ = true>
 __device__ __forceinline__ void stage(cplx<T> *lds, const cplx<T> *__restrict__ tw, int tid)
 {
     constexpr int NB = N / R;
@@ -199,7 +203,7 @@ __device__ __forceinline__ void stage(cplx<T> *lds, const cplx<T> *__restrict__ 
         if (TOT % NT == 0 || b < TOT) {
             const int l = b % TLC, j = b / TLC;
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[it][r] = lds[lidx<TLC, T>(j + r * NB, l)];
+            for (int r = 0; r < R; ++r) v[it][r] = lds[lidx<TLC, T, SWZ>(j + r * NB, l)];
             if constexpr (NS > 1) {
                 const int k = j % NS;
 #pragma unroll
@@ -217,7 +221,7 @@ __device__ __forceinline__ void stage(cplx<T> *lds, const cplx<T> *__restrict__ 
             const int k = j % NS;
             const int d = (j / NS) * NS * R + k;
 #pragma unroll
-            for (int r = 0; r < R; ++r) lds[lidx<TLC, T>(d + r * NS, l)] = v[it][r];
+            for (int r = 0; r < R; ++r) lds[lidx<TLC, T, SWZ>(d + r * NS, l)] = v[it][r];
         }
     }
     __syncthreads();
@@ -254,12 +258,12 @@ __device__ __forceinline__ void stage_twiddles(cplx<T> *stw, const cplx<T> *__re
     for (int i = tid; i < N; i += NTH) stw[i] = tw[i];
 }
 
-template <typename T, int N, int TLC, int S, int ST = 0, int PLAN = 0>
+template <typename T, int N, int TLC, int S, int ST = 0, int PLAN = 0, bool SWZ = true>
 __device__ __forceinline__ void fft_lds(cplx<T> *lds, const cplx<T> *__restrict__ tw, int tid)
 {
     if constexpr (ST < num_stages(N)) {
-        stage<T, N, TLC, S, stage_radix_p<PLAN>(N, ST), stage_span_p<PLAN>(N, ST)>(lds, tw, tid);
-        fft_lds<T, N, TLC, S, ST + 1, PLAN>(lds, tw, tid);
+        stage<T, N, TLC, S, stage_radix_p<PLAN>(N, ST), stage_span_p<PLAN>(N, ST), SWZ>(lds, tw, tid);
+        fft_lds<T, N, TLC, S, ST + 1, PLAN, SWZ>(lds, tw, tid);
     }
 }
 

@@ -1338,6 +1338,135 @@ k_zc_dma(const GroupDev *__restrict__ groups, int ngroups, const LegDev *__restr
     }
 }
 
+// G2 for the D-only waves in FP32, with the band values copied into LDS by
+// the memory system (global_load_lds_dwordx4, 16 bytes = two adjacent
+// columns per lane) one tile ahead, into three tile buffers.  Tiles, the
+// XCD pairing of half-line tiles and the transform are those of k_inv_g2;
+// the tile layout is the plain interleaving e * TLC + l (every access of
+// the tile has the column index fastest).  Band lines outside the band or
+// the shell's disk are copied from a zero block.  Tile i runs as: wait for
+// every vector memory instruction of this wavefront (its copies of tile i,
+// and the stores of tile i - 2, issued a transform ago), barrier, copies of
+// tile i + 1 into buffer (i + 1) mod 3, stores of tile i - 1 from its
+// buffer, transform of tile i.  Global stores and loads share one counter
+// and may complete out of order, so the wait is for all of them; issuing the
+// stores before the transform gives them its duration to complete.
+__device__ __align__(16) float g_zero16[4 * 64];  // zeros: the copy source of empty band lines
+
+// where the three tile buffers leave LDS for two workgroups per CU
+template <typename T, int N, int WD> constexpr bool g2_dma_ok()
+{
+    constexpr int TLC = tile_lines<T, N>() / WD;
+    return sizeof(T) == 4 && TLC % 2 == 0 && (3 * TLC + 1) * N * (int)sizeof(cplx<T>) <= 80 * 1024;
+}
+
+template <typename T, int N, int WD>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(band_pass_waves<T, N>())))
+k_inv_g2_dma(const LegDev *__restrict__ legs, int nlegs, int64_t t0, int64_t t1, const cplx<T> *__restrict__ tw)
+{
+    static_assert(sizeof(T) == 4, "16-byte copies hold two FP32 complex values");
+    constexpr int TLC = tile_lines<T, N>() / WD;
+    constexpr int TOT = TLC * N;     // complex values of a tile
+    constexpr int CH = TOT / 2;      // 16-byte pairs of a tile
+    constexpr int CPT = (CH + NT - 1) / NT;
+    constexpr int HP = TLC / 2;      // pairs per band line of a tile
+    static_assert(TLC % 2 == 0, "pairs of adjacent columns");
+    __shared__ cplx<T> tb[3][TOT];
+    __shared__ cplx<T> stw[N];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid / 64);
+    constexpr int nyt = N / TLC;
+    stage_twiddles<T, N>(stw, tw, tid);
+    const PS_CONST LegDev *clegs = const_tab(legs);
+    const unsigned tb_lds = (unsigned)(uintptr_t)(__attribute__((address_space(3))) void *)&tb[0][0];
+    const PS_GLOBAL f32x4 *zero = (const PS_GLOBAL f32x4 *)g_zero16;
+    struct Tile {
+        const cplx<T> *g;
+        cplx<T> *o;
+        int kz, y0, Kx, Bx, kx2;
+    };
+    constexpr bool PAIR = TLC * (int)sizeof(cplx<T>) < 128;
+    auto decode = [&](int64_t tile) -> Tile {
+        if (PAIR && (tile & ~(int64_t)15) >= t0 && (tile | 15) < t1) tile = (tile & ~(int64_t)15) | ((tile & 7) << 1) | ((tile >> 3) & 1);
+        int li = 0;
+        {
+            int hi = nlegs - 1;
+            while (li < hi) {
+                const int mid = (li + hi + 1) >> 1;
+                if (clegs[mid].g2_begin <= tile) li = mid;
+                else hi = mid - 1;
+            }
+        }
+        const PS_CONST LegDev &L = clegs[li];
+        const int64_t local = tile - L.g2_begin;
+        const int64_t per_part = (int64_t)(L.Kz + 1) * nyt;
+        const int part = (int)(local / per_part);
+        const int64_t rest = local - part * per_part;
+        Tile t;
+        t.g = reinterpret_cast<const cplx<T> *>(part ? L.g1i : L.g1d);
+        t.o = reinterpret_cast<cplx<T> *>(part ? L.g2i : L.g2d);
+        t.kz = (int)(rest / nyt);
+        t.y0 = (int)(rest % nyt) * TLC;
+        t.Kx = L.Kx;
+        t.Bx = L.Bx;
+        const int mz = lattice(t.kz, N);
+        t.kx2 = (L.r2 < 0) ? N * N : L.r2 - mz * mz;
+        return t;
+    };
+    auto issue = [&](const Tile &t, int b) {
+        const PS_GLOBAL f32x4 *g = (const PS_GLOBAL f32x4 *)t.g;
+#pragma unroll
+        for (int m = 0; m < CPT; ++m) {
+            const int base = NT * m + 64 * w;  // uniform over the wavefront
+            if (CH % NT == 0 || base < CH) {
+                const int c = base + lane;
+                if (CH % NT == 0 || c < CH) {
+                    const int kx = c / HP, l = 2 * (c % HP);
+                    const int j = k_to_band(kx, t.Kx, t.Bx, N);
+                    const int mx = lattice(kx, N);
+                    const PS_GLOBAL f32x4 *src = (j >= 0 && g && mx * mx <= t.kx2)
+                                                     ? g + (((int64_t)t.kz * t.Bx + j) * N + t.y0 + l) / 2
+                                                     : zero + lane;
+                    lds_copy16(src, tb_lds + 16u * (unsigned)(b * CH + base));
+                }
+            }
+        }
+    };
+    auto put = [&](const Tile &t, const cplx<T> *buf) {
+        if (t.o) {
+            tile_loop_u<TOT, 8>(tid, [&](int idx) {
+                const int l = idx % TLC, x = idx / TLC;
+                gstore<T>(t.o, ((int64_t)t.kz * N + x) * N + t.y0 + l, buf[lidx<TLC, T, false>(x, l)]);
+            });
+        }
+    };
+    int64_t tile = t0 + blockIdx.x;
+    if (tile >= t1) return;
+    Tile cur = decode(tile);
+    Tile prev = cur;
+    issue(cur, 0);
+    int i = 0;
+    for (;; ++i) {
+        const int b = i % 3;
+        // the copies of this tile (and the stores issued a transform ago)
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        __syncthreads();
+        const int64_t nt = tile + gridDim.x;
+        Tile nx = cur;
+        if (nt < t1) {
+            nx = decode(nt);
+            issue(nx, (i + 1) % 3);
+        }
+        if (i > 0) put(prev, tb[(i + 2) % 3]);
+        fft_lds<T, N, TLC, +1, 0, 0, false>(tb[b], stw, tid);
+        if (nt >= t1) break;
+        prev = cur;
+        cur = nx;
+        tile = nt;
+    }
+    put(cur, tb[i % 3]);  // the last tile (the transform ended with a barrier)
+}
+
 // Amplitude scale of each leg: the rms of |spectrum| over (a sample of about
 // 4k) cells of its shell.  Packing D + i I into one complex transform
 // in the last pass gives both parts the rounding noise of the larger one;
@@ -1610,6 +1739,16 @@ static int launch_forward_n(const void *field, int field_is_double, void *H, con
     return launch_forward_v<T, N, TL, 0>(field, field_is_double, H, tw, st);
 }
 
+// the copy-pipelined G2 of the D-only waves (PS_G2_DMA=0: k_inv_g2)
+static bool g2_dma_enabled()
+{
+    static const bool on = [] {
+        const char *e = getenv("PS_G2_DMA");
+        return !(e && e[0] == '0');
+    }();
+    return on;
+}
+
 template <typename T, int N>
 static int launch_inverse_g_n(const LegDev *legs_dev, int nlegs, int64_t g1a, int64_t g1b, int64_t g2a, int64_t g2b,
                               int grid_cap, int g2_div, int dmode, const void *tw, hipStream_t st)
@@ -1624,6 +1763,23 @@ static int launch_inverse_g_n(const LegDev *legs_dev, int nlegs, int64_t g1a, in
     }
     if (g2b > g2a) {
         const unsigned g = (unsigned)((g2b - g2a < grid_cap) ? g2b - g2a : grid_cap);
+        if (dmode && g2_dma_enabled()) {
+            const bool half = g2_tile_cols(N, 0, g2_div) < tile_lines<T, N>();
+            if constexpr (g2_dma_ok<T, N, 2>()) {
+                if (half) {
+                    hipLaunchKernelGGL((k_inv_g2_dma<T, N, 2>), dim3(g), dim3(NT), 0, st, legs_dev, nlegs, g2a, g2b,
+                                       twp);
+                    return (int)hipGetLastError();
+                }
+            }
+            if constexpr (g2_dma_ok<T, N, 1>()) {
+                if (!half) {
+                    hipLaunchKernelGGL((k_inv_g2_dma<T, N, 1>), dim3(g), dim3(NT), 0, st, legs_dev, nlegs, g2a, g2b,
+                                       twp);
+                    return (int)hipGetLastError();
+                }
+            }
+        }
         if (g2_tile_cols(N, sizeof(T) == 8, g2_div) < tile_lines<T, N>())
             hipLaunchKernelGGL((k_inv_g2<T, N, 2>), dim3(g), dim3(NT), 0, st, legs_dev, nlegs, g2a, g2b, twp);
         else
