@@ -190,9 +190,10 @@ int ps_batch_create_ex(ps_ctx *ctx, ps_polygon *polys, int npoly, double kbinwid
                        const ps_launch_params *params, ps_batch **out);
 
 /* Autotuner: times candidate launch shapes on this batch (the fastest of reps
- * replays each), keeps the fastest shape for later ps_batch_create calls of
- * the same batch shape (and appends it to the file PS_TUNE_FILE names, if
- * set); best_ms receives its replay time. */
+ * replays each), times the four fastest again (the median of 3 reps replays)
+ * and keeps the best of those for later ps_batch_create calls of the same
+ * batch shape (and appends it to the file PS_TUNE_FILE names, if set);
+ * best_ms receives its median replay time. */
 int ps_batch_autotune(ps_ctx *ctx, ps_polygon *polys, int npoly, double kbinwidth, int kbinning_case, int reps,
                       ps_launch_params *best, double *best_ms);
 int ps_batch_params(const ps_batch *b, ps_launch_params *out);
