@@ -323,6 +323,8 @@ def main():
     ap.add_argument("--refine-tol", type=float, default=1e-5,
                     help="mixed precision: triangles whose error estimate exceeds this are evaluated again in FP64 "
                          "inside the step (0: off)")
+    ap.add_argument("--skip-unrefined", action="store_true",
+                    help="do not time the step without refinement beside the refined one")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy 5 times (scale for the FETCH/WRITE counters)")
     args = ap.parse_args()
@@ -378,7 +380,7 @@ def main():
     value = ntri * args.steps / elapsed
     refined, err_est = batch.refinement()
     unrefined = None
-    if refine:
+    if refine and not args.skip_unrefined:
         # the same batch without refinement (and without the magnitude sums)
         ctx.set_refinement(0.0)
         for _ in range(args.warmup):

@@ -7,7 +7,7 @@ in them can be regenerated from the sources.  On the GPU machine, with the
 repository at $R and the output at $OUT:
 
     cd /tmp && export TMPDIR=/tmp
-    B="$R/bench.py --steps 20 --warmup 3 --skip-cpu --skip-extras --plain-samples 1 --no-autotune"
+    B="$R/bench.py --steps 20 --warmup 3 --skip-cpu --skip-extras --plain-samples 1 --no-autotune --skip-unrefined"
     rocprofv3 --kernel-trace --stats --output-format csv -d $OUT/trace -o run -- python3 $B
     rocprofv3 --pmc FETCH_SIZE --output-format csv -d $OUT/fetch -o run -- python3 $B
     rocprofv3 --pmc WRITE_SIZE --output-format csv -d $OUT/write -o run -- python3 $B
@@ -42,7 +42,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 PASSES = {
     "direct": [
         ("G1 masked gather + ky inverse", r"k_inv_g1d?<"),
-        ("G2 kx inverse (band -> all x)", r"k_inv_g2<"),
+        ("G2 kx inverse (band -> all x)", r"k_inv_g2<|k_inv_g2_dma<"),
         ("Zc fused C2R + product-reduce", r"k_inv_zc_fused<|k_zc_pairs<|k_zc_dma<"),
         ("final reduce", r"k_reduce_partials"),
     ],
@@ -265,7 +265,7 @@ def write_summaries(args):
         lines = [l for l in open(args.bench).read().splitlines() if l.startswith("{")]
         bench = json.loads(lines[-1]) if lines else None
     cmd_b = ("python3 $R/bench.py --steps 20 --warmup 3 --skip-cpu --skip-extras --plain-samples 1 --no-autotune"
-             + (" --method fourier" if method == "fourier" else ""))
+             + (" --method fourier" if method == "fourier" else " --skip-unrefined"))
     regen = (f"python3 -m polyspectrum_amd.profile_summary --dir $OUT --method {method} --out $R/profiles"
              + (" --isa " + " ".join("$OUT/" + os.path.basename(x) for x in args.isa) if args.isa else ""))
     isa_cmd = ("hipcc -O3 -std=c++17 --offload-arch=gfx950 -Xclang -target-feature -Xclang -packed-fp32-ops "
@@ -291,10 +291,11 @@ def write_summaries(args):
           "and `roofline` time them with HIP events).  Each grid wave has its own instance (`<float, M>`, M the "
           "wave's alias-free transform grid); in the graph the waves run side by side on branch streams, so the "
           "per-call averages of the first table include that sharing of the CUs.  Kernels `<double, ...>` are "
-          "the FP64 planning run of the polygon counts and bench.py's FP64 accuracy reference "
+          "the FP64 planning runs of the polygon counts, bench.py's FP64 accuracy reference "
           "(`<double, 512>`: its reference-shaped sample and the condition numbers of the sums, one "
-          "reference-shaped evaluation per triangle), `k_direct_pairs` its transform-free check and "
-          "`k_fwd_z`/`k_c2c_*` the forward transforms: not part of the step.", ""]
+          "reference-shaped evaluation per triangle) and, within each step, the FP64 batch of the triangles the "
+          "refinement evaluates again (small grids); `k_direct_pairs` is the transform-free check and "
+          "`k_fwd_z`/`k_c2c_*` the forward transforms.  The pass sums below are of the FP32 kernels.", ""]
     md.append(_table(["kernel", "calls", "avg us", "total ms", "% of traced GPU time"],
                      [(f"`{k}`", n, f"{a:.1f}", f"{t:.3f}", f"{p:.2f}") for k, n, a, t, p in stats[:args.top]]))
     isa = read_isa(args.isa) if args.isa else {}
