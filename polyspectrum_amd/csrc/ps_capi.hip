@@ -2240,6 +2240,7 @@ extern "C" int ps_batch_profile(ps_batch *b, int reps, double *stage_ms, double 
     if (!b || reps < 1 || !stage_ms) return PS_ERR_INVALID;
     ps_ctx *c = b->c;
     HIPCK(hipSetDevice(c->device));
+    PSCK(batch_sync_refinement(b));  // the passes as the captured step runs them
     const int fp = c->fp64;
     const hipStream_t s = c->st;
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> marks;
@@ -2270,7 +2271,8 @@ extern "C" int ps_batch_profile(ps_batch *b, int reps, double *stage_ms, double 
                 });
                 timed(2, [&] {
                     return run_zc_fused(fp, M, w.dgroups, (int)w.groups.size(), w.dlegs_d, w.dsuffix, 0, w.items,
-                                        b->dpartials, b->max_chunks, b->grid_cap, w.kz_cap, 1, w.tw, s);
+                                        b->dpartials, b->max_chunks, b->grid_cap, w.kz_cap, b->abs_rec ? 3 : 1, w.tw,
+                                        s);
                 });
                 continue;
             }
@@ -2305,7 +2307,7 @@ extern "C" int ps_batch_profile(ps_batch *b, int reps, double *stage_ms, double 
             }
         }
         timed(b->nstages - 1, [&] {
-            double *outI = (b->method == PS_METHOD_DIRECT) ? nullptr : b->dsum + b->nout;
+            double *outI = (b->method == PS_METHOD_DIRECT) ? (b->abs_rec ? b->dabs : nullptr) : b->dsum + b->nout;
             return run_reduce_partials(b->dpartials, b->max_chunks, b->dcounts, 0, b->nout, b->dsum, outI, s);
         });
     }
