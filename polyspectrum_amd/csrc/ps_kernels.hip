@@ -1065,9 +1065,14 @@ k_zc_pairs(const GroupDev *__restrict__ groups, int ngroups, const LegDev *__res
 // global memory (whole 128-byte lines), and the first butterfly stage, which
 // reads consecutive kz of one line, finds them in distinct 16-byte bank
 // groups of every 256 bytes.
+//
+// Used where k_zc_pairs is, and at 128, where it measured faster than the
+// compact form of k_inv_zc_fused (Zc 0.410 -> 0.400 ms on the 512^3 sweep,
+// with three workgroups per CU against four).
 template <typename T, int N> constexpr bool zc_dma_ok()
 {
-    return zc_pair_ok<T, N>() && (ZcCfg<T, N, true>::SL % 8 == 0) && N >= 96;
+    return (zc_pair_ok<T, N>() || (sizeof(T) == 4 && zc_dmode_ok<T, N>() && N == 128)) &&
+           (ZcCfg<T, N, true>::SL % 8 == 0) && N >= 96;
 }
 
 // copy 16 bytes per active lane from g to LDS byte offset lds_base + 16 lane
@@ -1371,7 +1376,7 @@ k_inv_g2_dma(const LegDev *__restrict__ legs, int nlegs, int64_t t0, int64_t t1,
     constexpr int CPT = (CH + NT - 1) / NT;
     constexpr int HP = TLC / 2;      // pairs per band line of a tile
     static_assert(TLC % 2 == 0, "pairs of adjacent columns");
-    __shared__ cplx<T> tb[3][TOT];
+    __shared__ __align__(16) cplx<T> tb[3][TOT];
     __shared__ cplx<T> stw[N];
     const int tid = threadIdx.x, lane = tid & 63;
     const int w = __builtin_amdgcn_readfirstlane(tid / 64);
@@ -1432,11 +1437,15 @@ k_inv_g2_dma(const LegDev *__restrict__ legs, int nlegs, int64_t t0, int64_t t1,
             }
         }
     };
+    // 16-byte stores of two adjacent columns, read from LDS as one pair
+    // (element x of line l at x * TLC + l)
     auto put = [&](const Tile &t, const cplx<T> *buf) {
         if (t.o) {
-            tile_loop_u<TOT, 8>(tid, [&](int idx) {
-                const int l = idx % TLC, x = idx / TLC;
-                gstore<T>(t.o, ((int64_t)t.kz * N + x) * N + t.y0 + l, buf[lidx<TLC, T, false>(x, l)]);
+            const f32x4 *b4 = reinterpret_cast<const f32x4 *>(buf);
+            PS_GLOBAL f32x4 *o4 = (PS_GLOBAL f32x4 *)t.o;
+            tile_loop_u<CH, 8>(tid, [&](int c) {
+                const int x = c / HP, l = 2 * (c % HP);
+                o4[(((int64_t)t.kz * N + x) * N + t.y0 + l) / 2] = b4[c];
             });
         }
     };
