@@ -173,9 +173,10 @@ int ps_batch_create(ps_ctx *ctx, ps_polygon *polys, int npoly, double kbinwidth,
  * (DIRECT) or dot-pass steps (FOURIER) per work item (sets how many
  * compensated partial sums each polygon has), the divisor of the G2/Fx tile
  * width (1: 16 FP32 / 8 FP64 columns, 2: half), the schedule of the waves
- * (groups sharing one transform grid; overlap 1: waves side by side on
- * branch streams when their buffers fit together, 0: one after another,
- * reusing one buffer region) and the method (PS_METHOD_*). */
+ * (groups sharing one transform grid; overlap 1: waves side by side on up to
+ * 4 branch streams when their buffers fit together, 2 .. 4: on at most that
+ * many, 0: one after another, reusing one buffer region) and the method
+ * (PS_METHOD_*). */
 typedef struct ps_launch_params {
     int grid_cap;
     int chunk_tiles;
@@ -190,10 +191,12 @@ int ps_batch_create_ex(ps_ctx *ctx, ps_polygon *polys, int npoly, double kbinwid
                        const ps_launch_params *params, ps_batch **out);
 
 /* Autotuner: times candidate launch shapes on this batch (the fastest of reps
- * replays each), times the four fastest again (the median of 3 reps replays)
- * and keeps the best of those for later ps_batch_create calls of the same
- * batch shape (and appends it to the file PS_TUNE_FILE names, if set);
- * best_ms receives its median replay time. */
+ * executions each, on the host clock from the call to its results, with the
+ * refinement where the context refines), times the four fastest again (the
+ * median of 3 reps executions), then the best of those with the waves on at
+ * most 3 and 2 branch streams and one after another, and keeps the best for
+ * later ps_batch_create calls of the same batch shape (and appends it to the
+ * file PS_TUNE_FILE names, if set); best_ms receives its median time. */
 int ps_batch_autotune(ps_ctx *ctx, ps_polygon *polys, int npoly, double kbinwidth, int kbinning_case, int reps,
                       ps_launch_params *best, double *best_ms);
 int ps_batch_params(const ps_batch *b, ps_launch_params *out);

@@ -41,8 +41,9 @@ class Polygon(ctypes.Structure):
 
 class LaunchParams(ctypes.Structure):
     """ps_launch_params: persistent grid size, work-item size, G2/Fx tile
-    divisor, two-stream overlapped schedule (0/1, direct method) and the
-    method (METHOD_FOURIER / METHOD_DIRECT)."""
+    divisor, schedule of the waves (0: one after another; 1: side by side on
+    up to 4 branch streams; 2 .. 4: on at most that many) and the method
+    (METHOD_FOURIER / METHOD_DIRECT)."""
     _fields_ = [("grid_cap", ctypes.c_int), ("chunk_tiles", ctypes.c_int), ("g2_div", ctypes.c_int),
                 ("overlap", ctypes.c_int), ("method", ctypes.c_int)]
 

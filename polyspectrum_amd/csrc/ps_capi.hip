@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -1404,7 +1405,8 @@ static int plan_waves(ps_batch *b, std::vector<int> &counts)
         }
     }
     if (b->concurrent) {
-        const int nb = std::min((int)W.size(), 4);
+        // overlap 1: up to 4 branch streams; 2 .. 4: at most that many
+        const int nb = std::min((int)W.size(), b->overlap >= 2 ? b->overlap : 4);
         // waves to branches, largest first onto the least loaded branch; each
         // branch runs its waves smallest first (the small grids are latency
         // bound and overlap the large ones), and the capture takes the
@@ -1505,7 +1507,7 @@ extern "C" int ps_batch_create_ex(ps_ctx *c, ps_polygon *polys, int npoly, doubl
         if (it != g_tuned.end()) lp = it->second;
     }
     if (lp.grid_cap < 1 || lp.chunk_tiles < 1 || (lp.g2_div != 1 && lp.g2_div != 2) ||
-        (lp.overlap != 0 && lp.overlap != 1) || (lp.method != PS_METHOD_FOURIER && lp.method != PS_METHOD_DIRECT))
+        lp.overlap < 0 || lp.overlap > 4 || (lp.method != PS_METHOD_FOURIER && lp.method != PS_METHOD_DIRECT))
         return PS_ERR_INVALID;
     for (int p = 0; p < npoly; ++p) {
         if (polys[p].n < 2 || polys[p].n > 3) return PS_ERR_INVALID;
@@ -2165,8 +2167,10 @@ extern "C" int ps_batch_autotune(ps_ctx *c, ps_polygon *polys, int npoly, double
     const int divs[] = {1, 2};
     std::vector<ps_polygon> work(polys, polys + npoly);
     std::vector<double> v(npoly);
-    // one candidate, after a warm-up replay: the fastest of n replays (the
-    // least disturbed by other work on the device), or their median
+    // one candidate, after a warm-up execution: the fastest of n executions
+    // (the least disturbed by other work on the device), or their median;
+    // each timed on the host from the call to its results, as a caller sees
+    // it (with the FP64 refinement beside the step, where the context refines)
     auto trial = [&](const ps_launch_params &lp, double *ms, int n = 0, bool median = false) -> int {
         ps_batch *b = nullptr;
         std::copy(polys, polys + npoly, work.begin());
@@ -2176,8 +2180,9 @@ extern "C" int ps_batch_autotune(ps_ctx *c, ps_polygon *polys, int npoly, double
         std::vector<double> ts;
         s = ps_batch_execute(b, v.data(), nullptr, nullptr);
         for (int r = 0; r < n && s == PS_OK; ++r) {
+            const auto t0 = std::chrono::steady_clock::now();
             s = ps_batch_execute(b, v.data(), nullptr, nullptr);
-            ts.push_back(b->last_ms);
+            ts.push_back(std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
         }
         ps_batch_destroy(b);
         std::sort(ts.begin(), ts.end());
@@ -2214,9 +2219,14 @@ extern "C" int ps_batch_autotune(ps_ctx *c, ps_polygon *polys, int npoly, double
             bp = ranked[i].second;
         }
     }
-    {  // the waves one after another at the best shape
-        ps_launch_params lp = bp;
-        lp.overlap = 0;
+    // at the best shape: the waves on at most 3 or 2 branch streams, and one
+    // after another (fewer kernels side by side starve the ones with the
+    // largest LDS and register footprint less: 3 measured 3% faster than 4
+    // on the 512^3 sweep)
+    const ps_launch_params shape = bp;
+    for (int ov : {3, 2, 0}) {
+        ps_launch_params lp = shape;
+        lp.overlap = ov;
         double t = 0.0;
         const int s = trial(lp, &t, 3 * reps, true);
         if (s != PS_OK) return s;

@@ -144,8 +144,9 @@ def test_autotune_and_params(gpu_lib):
 @pytest.mark.parametrize("prec_name", ["PREC_FP64", "PREC_MIXED"])
 def test_overlap_schedule_matches(gpu_lib, prec_name):
     """Several polygon groups on different transform grids (three triangle
-    prefixes, power spectra): the waves run side by side on branch streams or
-    one after another, with the same values bit for bit, in both methods."""
+    prefixes, power spectra): the waves run side by side on up to 4, 2 or 3
+    branch streams or one after another, with the same values bit for bit, in
+    both methods."""
     P = gpu_lib
     N, L = 64, 100.0
     f = lognormal_field(N, 41)
@@ -156,15 +157,16 @@ def test_overlap_schedule_matches(gpu_lib, prec_name):
         out = {}
         with P.Context(N, L, getattr(P, prec_name)) as ctx:
             ctx.load_field(f)
-            for ov in (0, 1):
+            for ov in (0, 1, 2, 3):
                 b = ctx.batch(polys, 1.0, 2, params={"grid_cap": 512, "chunk_tiles": 16 if method else 2,
                                                      "g2_div": 1, "overlap": ov, "method": method})
                 out[ov] = b.execute()
                 again = b.execute()
                 b.close()
                 np.testing.assert_array_equal(out[ov][0], again[0])
-        np.testing.assert_array_equal(out[1][0], out[0][0])
-        np.testing.assert_array_equal(out[1][1], out[0][1])
+        for ov in (1, 2, 3):
+            np.testing.assert_array_equal(out[ov][0], out[0][0])
+            np.testing.assert_array_equal(out[ov][1], out[0][1])
 
 
 @pytest.mark.parametrize("prec_name", ["PREC_FP64", "PREC_MIXED"])
