@@ -579,6 +579,7 @@ struct ps_batch {
     MaskParams *dmasks = nullptr;        // suffix masks in suffix order (Fourier method)
     int2 *divl = nullptr;                // their m^2 intervals and counts
     void *arena = nullptr;
+    size_t arena_bytes = 0;
     int concurrent = 0;                  // waves have disjoint buffers and run on branch streams
     hipEvent_t fork = nullptr;
     std::vector<hipEvent_t> wend;        // per branch stream: its waves are done
@@ -679,10 +680,10 @@ static int record_wave(ps_batch *b, Wave &w, hipStream_t ws)
     return PS_OK;
 }
 
-// Mixed precision: the I sums taken with FP64 transforms of the bare masks on
-// temporary FP64 buffers, wave by wave, so that the polygon counts are exact
-// integers as the reference's (FP32 transforms leave a relative error of about
-// 1e-7: a few polygons in 3.5e7).  The FP64 tiles and work items are numbered
+// Mixed precision: the I sums taken with FP64 transforms of the bare masks in
+// the batch's arena (before its first use by an execution), wave by wave, so
+// that the polygon counts are exact integers as the reference's (FP32
+// transforms leave a relative error of about 1e-7: a few polygons in 3.5e7).  The FP64 tiles and work items are numbered
 // here; the I sums go to dsum[nout ..].
 static int batch_i_sums_fp64(ps_batch *b)
 {
@@ -709,7 +710,7 @@ static int batch_i_sums_fp64(ps_batch *b)
     for (size_t wi = 0; wi < b->waves.size() && st == PS_OK; ++wi) {
         const Wave &w = b->waves[wi];
         const int M = w.M, nl = (int)w.legs.size(), ng = (int)w.groups.size();
-        void *tw = nullptr, *arena = nullptr, *dlegs = nullptr, *dgroups = nullptr;
+        void *tw = nullptr, *arena = nullptr, *own = nullptr, *dlegs = nullptr, *dgroups = nullptr;
         std::vector<LegDev> legs(w.legs);
         std::vector<GroupDev> groups(w.groups);
         const int tl = g_tile_lines(M, 1);
@@ -733,7 +734,12 @@ static int batch_i_sums_fp64(ps_batch *b)
             G.item_begin = items;
             items += nchunks;
         }
-        if (make_twiddles<double>(M, &tw) != PS_OK || hipMalloc(&arena, std::max<size_t>(bytes, 16)) != hipSuccess ||
+        // the wave's FP64 I buffers take as many bytes as its mixed D and I
+        // buffers: they go into the batch's arena, unused until the first
+        // execution (freeing a temporary buffer of 1.9 GB took up to 51 ms)
+        if (bytes <= b->arena_bytes) arena = b->arena;
+        else if (hipMalloc(&own, bytes) == hipSuccess) arena = own;
+        if (make_twiddles<double>(M, &tw) != PS_OK || !arena ||
             hipMalloc(&dlegs, sizeof(LegDev) * std::max(nl, 1)) != hipSuccess ||
             hipMalloc(&dgroups, sizeof(GroupDev) * std::max(ng, 1)) != hipSuccess) {
             st = PS_ERR_NOMEM;
@@ -759,7 +765,7 @@ static int batch_i_sums_fp64(ps_batch *b)
             if (st > 0) st = PS_ERR_HIP;
             if (hipStreamSynchronize(c->st) != hipSuccess && st == PS_OK) st = PS_ERR_HIP;
         }
-        void *bufs[] = {tw, arena, dlegs, dgroups};
+        void *bufs[] = {tw, own, dlegs, dgroups};
         for (void *q : bufs)
             if (q) (void)hipFree(q);
     }
@@ -1335,6 +1341,7 @@ static int plan_waves(ps_batch *b, std::vector<int> &counts)
     if (b->concurrent) arena = total_bytes;
     // ---- buffers and device tables
     if (arena > 0) PCK(hipMalloc(&b->arena, arena));
+    b->arena_bytes = arena;
     // amplitude scales: the direct method in mixed precision (its last pass
     // packs D + i I of one leg); the Fourier method in both precisions (its
     // forward transform packs parts of the prefix product, P_D ~ |H|^2 P_I)
