@@ -1247,6 +1247,13 @@ k_zc_dma(const GroupDev *__restrict__ groups, int ngroups, const LegDev *__restr
                 if (edge) wave_fft<T, N, LW, +1, 0, true, RegTw<T, N, LW>>(wb, tw, lane, src, sink);
                 else wave_fft<T, N, LW, +1, 0, false, RegTw<T, N, LW>>(wb, tw, lane, src, sink);
             };
+            // the magnitude sums on every kAbsStride-th x plane (an estimate)
+            const bool ab = ABS && ((int)(t0s / tiles_per_x) % kAbsStride == 0);
+            // Parseval form (the kernel without magnitude sums: in the one
+            // with them, the two paths side by side cost a workgroup per CU
+            // at 144 and spilled at 192): a group with a prefix and plain
+            // suffixes sums each suffix leg in the spectrum
+            const bool pv = !ABS && G.npre >= 1 && !G.suf_sq;
             if (q < G.npre) {
                 auto sink1 = [&](int i, int r, int, int, cplx<T> v) {
                     pd[i][r] *= v.x;
@@ -1258,6 +1265,73 @@ k_zc_dma(const GroupDev *__restrict__ groups, int ngroups, const LegDev *__restr
                 };
                 if (G.pre_sq) run(sink2);
                 else run(sink1);
+                if (pv && q == G.npre - 1) {
+                    // F = forward transform of Q = P(row a) + i P(row b) along
+                    // z, into this wavefront's buffer (element k of line l at
+                    // l LP + pad(k)) for the suffix steps of the tile
+                    using Lay = WLay<T, N>;
+                    using LS = WaveStage<T, N, LW, num_stages(N) - 1>;
+                    wave_sync();
+#pragma unroll
+                    for (int i = 0; i < WO::IT; ++i) {
+                        const int bb = lane + 64 * i;
+                        if (LS::TOT % 64 == 0 || bb < LS::TOT) {
+                            const int j = bb % LS::NB, l = bb / LS::NB;
+#pragma unroll
+                            for (int r = 0; r < WO::R; ++r) {
+                                wb[l * LP + Lay::pad(j + r * LS::NB)] = mk<T>(pd[i][r], pi[i][r]);
+                                pd[i][r] = (T)1;  // (not read again in this tile: frees the registers)
+                                pi[i][r] = (T)1;
+                            }
+                        }
+                    }
+                    wave_sync();
+                    // the first stage reads all its inputs before it writes;
+                    // the last stage's outputs go to the positions its own
+                    // reads came from, after them (one wavefront's LDS
+                    // instructions complete in order)
+                    auto srcq = [&](int l, int e) -> cplx<T> { return wb[l * LP + Lay::pad(e)]; };
+                    auto sinkq = [&](int, int, int l, int k, cplx<T> v) { wb[l * LP + Lay::pad(k)] = v; };
+                    const RegTwConj<T, N, LW> ctw{twr};
+                    wave_fft<T, N, LW, -1, 0, false, RegTwConj<T, N, LW>>(wb, ctw, lane, srcq, sinkq);
+                    wave_sync();
+                }
+            } else if (pv) {
+                // sum_z P(a) D(a) + P(b) D(b) = Re sum_e F(e) conj(E(e)), E
+                // the staged leg's complex line spectrum D(a) + i D(b):
+                // E(k) = (x - w, y + z), E(N - k) = (x + w, z - y) for the
+                // raw pair (x, y, z, w) = (D(a)[k], D(b)[k]) of plane k
+                // (imaginary parts dropped at k = 0 and N / 2)
+                using Lay = WLay<T, N>;
+                double sd = 0.0;
+                constexpr int KS = (N / 2 + 64) / 64;  // planes k = lane + 64 s, k <= N / 2
+#pragma unroll 1
+                for (int s2 = 0; s2 < KS; ++s2) {
+                    const int k = lane + 64 * s2;
+                    if (k <= Kz) {
+                        const bool edge_k = (k == 0) || (2 * k == N);
+#pragma unroll 1
+                        for (int l = 0; l < LW; ++l) {
+                            const int line = w * LW + l;
+                            const f32x4 v = sb[k * SL + ((line + k) & (SL - 1))];
+                            const cplx<T> A = wb[l * LP + Lay::pad(k)];
+                            const cplx<T> B = edge_k ? mk<T>(0, 0) : wb[l * LP + Lay::pad(N - k)];
+                            const T vy = edge_k ? (T)0 : v.y, vw = edge_k ? (T)0 : v.w;
+                            const T t = v.x * (A.x + B.x) + vy * (A.y - B.y) + v.z * (A.y + B.y) + vw * (B.x - A.x);
+                            sd += (double)t;
+                        }
+                    }
+                }
+                sd = wave_sum_dpp(sd);
+                if (lane == 0) {
+                    const int o = q - G.npre;
+                    KSum a;
+                    a.s = acc[w][o][0];
+                    a.c = acc[w][o][1];
+                    a.add(sd);
+                    acc[w][o][0] = a.s;
+                    acc[w][o][1] = a.c;
+                }
             } else {
                 double sd = 0.0, sa = 0.0;
                 auto sink1 = [&](auto with_abs) {
@@ -1281,8 +1355,6 @@ k_zc_dma(const GroupDev *__restrict__ groups, int ngroups, const LegDev *__restr
                         }
                     };
                 };
-                // the magnitude sums on every kAbsStride-th x plane (an estimate)
-                const bool ab = ABS && ((int)(t0s / tiles_per_x) % kAbsStride == 0);
                 if (ab) {
                     auto s1 = sink1(std::true_type{});
                     auto s2 = sink2(std::true_type{});
