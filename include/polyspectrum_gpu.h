@@ -99,6 +99,7 @@ int ps_grid_supported(int nbins);
  * Replaces: initGrid_with_values grid.c:47-84, allocate_3D_array_fftw_complex
  * utils_fftw.c:18-30. */
 int ps_ctx_create(int nbins, double boxsize, int precision, int nslots, ps_ctx **out);
+/* Batches and multi-batches refer to their context: destroy them first. */
 int ps_ctx_destroy(ps_ctx *ctx);
 
 /* Upload a real N^3 field (float32 if field_is_double == 0, float64

@@ -9,6 +9,7 @@ without a GPU, every entry point raises.
 """
 import ctypes
 import os
+import weakref
 
 import numpy as np
 
@@ -201,9 +202,18 @@ class Context:
         self.boxsize = float(boxsize)
         self.precision = precision
         self.nslots = nslots
+        self._dependents = weakref.WeakSet()  # batches (and multi-batches over them) of this context
 
     def close(self):
+        """Destroys the context, after every batch of it still open (a
+        batch must not outlive its context: ps_batch_destroy reads it)."""
         if getattr(self, "_h", None):
+            deps = list(getattr(self, "_dependents", ()))
+            for d in deps:
+                if isinstance(d, MultiBatch):
+                    d.close()
+            for d in deps:
+                d.close()
             load_library().ps_ctx_destroy(self._h)
             self._h = None
 
@@ -377,6 +387,7 @@ class Batch:
                                                  ctypes.byref(lp) if lp is not None else None, ctypes.byref(h)))
         self._h = h
         self._ctx = ctx  # keeps the context alive
+        ctx._dependents.add(self)
         self.polygons = arr
         self.n = len(polygons)
 
@@ -505,6 +516,8 @@ class MultiBatch:
         _check(load_library().ps_multi_create(arr, len(batches), ctypes.byref(h)))
         self._h = h
         self.batches = list(batches)
+        for b in self.batches:
+            b._ctx._dependents.add(self)
 
     def execute(self):
         """Replays the graph; returns [(values, npolygons, sums)] per batch."""

@@ -545,7 +545,9 @@ template <typename T, int N> constexpr int zc_wave_rows() { return (sizeof(T) ==
 
 // D-only passes with magnitude sums (ABS): sum |prod D| is taken on the x
 // planes x = 0 mod kAbsStride only and scaled by kAbsStride (an estimate of
-// the condition number's numerator; every supported grid is a multiple of 4)
+// the condition number's numerator; every supported grid is a multiple of 4).
+// The copy-pipelined pass (k_zc_dma) takes an upper bound instead, on every
+// tile: sqrt(sum P^2 sum D^2) per wavefront and leg (Cauchy-Schwarz).
 constexpr int kAbsStride = 4;
 
 template <typename T, int N, bool DM = false> struct ZcCfg {
@@ -1107,10 +1109,11 @@ __device__ __forceinline__ void wait_vm_at_most(int n)
 }
 
 // NBUF staging buffers: copies are requested NBUF - 1 steps ahead (3: two
-// steps; 2: one step, with the registers capped for four waves per SIMD)
+// steps; 2: one step), with the registers capped for four waves per SIMD
+// where the LDS allows four workgroups per CU (two buffers, or three at 144)
 template <typename T, int N, bool ABS = false, int NBUF = 3>
 __global__ void __launch_bounds__((ZcCfg<T, N, true>::BT))
-__attribute__((amdgpu_waves_per_eu(NBUF == 2 ? 4 : zc_waves<T, N, true>())))
+__attribute__((amdgpu_waves_per_eu((NBUF == 2 || N == 144) ? 4 : zc_waves<T, N, true>())))
 k_zc_dma(const GroupDev *__restrict__ groups, int ngroups, const LegDev *__restrict__ legs,
          const int *__restrict__ suffix, int64_t i0, int64_t i1, double2 *__restrict__ partials, int max_chunks,
          const cplx<T> *__restrict__ twg)
@@ -1209,6 +1212,7 @@ k_zc_dma(const GroupDev *__restrict__ groups, int ngroups, const LegDev *__restr
         issue(t0s, q0, 0);
         if (NBUF == 3 && 1 < nsteps) n1 = issue(t1s, q1, 1);
         T pd[WO::IT][WO::R], pi[WO::IT][WO::R];
+        double pp = 0.0;  // (magnitude bound) sum of P^2 over this wavefront's rows of the tile
         for (int64_t s = 0; s < nsteps; ++s) {
             const int b = (int)(s % NBUF);
             // this wavefront's copies of step s are in LDS once at most the
@@ -1247,13 +1251,11 @@ k_zc_dma(const GroupDev *__restrict__ groups, int ngroups, const LegDev *__restr
                 if (edge) wave_fft<T, N, LW, +1, 0, true, RegTw<T, N, LW>>(wb, tw, lane, src, sink);
                 else wave_fft<T, N, LW, +1, 0, false, RegTw<T, N, LW>>(wb, tw, lane, src, sink);
             };
-            // the magnitude sums on every kAbsStride-th x plane (an estimate)
-            const bool ab = ABS && ((int)(t0s / tiles_per_x) % kAbsStride == 0);
-            // Parseval form (the kernel without magnitude sums: in the one
-            // with them, the two paths side by side cost a workgroup per CU
-            // at 144 and spilled at 192): a group with a prefix and plain
-            // suffixes sums each suffix leg in the spectrum
-            const bool pv = !ABS && G.npre >= 1 && !G.suf_sq;
+            // Parseval form: a group with a prefix and plain suffixes sums
+            // each suffix leg in the spectrum (and bounds sum |prod D| by
+            // Cauchy-Schwarz where magnitude sums are taken); the others sum
+            // squares (power spectra: sum |prod D| is the sum itself)
+            const bool pv = G.npre >= 1 && !G.suf_sq;
             if (q < G.npre) {
                 auto sink1 = [&](int i, int r, int, int, cplx<T> v) {
                     pd[i][r] *= v.x;
@@ -1271,6 +1273,17 @@ k_zc_dma(const GroupDev *__restrict__ groups, int ngroups, const LegDev *__restr
                     // l LP + pad(k)) for the suffix steps of the tile
                     using Lay = WLay<T, N>;
                     using LS = WaveStage<T, N, LW, num_stages(N) - 1>;
+                    if constexpr (ABS) {
+                        float q2 = 0.f;
+#pragma unroll
+                        for (int i = 0; i < WO::IT; ++i) {
+                            const int bb = lane + 64 * i;
+                            if (LS::TOT % 64 == 0 || bb < LS::TOT)
+#pragma unroll
+                                for (int r = 0; r < WO::R; ++r) q2 += pd[i][r] * pd[i][r] + pi[i][r] * pi[i][r];
+                        }
+                        pp = wave_sum_dpp((double)q2);
+                    }
                     wave_sync();
 #pragma unroll
                     for (int i = 0; i < WO::IT; ++i) {
@@ -1304,6 +1317,7 @@ k_zc_dma(const GroupDev *__restrict__ groups, int ngroups, const LegDev *__restr
                 // (imaginary parts dropped at k = 0 and N / 2)
                 using Lay = WLay<T, N>;
                 double sd = 0.0;
+                float dd = 0.f;  // (magnitude bound) sum_k w_k |E|^2 terms: sum_z D^2 / N
                 constexpr int KS = (N / 2 + 64) / 64;  // planes k = lane + 64 s, k <= N / 2
 #pragma unroll 1
                 for (int s2 = 0; s2 < KS; ++s2) {
@@ -1319,12 +1333,17 @@ k_zc_dma(const GroupDev *__restrict__ groups, int ngroups, const LegDev *__restr
                             const T vy = edge_k ? (T)0 : v.y, vw = edge_k ? (T)0 : v.w;
                             const T t = v.x * (A.x + B.x) + vy * (A.y - B.y) + v.z * (A.y + B.y) + vw * (B.x - A.x);
                             sd += (double)t;
+                            if constexpr (ABS) dd += (edge_k ? 1.f : 2.f) * (v.x * v.x + vy * vy + v.z * v.z + vw * vw);
                         }
                     }
                 }
                 sd = wave_sum_dpp(sd);
+                // sum over this wavefront's rows and z of |P D| <= sqrt(sum P^2 sum D^2)
+                double sa = 0.0;
+                if constexpr (ABS) sa = sqrt(pp * (double)N * wave_sum_dpp((double)dd));
                 if (lane == 0) {
                     const int o = q - G.npre;
+                    if constexpr (ABS) acca[w][o] += (float)sa;
                     KSum a;
                     a.s = acc[w][o][0];
                     a.c = acc[w][o][1];
@@ -1333,44 +1352,22 @@ k_zc_dma(const GroupDev *__restrict__ groups, int ngroups, const LegDev *__restr
                     acc[w][o][1] = a.c;
                 }
             } else {
-                double sd = 0.0, sa = 0.0;
-                auto sink1 = [&](auto with_abs) {
-                    return [&](int i, int r, int, int, cplx<T> v) {
-                        sd += (double)pd[i][r] * (double)v.x;
-                        sd += (double)pi[i][r] * (double)v.y;
-                        if constexpr (decltype(with_abs)::value) {
-                            sa += fabs((double)pd[i][r]) * fabs((double)v.x);
-                            sa += fabs((double)pi[i][r]) * fabs((double)v.y);
-                        }
-                    };
+                // squared suffixes (power spectra): the suffix leg transformed
+                double sd = 0.0;
+                auto sink1 = [&](int i, int r, int, int, cplx<T> v) {
+                    sd += (double)pd[i][r] * (double)v.x;
+                    sd += (double)pi[i][r] * (double)v.y;
                 };
-                auto sink2 = [&](auto with_abs) {
-                    return [&](int i, int r, int, int, cplx<T> v) {
-                        const double dv = (double)v.x, iv = (double)v.y;
-                        sd += (double)pd[i][r] * (dv * dv);
-                        sd += (double)pi[i][r] * (iv * iv);
-                        if constexpr (decltype(with_abs)::value) {
-                            sa += fabs((double)pd[i][r]) * (dv * dv);
-                            sa += fabs((double)pi[i][r]) * (iv * iv);
-                        }
-                    };
+                auto sink2 = [&](int i, int r, int, int, cplx<T> v) {
+                    const double dv = (double)v.x, iv = (double)v.y;
+                    sd += (double)pd[i][r] * (dv * dv);
+                    sd += (double)pi[i][r] * (iv * iv);
                 };
-                if (ab) {
-                    auto s1 = sink1(std::true_type{});
-                    auto s2 = sink2(std::true_type{});
-                    if (G.suf_sq) run(s2);
-                    else run(s1);
-                } else {
-                    auto s1 = sink1(std::false_type{});
-                    auto s2 = sink2(std::false_type{});
-                    if (G.suf_sq) run(s2);
-                    else run(s1);
-                }
+                if (G.suf_sq) run(sink2);
+                else run(sink1);
                 sd = wave_sum_dpp(sd);
-                if (ab) sa = wave_sum_dpp(sa);
                 if (lane == 0) {
                     const int o = q - G.npre;
-                    if constexpr (ABS) acca[w][o] += (float)sa;
                     KSum a;
                     a.s = acc[w][o][0];
                     a.c = acc[w][o][1];
@@ -1409,7 +1406,10 @@ k_zc_dma(const GroupDev *__restrict__ groups, int ngroups, const LegDev *__restr
             double sabs = 0.0;
             if constexpr (ABS)
                 for (int ww = 0; ww < NWV; ++ww) sabs += (double)acca[ww][o];
-            partials[2 * po + 1] = make_double2(sabs * f * kAbsStride, 0.0);
+            // the Cauchy-Schwarz bounds of every tile (Parseval form), or the
+            // sum of squares itself
+            if (!(G.npre >= 1 && !G.suf_sq)) sabs = fabs(a.s + a.c);
+            partials[2 * po + 1] = make_double2(sabs * f, 0.0);
         }
         __syncthreads();
     }

@@ -356,3 +356,24 @@ def test_persistent_grid_sizes_agree(gpu_lib, prec_name):
         np.testing.assert_array_equal(out[(3, div)][0], out[(4096, div)][0])
         np.testing.assert_array_equal(out[(3, div)][1], out[(4096, div)][1])
     np.testing.assert_allclose(out[(3, 1)][0], out[(3, 2)][0], rtol=1e-12 if prec_name == "PREC_FP64" else 1e-5)
+
+
+def test_context_close_closes_its_batches(gpu_lib):
+    """Closing a context first closes the batches and multi-batches still
+    open on it; their later close (or collection) does nothing."""
+    P = gpu_lib
+    N, L = 32, 100.0
+    kf = 2 * math.pi / L
+    polys = _polys_triangles(kf, [(3.0, 3.0)], [0.4, 1.3], 2)
+    ctx = P.Context(N, L, P.PREC_FP64)
+    ctx.load_field(lognormal_field(N, 43))
+    b1 = ctx.batch(polys, 1.0, 2)
+    b2 = ctx.batch(polys, 1.0, 2)
+    v1 = b1.execute()[0]
+    m = P.MultiBatch([b1, b2])
+    out = m.execute()
+    np.testing.assert_array_equal(out[0][0], v1)
+    ctx.close()
+    assert b1._h is None and b2._h is None and m._h is None
+    b1.close()
+    m.close()
