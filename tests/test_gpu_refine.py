@@ -146,3 +146,32 @@ def test_config_512_sweep_within_1e5_with_refinement(gpu_lib):
     np.testing.assert_allclose(v[flags], v64[flags], rtol=1e-12, atol=0)
     assert np.all(rel[~flags] <= est[~flags])
     np.testing.assert_array_equal(v_again, v)  # replays (and the cached FP64 set) reproduce bit for bit
+
+
+def test_magnitude_sums_not_below_kappa_numerator(gpu_lib):
+    """The refinement's numerator of kappa (sum |prod D| on sampled planes, or
+    its Cauchy-Schwarz bound in the copy-pipelined pass) against the exact
+    sum |prod D| of the FP64 reference-shaped path, on the benchmark sweep:
+    the bound is never below it, the samples within their sampling error."""
+    P = gpu_lib
+    import bench
+    N, L = 512, 300.0
+    field = lognormal_field(N, 2000, sigma=1.0, alpha=1.5)
+    polys = bench.triangles()
+    case = P.BINNING["GIVEN_BINNING"]
+    with P.Context(N, L, P.PREC_FP64) as c64:
+        c64.load_field(field)
+        kappa = c64.condition_numbers(polys, 1.0, case)
+    with P.Context(N, L, P.PREC_MIXED) as ctx:
+        ctx.set_refinement(math.inf)  # estimates only
+        ctx.load_field(field)
+        b = ctx.batch(polys, 1.0, case)
+        _, npol, _ = b.execute()
+        _, est = b.refinement()
+        b.close()
+    u32 = 2.0 ** -24
+    kappa_used = (est / u32 - 16.0) * np.sqrt(np.maximum(npol, 1.0)) / 32.0
+    ratio = kappa_used / kappa
+    print(f"numerator used / exact: min {ratio.min():.3f} median {np.median(ratio):.3f} max {ratio.max():.3f}")
+    assert np.all(ratio >= 0.8), (np.argmin(ratio), ratio.min())
+    assert np.median(ratio) >= 1.0
