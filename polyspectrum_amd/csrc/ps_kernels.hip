@@ -543,12 +543,13 @@ __global__ void __launch_bounds__(NT) k_inv_zc_plain(const cplx<T> *__restrict__
 // (6 rows of 144 = 3 paired lines held 212 VGPRs, two waves per SIMD)
 template <typename T, int N> constexpr int zc_wave_rows() { return (sizeof(T) == 4 && N == 144) ? 4 : wave_lines<T, N>(); }
 
-// D-only passes with magnitude sums (ABS): sum |prod D| is taken on the x
-// planes x = 0 mod kAbsStride only and scaled by kAbsStride (an estimate of
-// the condition number's numerator; every supported grid is a multiple of 4).
-// The copy-pipelined pass (k_zc_dma) takes an upper bound instead, on every
-// tile: sqrt(sum P^2 sum D^2) per wavefront and leg (Cauchy-Schwarz).
-constexpr int kAbsStride = 4;
+// D-only passes with magnitude sums (ABS): sum |prod D| is taken on every x
+// plane of grids below 96 (exact), and on the planes x = 0 mod 4 of larger
+// ones, scaled by 4 (an estimate of the condition number's numerator; every
+// supported grid is a multiple of 4).  The copy-pipelined pass (k_zc_dma)
+// takes an upper bound instead, on every tile: sqrt(sum P^2 sum D^2) per
+// wavefront and leg (Cauchy-Schwarz).
+template <int N> constexpr int abs_stride() { return N >= 96 ? 4 : 1; }
 
 template <typename T, int N, bool DM = false> struct ZcCfg {
     static constexpr int LW0 = zc_wave_rows<T, N>();  // y rows per wavefront
@@ -757,8 +758,8 @@ __global__ void __launch_bounds__((ZcCfg<T, N, DM>::BT)) __attribute__((amdgpu_w
                         }
                     };
                 };
-                // the magnitude sums on every kAbsStride-th x plane (an estimate)
-                const bool ab = DM && ABS && ((int)(tcur / tiles_per_x) % kAbsStride == 0);
+                // the magnitude sums on every abs_stride-th x plane
+                const bool ab = DM && ABS && ((int)(tcur / tiles_per_x) % abs_stride<N>() == 0);
                 if (ab) {
                     auto s1 = sink1(std::true_type{});
                     auto s2 = sink2(std::true_type{});
@@ -809,7 +810,7 @@ __global__ void __launch_bounds__((ZcCfg<T, N, DM>::BT)) __attribute__((amdgpu_w
             const int64_t po = (int64_t)(G.out_begin + o) * max_chunks + chunk;
             partials[2 * po + 0] = make_double2(a.s * f, a.c * f);
             // I parts: the grid ratio only; |prod D| (ABS): the D part's scale
-            const double fi = (DM && ABS) ? f * kAbsStride : g;
+            const double fi = (DM && ABS) ? f * abs_stride<N>() : g;
             partials[2 * po + 1] = make_double2(c.s * fi, c.c * fi);
         }
         __syncthreads();
@@ -982,8 +983,8 @@ k_zc_pairs(const GroupDev *__restrict__ groups, int ngroups, const LegDev *__res
                         }
                     };
                 };
-                // the magnitude sums on every kAbsStride-th x plane (an estimate)
-                const bool ab = ABS && ((int)(tcur / tiles_per_x) % kAbsStride == 0);
+                // the magnitude sums on every abs_stride-th x plane
+                const bool ab = ABS && ((int)(tcur / tiles_per_x) % abs_stride<N>() == 0);
                 if (ab) {
                     auto s1 = sink1(std::true_type{});
                     auto s2 = sink2(std::true_type{});
@@ -1037,7 +1038,7 @@ k_zc_pairs(const GroupDev *__restrict__ groups, int ngroups, const LegDev *__res
             double sabs = 0.0;
             if constexpr (ABS)
                 for (int ww = 0; ww < NWV; ++ww) sabs += (double)acca[ww][o];
-            partials[2 * po + 1] = make_double2(sabs * f * kAbsStride, 0.0);
+            partials[2 * po + 1] = make_double2(sabs * f * abs_stride<N>(), 0.0);
         }
         __syncthreads();
     }

@@ -149,10 +149,11 @@ def test_config_512_sweep_within_1e5_with_refinement(gpu_lib):
 
 
 def test_magnitude_sums_not_below_kappa_numerator(gpu_lib):
-    """The refinement's numerator of kappa (sum |prod D| on sampled planes, or
-    its Cauchy-Schwarz bound in the copy-pipelined pass) against the exact
-    sum |prod D| of the FP64 reference-shaped path, on the benchmark sweep:
-    the bound is never below it, the samples within their sampling error."""
+    """The refinement's numerator of kappa (sum |prod D| itself on grids
+    below 96, its Cauchy-Schwarz bound in the copy-pipelined pass on the
+    sweep's larger grids) against the exact sum |prod D| of the FP64
+    reference-shaped path, on the benchmark sweep: never below it (up to the
+    mixed-precision sum in kappa's denominator)."""
     P = gpu_lib
     import bench
     N, L = 512, 300.0
@@ -173,5 +174,5 @@ def test_magnitude_sums_not_below_kappa_numerator(gpu_lib):
     kappa_used = (est / u32 - 16.0) * np.sqrt(np.maximum(npol, 1.0)) / 32.0
     ratio = kappa_used / kappa
     print(f"numerator used / exact: min {ratio.min():.3f} median {np.median(ratio):.3f} max {ratio.max():.3f}")
-    assert np.all(ratio >= 0.8), (np.argmin(ratio), ratio.min())
+    assert np.all(ratio >= 0.99), (np.argmin(ratio), ratio.min())
     assert np.median(ratio) >= 1.0
