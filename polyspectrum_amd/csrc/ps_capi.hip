@@ -966,10 +966,13 @@ static ps_launch_params default_params(const ps_ctx *c)
     lp.grid_cap = 16 * c->cus;
     lp.method = c->method;
     lp.chunk_tiles = 1;  // dot steps (Fourier) or Zc tiles (direct) per work item
-    // the autotuner's usual picks on the 512^3 sweep: half-width G2 tiles in
-    // FP32 and the waves on at most 3 branch streams (refined step 0.939 ms
-    // against 0.974-0.981 ms with g2_div 1 and 4 streams, 4096 workgroups)
-    lp.g2_div = c->fp64 ? 1 : 2;
+    // G2 tiles of full width where the band fits the prefetch (half-width
+    // tiles everywhere took the realisation graph of 128-256^3 fields from
+    // 2.35-2.39 to 2.61-2.63 ms; the autotuner picks them where they pay);
+    // the waves on at most 3 branch streams (the 512^3 sweep's refined step
+    // 0.955-0.957 ms against 0.974-0.981 ms with 4, at these tile widths and
+    // 4096 workgroups)
+    lp.g2_div = 1;
     lp.overlap = 3;
     if (const char *e = getenv("PS_CHUNK_TILES")) lp.chunk_tiles = std::max(1, atoi(e));
     if (const char *e = getenv("PS_GRID_CAP")) lp.grid_cap = std::max(1, atoi(e));
