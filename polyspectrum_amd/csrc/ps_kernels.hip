@@ -543,13 +543,11 @@ __global__ void __launch_bounds__(NT) k_inv_zc_plain(const cplx<T> *__restrict__
 // (6 rows of 144 = 3 paired lines held 212 VGPRs, two waves per SIMD)
 template <typename T, int N> constexpr int zc_wave_rows() { return (sizeof(T) == 4 && N == 144) ? 4 : wave_lines<T, N>(); }
 
-// D-only passes with magnitude sums (ABS): sum |prod D| is taken on every x
-// plane of grids below 96 (exact), and on the planes x = 0 mod 4 of larger
-// ones, scaled by 4 (an estimate of the condition number's numerator; every
-// supported grid is a multiple of 4).  The copy-pipelined pass (k_zc_dma)
-// takes an upper bound instead, on every tile: sqrt(sum P^2 sum D^2) per
-// wavefront and leg (Cauchy-Schwarz).
-template <int N> constexpr int abs_stride() { return N >= 96 ? 4 : 1; }
+// D-only passes with magnitude sums (ABS): sum |prod D|, the condition
+// number's numerator, is taken on every x plane of grids up to 192 (exact),
+// and on the planes x = 0 mod 4 of larger ones, scaled by 4 (an estimate;
+// every supported grid is a multiple of 4)
+template <int N> constexpr int abs_stride() { return N <= 192 ? 1 : 4; }
 
 template <typename T, int N, bool DM = false> struct ZcCfg {
     static constexpr int LW0 = zc_wave_rows<T, N>();  // y rows per wavefront
@@ -1213,7 +1211,6 @@ k_zc_dma(const GroupDev *__restrict__ groups, int ngroups, const LegDev *__restr
         issue(t0s, q0, 0);
         if (NBUF == 3 && 1 < nsteps) n1 = issue(t1s, q1, 1);
         T pd[WO::IT][WO::R], pi[WO::IT][WO::R];
-        double pp = 0.0;  // (magnitude bound) sum of P^2 over this wavefront's rows of the tile
         for (int64_t s = 0; s < nsteps; ++s) {
             const int b = (int)(s % NBUF);
             // this wavefront's copies of step s are in LDS once at most the
@@ -1252,11 +1249,6 @@ k_zc_dma(const GroupDev *__restrict__ groups, int ngroups, const LegDev *__restr
                 if (edge) wave_fft<T, N, LW, +1, 0, true, RegTw<T, N, LW>>(wb, tw, lane, src, sink);
                 else wave_fft<T, N, LW, +1, 0, false, RegTw<T, N, LW>>(wb, tw, lane, src, sink);
             };
-            // Parseval form: a group with a prefix and plain suffixes sums
-            // each suffix leg in the spectrum (and bounds sum |prod D| by
-            // Cauchy-Schwarz where magnitude sums are taken); the others sum
-            // squares (power spectra: sum |prod D| is the sum itself)
-            const bool pv = G.npre >= 1 && !G.suf_sq;
             if (q < G.npre) {
                 auto sink1 = [&](int i, int r, int, int, cplx<T> v) {
                     pd[i][r] *= v.x;
@@ -1268,107 +1260,47 @@ k_zc_dma(const GroupDev *__restrict__ groups, int ngroups, const LegDev *__restr
                 };
                 if (G.pre_sq) run(sink2);
                 else run(sink1);
-                if (pv && q == G.npre - 1) {
-                    // F = forward transform of Q = P(row a) + i P(row b) along
-                    // z, into this wavefront's buffer (element k of line l at
-                    // l LP + pad(k)) for the suffix steps of the tile
-                    using Lay = WLay<T, N>;
-                    using LS = WaveStage<T, N, LW, num_stages(N) - 1>;
-                    if constexpr (ABS) {
-                        float q2 = 0.f;
-#pragma unroll
-                        for (int i = 0; i < WO::IT; ++i) {
-                            const int bb = lane + 64 * i;
-                            if (LS::TOT % 64 == 0 || bb < LS::TOT)
-#pragma unroll
-                                for (int r = 0; r < WO::R; ++r) q2 += pd[i][r] * pd[i][r] + pi[i][r] * pi[i][r];
+            } else {
+                double sd = 0.0, sa = 0.0;
+                auto sink1 = [&](auto with_abs) {
+                    return [&](int i, int r, int, int, cplx<T> v) {
+                        sd += (double)pd[i][r] * (double)v.x;
+                        sd += (double)pi[i][r] * (double)v.y;
+                        if constexpr (decltype(with_abs)::value) {
+                            sa += fabs((double)pd[i][r]) * fabs((double)v.x);
+                            sa += fabs((double)pi[i][r]) * fabs((double)v.y);
                         }
-                        pp = wave_sum_dpp((double)q2);
-                    }
-                    wave_sync();
-#pragma unroll
-                    for (int i = 0; i < WO::IT; ++i) {
-                        const int bb = lane + 64 * i;
-                        if (LS::TOT % 64 == 0 || bb < LS::TOT) {
-                            const int j = bb % LS::NB, l = bb / LS::NB;
-#pragma unroll
-                            for (int r = 0; r < WO::R; ++r) {
-                                wb[l * LP + Lay::pad(j + r * LS::NB)] = mk<T>(pd[i][r], pi[i][r]);
-                                pd[i][r] = (T)1;  // (not read again in this tile: frees the registers)
-                                pi[i][r] = (T)1;
-                            }
+                    };
+                };
+                auto sink2 = [&](auto with_abs) {
+                    return [&](int i, int r, int, int, cplx<T> v) {
+                        const double dv = (double)v.x, iv = (double)v.y;
+                        sd += (double)pd[i][r] * (dv * dv);
+                        sd += (double)pi[i][r] * (iv * iv);
+                        if constexpr (decltype(with_abs)::value) {
+                            sa += fabs((double)pd[i][r]) * (dv * dv);
+                            sa += fabs((double)pi[i][r]) * (iv * iv);
                         }
-                    }
-                    wave_sync();
-                    // the first stage reads all its inputs before it writes;
-                    // the last stage's outputs go to the positions its own
-                    // reads came from, after them (one wavefront's LDS
-                    // instructions complete in order)
-                    auto srcq = [&](int l, int e) -> cplx<T> { return wb[l * LP + Lay::pad(e)]; };
-                    auto sinkq = [&](int, int, int l, int k, cplx<T> v) { wb[l * LP + Lay::pad(k)] = v; };
-                    const RegTwConj<T, N, LW> ctw{twr};
-                    wave_fft<T, N, LW, -1, 0, false, RegTwConj<T, N, LW>>(wb, ctw, lane, srcq, sinkq);
-                    wave_sync();
-                }
-            } else if (pv) {
-                // sum_z P(a) D(a) + P(b) D(b) = Re sum_e F(e) conj(E(e)), E
-                // the staged leg's complex line spectrum D(a) + i D(b):
-                // E(k) = (x - w, y + z), E(N - k) = (x + w, z - y) for the
-                // raw pair (x, y, z, w) = (D(a)[k], D(b)[k]) of plane k
-                // (imaginary parts dropped at k = 0 and N / 2)
-                using Lay = WLay<T, N>;
-                double sd = 0.0;
-                float dd = 0.f;  // (magnitude bound) sum_k w_k |E|^2 terms: sum_z D^2 / N
-                constexpr int KS = (N / 2 + 64) / 64;  // planes k = lane + 64 s, k <= N / 2
-#pragma unroll 1
-                for (int s2 = 0; s2 < KS; ++s2) {
-                    const int k = lane + 64 * s2;
-                    if (k <= Kz) {
-                        const bool edge_k = (k == 0) || (2 * k == N);
-#pragma unroll 1
-                        for (int l = 0; l < LW; ++l) {
-                            const int line = w * LW + l;
-                            const f32x4 v = sb[k * SL + ((line + k) & (SL - 1))];
-                            const cplx<T> A = wb[l * LP + Lay::pad(k)];
-                            const cplx<T> B = edge_k ? mk<T>(0, 0) : wb[l * LP + Lay::pad(N - k)];
-                            const T vy = edge_k ? (T)0 : v.y, vw = edge_k ? (T)0 : v.w;
-                            const T t = v.x * (A.x + B.x) + vy * (A.y - B.y) + v.z * (A.y + B.y) + vw * (B.x - A.x);
-                            sd += (double)t;
-                            if constexpr (ABS) dd += (edge_k ? 1.f : 2.f) * (v.x * v.x + vy * vy + v.z * v.z + vw * vw);
-                        }
-                    }
+                    };
+                };
+                // the magnitude sums on every abs_stride-th x plane
+                const bool ab = ABS && ((int)(t0s / tiles_per_x) % abs_stride<N>() == 0);
+                if (ab) {
+                    auto s1 = sink1(std::true_type{});
+                    auto s2 = sink2(std::true_type{});
+                    if (G.suf_sq) run(s2);
+                    else run(s1);
+                } else {
+                    auto s1 = sink1(std::false_type{});
+                    auto s2 = sink2(std::false_type{});
+                    if (G.suf_sq) run(s2);
+                    else run(s1);
                 }
                 sd = wave_sum_dpp(sd);
-                // sum over this wavefront's rows and z of |P D| <= sqrt(sum P^2 sum D^2)
-                double sa = 0.0;
-                if constexpr (ABS) sa = sqrt(pp * (double)N * wave_sum_dpp((double)dd));
+                if (ab) sa = wave_sum_dpp(sa);
                 if (lane == 0) {
                     const int o = q - G.npre;
                     if constexpr (ABS) acca[w][o] += (float)sa;
-                    KSum a;
-                    a.s = acc[w][o][0];
-                    a.c = acc[w][o][1];
-                    a.add(sd);
-                    acc[w][o][0] = a.s;
-                    acc[w][o][1] = a.c;
-                }
-            } else {
-                // squared suffixes (power spectra): the suffix leg transformed
-                double sd = 0.0;
-                auto sink1 = [&](int i, int r, int, int, cplx<T> v) {
-                    sd += (double)pd[i][r] * (double)v.x;
-                    sd += (double)pi[i][r] * (double)v.y;
-                };
-                auto sink2 = [&](int i, int r, int, int, cplx<T> v) {
-                    const double dv = (double)v.x, iv = (double)v.y;
-                    sd += (double)pd[i][r] * (dv * dv);
-                    sd += (double)pi[i][r] * (iv * iv);
-                };
-                if (G.suf_sq) run(sink2);
-                else run(sink1);
-                sd = wave_sum_dpp(sd);
-                if (lane == 0) {
-                    const int o = q - G.npre;
                     KSum a;
                     a.s = acc[w][o][0];
                     a.c = acc[w][o][1];
@@ -1407,10 +1339,7 @@ k_zc_dma(const GroupDev *__restrict__ groups, int ngroups, const LegDev *__restr
             double sabs = 0.0;
             if constexpr (ABS)
                 for (int ww = 0; ww < NWV; ++ww) sabs += (double)acca[ww][o];
-            // the Cauchy-Schwarz bounds of every tile (Parseval form), or the
-            // sum of squares itself
-            if (!(G.npre >= 1 && !G.suf_sq)) sabs = fabs(a.s + a.c);
-            partials[2 * po + 1] = make_double2(sabs * f, 0.0);
+            partials[2 * po + 1] = make_double2(sabs * f * abs_stride<N>(), 0.0);
         }
         __syncthreads();
     }

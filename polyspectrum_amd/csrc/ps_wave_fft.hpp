@@ -184,16 +184,6 @@ template <typename T, int N, int LW> struct RegTw {
     }
 };
 
-// register twiddles loaded for one direction, used for the other (conjugates)
-template <typename T, int N, int LW> struct RegTwConj {
-    const WaveTw<T, N, LW> &w;
-    template <int S, int SIGN> __device__ __forceinline__ cplx<T> get(int i, int r, int) const
-    {
-        const cplx<T> t = w.w[WaveTw<T, N, LW>::template at<S>(i, r)];
-        return mk<T>(t.x, -t.y);
-    }
-};
-
 // Inverse (SIGN = +1) or forward (SIGN = -1) transform of LW lines.
 //   src(l, e)          -> input element e of line l (first stage)
 //   sink(i, r, l, z, v) <- output z of line l (held as v[i][r] of this thread)

@@ -149,11 +149,10 @@ def test_config_512_sweep_within_1e5_with_refinement(gpu_lib):
 
 
 def test_magnitude_sums_not_below_kappa_numerator(gpu_lib):
-    """The refinement's numerator of kappa (sum |prod D| itself on grids
-    below 96, its Cauchy-Schwarz bound in the copy-pipelined pass on the
-    sweep's larger grids) against the exact sum |prod D| of the FP64
-    reference-shaped path, on the benchmark sweep: never below it (up to the
-    mixed-precision sum in kappa's denominator)."""
+    """The refinement's numerator of kappa (sum |prod D| on every x plane of
+    grids up to 192) against the exact sum |prod D| of the FP64
+    reference-shaped path, on the benchmark sweep: equal up to the
+    mixed-precision sums (the measured ratio is 0.997-1.0)."""
     P = gpu_lib
     import bench
     N, L = 512, 300.0
@@ -175,4 +174,4 @@ def test_magnitude_sums_not_below_kappa_numerator(gpu_lib):
     ratio = kappa_used / kappa
     print(f"numerator used / exact: min {ratio.min():.3f} median {np.median(ratio):.3f} max {ratio.max():.3f}")
     assert np.all(ratio >= 0.99), (np.argmin(ratio), ratio.min())
-    assert np.median(ratio) >= 1.0
+    assert np.all(ratio <= 1.01), (np.argmax(ratio), ratio.max())
