@@ -33,6 +33,8 @@
 
 namespace ps {
 
+typedef float f32x4 __attribute__((ext_vector_type(4)));  // a pair of FP32 complex values, in registers
+
 // ---------------------------------------------------------------- helpers
 template <typename T> __device__ __forceinline__ T ld_real(const float *p, int64_t i) { return (T)p[i]; }
 template <typename T> __device__ __forceinline__ T ld_real(const double *p, int64_t i) { return (T)p[i]; }
@@ -260,11 +262,29 @@ k_inv_g1d(const LegDev *__restrict__ legs, int nlegs, int64_t t0, int64_t t1, co
     auto put = [&](const Tile &t) {
         const LegDev &L = legs[t.li];
         cplx<T> *gd = reinterpret_cast<cplx<T> *>(L.g1d);
-        tile_loop_u<TOT, 4>(tid, [&](int idx) {
-            const int l = idx / N, y = idx % N;
-            const int j = t.j0 + l;
-            if (j < L.Bx) gstore<T>(gd, ((int64_t)t.kz * L.Bx + j) * N + y, lds[lidx<LPD, T>(y, l)]);
-        });
+        if constexpr (sizeof(T) == 4) {
+            // FP32: two adjacent y of one line per 16-byte store
+            PS_GLOBAL f32x4 *g4 = (PS_GLOBAL f32x4 *)gd;
+            tile_loop_u<TOT / 2, 4>(tid, [&](int c) {
+                const int l = c / (N / 2), y = 2 * (c % (N / 2));
+                const int j = t.j0 + l;
+                if (j < L.Bx) {
+                    const cplx<T> a = lds[lidx<LPD, T>(y, l)], b = lds[lidx<LPD, T>(y + 1, l)];
+                    f32x4 v;
+                    v.x = a.x;
+                    v.y = a.y;
+                    v.z = b.x;
+                    v.w = b.y;
+                    g4[(((int64_t)t.kz * L.Bx + j) * N + y) / 2] = v;
+                }
+            });
+        } else {
+            tile_loop_u<TOT, 4>(tid, [&](int idx) {
+                const int l = idx / N, y = idx % N;
+                const int j = t.j0 + l;
+                if (j < L.Bx) gstore<T>(gd, ((int64_t)t.kz * L.Bx + j) * N + y, lds[lidx<LPD, T>(y, l)]);
+            });
+        }
     };
     if constexpr (PF) {
         int64_t tile = t0 + blockIdx.x;
@@ -827,7 +847,6 @@ __global__ void __launch_bounds__((ZcCfg<T, N, DM>::BT)) __attribute__((amdgpu_w
 // store per pair, and a line pitch of 16/SL (mod 16) 16-byte words: the
 // stores of a quarter wavefront (16/SL planes of all SL lines) and the reads
 // of the first stage (consecutive kz of one line) fill distinct banks.
-typedef float f32x4 __attribute__((ext_vector_type(4)));  // a pair of FP32 complex values, in registers
 // where it measured faster than k_inv_zc_fused on the 512^3 sweep (192: 260 ->
 // 241 us, 144: 96 -> 91 us, 96: 62 -> 55 us); the compact form of the latter
 // keeps four wavefronts per SIMD at 64, 128 and 256 (128: 59 against 76 us)
