@@ -123,8 +123,8 @@ int ps_set_method(ps_ctx *ctx, int method);
 /* Mixed-precision contexts: refine ill-conditioned values in FP64.  With tol
  * > 0 the context keeps a second, FP64 spectrum of every field loaded
  * afterwards (and of the field still in its upload buffer), the batches'
- * product passes also sum |prod D| or an upper bound of it (kappa =
- * sum |prod D| / |sum prod D|),
+ * product passes also sum |prod D| (kappa = sum |prod D| / |sum prod D|;
+ * every x plane up to grid 192, every fourth above, scaled by 4),
  * and ps_batch_execute evaluates every polygon whose error estimate
  * u32 (16 + 32 kappa / sqrt(#polygons)) (u32 = 2^-24) exceeds tol again
  * with the FP64 batched path (one FP64 batch per set of flagged polygons,
