@@ -175,3 +175,31 @@ def test_magnitude_sums_not_below_kappa_numerator(gpu_lib):
     print(f"numerator used / exact: min {ratio.min():.3f} median {np.median(ratio):.3f} max {ratio.max():.3f}")
     assert np.all(ratio >= 0.99), (np.argmin(ratio), ratio.min())
     assert np.all(ratio <= 1.01), (np.argmax(ratio), ratio.max())
+
+
+def test_refined_values_equal_across_schedules(gpu_lib):
+    """The refined step (magnitude sums, the FP64 batch beside it) gives the
+    same values bit for bit with its waves one after another and on two or
+    three branch streams, over repeated executions."""
+    P = gpu_lib
+    N, L = 128, 100.0
+    field = lognormal_field(N, 7005, sigma=1.0, alpha=1.5)
+    polys = _triangles(2 * math.pi / L, [(6.0, 6.0), (10.0, 10.0), (5.0, 9.0)], 16)
+    case = P.BINNING["GIVEN_BINNING"]
+    out = {}
+    with P.Context(N, L, P.PREC_MIXED) as ctx:
+        ctx.set_refinement(1e-6)
+        ctx.load_field(field)
+        for ov in (0, 2, 3):
+            b = ctx.batch(polys, 1.0, case, params={"grid_cap": 1024, "chunk_tiles": 1, "g2_div": 1, "overlap": ov,
+                                                    "method": P.METHOD_DIRECT})
+            vals = [b.execute()[0] for _ in range(3)]
+            flags, _ = b.refinement()
+            b.close()
+            for v in vals[1:]:
+                np.testing.assert_array_equal(v, vals[0])
+            out[ov] = (vals[0], flags)
+    assert out[0][1].any()  # some polygons are refined at this tolerance
+    for ov in (2, 3):
+        np.testing.assert_array_equal(out[ov][1], out[0][1])
+        np.testing.assert_array_equal(out[ov][0], out[0][0])
