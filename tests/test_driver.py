@@ -3,7 +3,8 @@
 CPU: the reference's error paths (main.c:53-58, parse_ini.h:16-22, kvectors.c:67-71,
 run.c:40-44, input_field.c:65-69) fire before any GPU use, with the same messages.
 GPU: the `.dat` files and stdout lines of run.c:94-222 for small grids, values
-checked against the oracle (FP64, 1e-6 after %e rounding).
+checked against the oracle (FP64): each row is the line the reference's
+format prints for the oracle's values, byte for byte.
 """
 import math
 import os
@@ -200,12 +201,14 @@ def test_driver_outputs(tmp_path, gpu_lib, kw):
     body = [list(map(float, l.split("\t"))) for l in lines[1:]]
     assert len(body) == len(rows)
     wnp = kw.get("wnp", 1)
-    for got, ref in zip(body, rows):
+    for line, got, ref in zip(lines[1:], body, rows):
         ref = list(ref) if wnp else list(ref)[:-1]
         if n == 2 or kw.get("eq", 0):
             ref = ref if n == 2 else [ref[1], ref[2]] + ([ref[3]] if wnp else [])
         assert len(got) == len(ref)
         np.testing.assert_allclose(got, ref, rtol=2e-6, atol=1e-300)
+        # the FP64 path prints the reference's rows (run.c:128-222, "%e" joined by tabs) byte for byte
+        assert line == "\t".join("%e" % x for x in ref), (line, ref)
     assert "kbinningCase = %d" % CASES[kw["case"]] in r.stdout
     assert "Execution took" in r.stdout
 
