@@ -1280,14 +1280,15 @@ k_zc_dma(const GroupDev *__restrict__ groups, int ngroups, const LegDev *__restr
                 if (G.pre_sq) run(sink2);
                 else run(sink1);
             } else {
-                double sd = 0.0, sa = 0.0;
+                double sd = 0.0;
+                T sa = 0;  // magnitude sums: an estimate, FP32 products and per-lane sums suffice
                 auto sink1 = [&](auto with_abs) {
                     return [&](int i, int r, int, int, cplx<T> v) {
                         sd += (double)pd[i][r] * (double)v.x;
                         sd += (double)pi[i][r] * (double)v.y;
                         if constexpr (decltype(with_abs)::value) {
-                            sa += fabs((double)pd[i][r]) * fabs((double)v.x);
-                            sa += fabs((double)pi[i][r]) * fabs((double)v.y);
+                            sa += fabs(pd[i][r]) * fabs(v.x);
+                            sa += fabs(pi[i][r]) * fabs(v.y);
                         }
                     };
                 };
@@ -1297,8 +1298,8 @@ k_zc_dma(const GroupDev *__restrict__ groups, int ngroups, const LegDev *__restr
                         sd += (double)pd[i][r] * (dv * dv);
                         sd += (double)pi[i][r] * (iv * iv);
                         if constexpr (decltype(with_abs)::value) {
-                            sa += fabs((double)pd[i][r]) * (dv * dv);
-                            sa += fabs((double)pi[i][r]) * (iv * iv);
+                            sa += fabs(pd[i][r]) * (v.x * v.x);
+                            sa += fabs(pi[i][r]) * (v.y * v.y);
                         }
                     };
                 };
@@ -1316,10 +1317,11 @@ k_zc_dma(const GroupDev *__restrict__ groups, int ngroups, const LegDev *__restr
                     else run(s1);
                 }
                 sd = wave_sum_dpp(sd);
-                if (ab) sa = wave_sum_dpp(sa);
+                double sat = 0.0;
+                if (ab) sat = wave_sum_dpp((double)sa);
                 if (lane == 0) {
                     const int o = q - G.npre;
-                    if constexpr (ABS) acca[w][o] += (float)sa;
+                    if constexpr (ABS) acca[w][o] += (float)sat;
                     KSum a;
                     a.s = acc[w][o][0];
                     a.c = acc[w][o][1];
