@@ -1819,7 +1819,13 @@ static int batch_results(ps_batch *b, double *values, double *npolygons, double 
     } else {
         std::vector<ps_polygon> q;
         for (int p : flagged) q.push_back(b->polys[p]);
-        PSCK(ps_batch_create(c->r64, q.data(), (int)q.size(), b->bw, b->kcase, &rb));
+        // its waves one after another on the FP64 context's stream: it runs
+        // beside the next execution's graph, whose branches hold the other
+        // streams (measured 0.978-0.985 ms per 512^3 step with two refined
+        // triangles, against 1.003-1.006 with its waves on branch streams)
+        ps_launch_params lp = default_params(c->r64);
+        lp.overlap = 0;
+        PSCK(ps_batch_create_ex(c->r64, q.data(), (int)q.size(), b->bw, b->kcase, &lp, &rb));
         if (b->ref_order.size() >= kRefineSets) {
             ps_batch_destroy(b->ref[b->ref_order.front()]);
             b->ref.erase(b->ref_order.front());
