@@ -1126,6 +1126,15 @@ __device__ __forceinline__ void wait_vm_at_most(int n)
     }
 }
 
+// the value of lane 0 (all lanes active), through two scalar registers
+__device__ __forceinline__ double lane0_f64(double v)
+{
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)u);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(u >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 // NBUF staging buffers: copies are requested NBUF - 1 steps ahead (3: two
 // steps; 2: one step), with the registers capped for four waves per SIMD
 // where the LDS allows four workgroups per CU (two buffers, or three at 144)
@@ -1147,6 +1156,7 @@ k_zc_dma(const GroupDev *__restrict__ groups, int ngroups, const LegDev *__restr
     static_assert(sizeof(T) == 4, "FP32 pairs of complex values are 16 bytes");
     static_assert(SL % 8 == 0 && (SL & (SL - 1)) == 0, "slot rotation: whole 128-byte segments per plane, SL a power of two");
     static_assert(MPT <= 7, "waits count at most 7 copy instructions (one step's)");
+    static_assert(MAX_SUFFIX <= 64, "one lane per suffix");
     __shared__ f32x4 stage[NBUF][SLOTS];
     __shared__ cplx<T> wbuf[NWV * LW * LP];
     __shared__ double acc[NWV][MAX_SUFFIX][2];
@@ -1186,9 +1196,11 @@ k_zc_dma(const GroupDev *__restrict__ groups, int ngroups, const LegDev *__restr
         const int64_t t_end = (t_begin + G.chunk_tiles < ntile) ? t_begin + G.chunk_tiles : ntile;
         const int nleg = G.npre + G.nsuf;
         const int64_t nsteps = (t_end - t_begin) * nleg;  // (tile, leg) steps of this item
-        for (int i = tid; i < NWV * MAX_SUFFIX * 2; i += BT) (&acc[0][0][0])[i] = 0.0;
-        if constexpr (ABS)
-            for (int i = tid; i < NWV * MAX_SUFFIX; i += BT) (&acca[0][0])[i] = 0.f;
+        // this wavefront's sums of suffix o in lane o's registers (the
+        // compensated sum of lane 0's wave totals, step by step)
+        KSum ks;
+        ks.init();
+        float ka = 0.f;
         auto leg_of = [&](int q) -> const PS_CONST LegDev & {
             return clegs[q < G.npre ? (q == 0 ? G.preA : G.preB) : csuffix[G.suf_begin + q - G.npre]];
         };
@@ -1316,24 +1328,23 @@ k_zc_dma(const GroupDev *__restrict__ groups, int ngroups, const LegDev *__restr
                     if (G.suf_sq) run(s2);
                     else run(s1);
                 }
-                sd = wave_sum_dpp(sd);
+                sd = lane0_f64(wave_sum_dpp(sd));
                 double sat = 0.0;
-                if (ab) sat = wave_sum_dpp((double)sa);
-                if (lane == 0) {
-                    const int o = q - G.npre;
-                    if constexpr (ABS) acca[w][o] += (float)sat;
-                    KSum a;
-                    a.s = acc[w][o][0];
-                    a.c = acc[w][o][1];
-                    a.add(sd);
-                    acc[w][o][0] = a.s;
-                    acc[w][o][1] = a.c;
+                if (ab) sat = lane0_f64(wave_sum_dpp((double)sa));
+                if (lane == q - G.npre) {
+                    if constexpr (ABS) ka += (float)sat;
+                    ks.add(sd);
                 }
             }
             t0s = t1s; q0 = q1;
             t1s = t2s; q1 = q2;
             next_of(t1s, q1, t2s, q2);
             n1 = n2;
+        }
+        if (lane < MAX_SUFFIX) {
+            acc[w][lane][0] = ks.s;
+            acc[w][lane][1] = ks.c;
+            if constexpr (ABS) acca[w][lane] = ka;
         }
         __syncthreads();
         for (int o = tid; o < G.nsuf; o += BT) {
